@@ -1,0 +1,881 @@
+// C ABI of the gfx950 linear-solve library: contexts, block-sparse matrices,
+// ILU(0)/identity preconditioners and restarted FGMRES.
+#include "cdg_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+// ---------------------------------------------------------------------------
+// errors and memory
+// ---------------------------------------------------------------------------
+int cdg_fail(const cdg_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int dev_alloc(cdg_ctx *ctx, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return CDG_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return cdg_fail(ctx, CDG_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    ctx->mem_used += (int64_t)bytes;
+    return CDG_OK;
+}
+
+void dev_free(cdg_ctx *ctx, void *p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    ctx->mem_used -= (int64_t)bytes;
+}
+
+int ctx_ensure_part(cdg_ctx *ctx, size_t n) {
+    if (n <= ctx->part_cap) return CDG_OK;
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dev_free(ctx, ctx->d_part, ctx->part_cap * sizeof(double));
+    ctx->d_part = nullptr;
+    ctx->part_cap = 0;
+    size_t cap = std::max<size_t>(n, 4096);
+    int e = dev_alloc(ctx, (void **)&ctx->d_part, cap * sizeof(double));
+    if (e) return e;
+    ctx->part_cap = cap;
+    return CDG_OK;
+}
+
+int ctx_ensure_small(cdg_ctx *ctx, size_t n) {
+    if (n <= ctx->small_cap) return CDG_OK;
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dev_free(ctx, ctx->d_small, ctx->small_cap * sizeof(double));
+    if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+    ctx->d_small = nullptr;
+    ctx->h_small = nullptr;
+    ctx->small_cap = 0;
+    size_t cap = std::max<size_t>(n, 256);
+    int e = dev_alloc(ctx, (void **)&ctx->d_small, cap * sizeof(double));
+    if (e) return e;
+    if (hipHostMalloc((void **)&ctx->h_small, cap * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return cdg_fail(ctx, CDG_ERR_OOM, "hipHostMalloc failed");
+    ctx->small_cap = cap;
+    return CDG_OK;
+}
+
+namespace {
+
+std::atomic<uint64_t> g_stamp{0};
+
+bool is_device_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+// Device view of a vector argument: device pointers pass through, host
+// pointers go through a per-slot staging buffer.
+struct VecArg {
+    double *dev = nullptr;
+    double *host = nullptr;
+    int64_t len = 0;
+};
+
+int stage_in(cdg_ctx *ctx, int slot, const double *p, int64_t len, VecArg &a) {
+    a.len = len;
+    if (is_device_ptr(p)) { a.dev = const_cast<double *>(p); return CDG_OK; }
+    a.host = const_cast<double *>(p);
+    size_t need = (size_t)std::max<int64_t>(len, 1);
+    if (ctx->stage_cap[slot] < need) {
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        dev_free(ctx, ctx->d_stage[slot], ctx->stage_cap[slot] * sizeof(double));
+        ctx->d_stage[slot] = nullptr;
+        ctx->stage_cap[slot] = 0;
+        int e = dev_alloc(ctx, (void **)&ctx->d_stage[slot], need * sizeof(double));
+        if (e) return e;
+        ctx->stage_cap[slot] = need;
+    }
+    a.dev = ctx->d_stage[slot];
+    return CDG_OK;
+}
+
+int upload_in(cdg_ctx *ctx, const VecArg &a) {
+    if (a.host && a.len) CDG_HIP_TRY(hipMemcpyAsync(a.dev, a.host, sizeof(double) * a.len, hipMemcpyHostToDevice, ctx->stream));
+    return CDG_OK;
+}
+
+int finish_out(cdg_ctx *ctx, const VecArg &a) {
+    if (a.host && a.len) CDG_HIP_TRY(hipMemcpyAsync(a.host, a.dev, sizeof(double) * a.len, hipMemcpyDeviceToHost, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CDG_OK;
+}
+
+int64_t slot_elems(int n) { return ((int64_t)n * n + 63) / 64 * 64; }
+
+// Allocates and fills a device array.  Waits for the copy: callers pass
+// temporaries, and an asynchronous copy from pageable memory may read its
+// source after the call returns.
+template <typename T>
+int upload_array(cdg_ctx *ctx, T **d, const std::vector<T> &h, std::vector<DevMem> *owned) {
+    size_t bytes = std::max<size_t>(1, h.size()) * sizeof(T);
+    int e = dev_alloc(ctx, (void **)d, bytes);
+    if (e) return e;
+    if (owned) owned->push_back({*d, bytes});
+    if (!h.empty()) {
+        CDG_HIP_TRY(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return CDG_OK;
+}
+
+// kind-0 structure, transposes and wavefront levels of a pattern
+void analyze(cdg_mat *A) {
+    const int nr = A->nrows;
+    A->diag.assign(nr, -1);
+    A->lo_ptr.assign(nr + 1, 0);
+    A->up_ptr.assign(nr + 1, 0);
+    A->lo_blk.clear(); A->lo_col.clear(); A->up_blk.clear(); A->up_col.clear();
+    A->struct_err = CDG_OK;
+    for (int i = 0; i < nr; ++i) {
+        for (int b = A->row_ptr[i]; b < A->row_ptr[i + 1]; ++b) {
+            if (A->kind[b] != CDG_KIND_LOCAL) continue;
+            const int j = A->col[b];
+            if (j == i) { if (A->diag[i] < 0) A->diag[i] = b; }
+            else if (j < i) { A->lo_blk.push_back(b); A->lo_col.push_back(j); }
+            else { A->up_blk.push_back(b); A->up_col.push_back(j); }
+        }
+        A->lo_ptr[i + 1] = (int32_t)A->lo_blk.size();
+        A->up_ptr[i + 1] = (int32_t)A->up_blk.size();
+        if (A->diag[i] < 0) A->struct_err = CDG_ERR_ARG;
+    }
+    auto find_t = [&](int row, int parent) {
+        for (int b = A->row_ptr[row]; b < A->row_ptr[row + 1]; ++b)
+            if (A->kind[b] == CDG_KIND_LOCAL && A->col[b] == parent) return b;
+        return -1;
+    };
+    A->lo_tr.resize(A->lo_blk.size());
+    A->up_tr.resize(A->up_blk.size());
+    for (int i = 0; i < nr; ++i) {
+        for (int q = A->lo_ptr[i]; q < A->lo_ptr[i + 1]; ++q) {
+            A->lo_tr[q] = find_t(A->lo_col[q], i);
+            if (A->lo_tr[q] < 0 && A->struct_err == CDG_OK) A->struct_err = CDG_ERR_NO_TRANSPOSE;
+        }
+        for (int u = A->up_ptr[i]; u < A->up_ptr[i + 1]; ++u) {
+            A->up_tr[u] = find_t(A->up_col[u], i);
+            if (A->up_tr[u] < 0 && A->struct_err == CDG_OK) A->struct_err = CDG_ERR_NO_TRANSPOSE;
+        }
+    }
+    A->flevel.assign(nr, 0);
+    A->blevel.assign(nr, 0);
+    A->nflev = nr ? 1 : 0;
+    A->nblev = nr ? 1 : 0;
+    for (int i = 0; i < nr; ++i) {
+        int l = 0;
+        for (int q = A->lo_ptr[i]; q < A->lo_ptr[i + 1]; ++q) l = std::max(l, A->flevel[A->lo_col[q]] + 1);
+        A->flevel[i] = l;
+        A->nflev = std::max(A->nflev, l + 1);
+    }
+    for (int i = nr - 1; i >= 0; --i) {
+        int l = 0;
+        for (int u = A->up_ptr[i]; u < A->up_ptr[i + 1]; ++u) l = std::max(l, A->blevel[A->up_col[u]] + 1);
+        A->blevel[i] = l;
+        A->nblev = std::max(A->nblev, l + 1);
+    }
+    A->max_row_blocks = 0;
+    for (int i = 0; i < nr; ++i) A->max_row_blocks = std::max(A->max_row_blocks, A->row_ptr[i + 1] - A->row_ptr[i]);
+}
+
+std::vector<int32_t> order_by_level(const std::vector<int32_t> &lev, int nlev, std::vector<int32_t> *ptr_out) {
+    std::vector<int32_t> ptr(nlev + 1, 0), rows(lev.size());
+    for (int32_t l : lev) ptr[l + 1]++;
+    for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t i = 0; i < lev.size(); ++i) rows[fill[lev[i]]++] = (int32_t)i;
+    if (ptr_out) *ptr_out = ptr;
+    return rows;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int cdg_ctx_create(int32_t device, cdg_ctx **out) {
+    if (!out) return CDG_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { (void)hipGetLastError(); return CDG_ERR_HIP; }
+    if (device < 0 || device >= ndev) return CDG_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CDG_ERR_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return CDG_ERR_HIP;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return CDG_ERR_UNSUPPORTED;
+    cdg_ctx *ctx = new cdg_ctx();
+    ctx->device = device;
+    ctx->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return CDG_ERR_HIP; }
+    if (ctx_ensure_small(ctx, 4096) || ctx_ensure_part(ctx, 1 << 16)) { cdg_ctx_destroy(ctx); return CDG_ERR_OOM; }
+    *out = ctx;
+    return CDG_OK;
+}
+
+extern "C" int cdg_ctx_destroy(cdg_ctx *ctx) {
+    if (!ctx) return CDG_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, ctx->d_part, ctx->part_cap * sizeof(double));
+    dev_free(ctx, ctx->d_small, ctx->small_cap * sizeof(double));
+    if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+    for (int s = 0; s < 3; ++s) dev_free(ctx, ctx->d_stage[s], ctx->stage_cap[s] * sizeof(double));
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CDG_OK;
+}
+
+extern "C" const char *cdg_last_error(const cdg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int cdg_ctx_mem_used(const cdg_ctx *ctx, int64_t *bytes) {
+    if (!ctx || !bytes) return CDG_ERR_ARG;
+    *bytes = ctx->mem_used;
+    return CDG_OK;
+}
+
+extern "C" int cdg_ctx_stream(const cdg_ctx *ctx, void **stream) {
+    if (!ctx || !stream) return CDG_ERR_ARG;
+    *stream = static_cast<void *>(ctx->stream);
+    return CDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// matrix
+// ---------------------------------------------------------------------------
+extern "C" int cdg_mat_create(cdg_ctx *ctx, int32_t nrows, int32_t block_n, const int32_t *row_ptr,
+                              const int32_t *col, const int32_t *kind, cdg_mat **out) {
+    if (!ctx || !out) return CDG_ERR_ARG;
+    *out = nullptr;
+    if (nrows < 0 || block_n <= 0 || block_n > 1024 || !row_ptr || (nrows > 0 && (!col || !kind)))
+        return cdg_fail(ctx, CDG_ERR_ARG, "cdg_mat_create: bad sizes or null arrays");
+    if (row_ptr[0] != 0) return cdg_fail(ctx, CDG_ERR_ARG, "row_ptr[0] must be 0");
+    for (int i = 0; i < nrows; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) return cdg_fail(ctx, CDG_ERR_ARG, "row_ptr not monotone at %d", i);
+    const int nnzb = row_ptr[nrows];
+    for (int b = 0; b < nnzb; ++b) {
+        if (col[b] < 0 || col[b] >= nrows) return cdg_fail(ctx, CDG_ERR_ARG, "col[%d]=%d out of range", b, col[b]);
+        if (kind[b] != CDG_KIND_LOCAL && kind[b] != CDG_KIND_COUPLING)
+            return cdg_fail(ctx, CDG_ERR_ARG, "kind[%d]=%d invalid", b, kind[b]);
+    }
+    CDG_HIP_TRY(hipSetDevice(ctx->device));
+    cdg_mat *A = new cdg_mat();
+    A->nrows = nrows;
+    A->n = block_n;
+    A->nnzb = nnzb;
+    A->ld = slot_elems(block_n);
+    A->row_ptr.assign(row_ptr, row_ptr + nrows + 1);
+    A->col.assign(col, col + nnzb);
+    A->kind.assign(kind, kind + nnzb);
+    analyze(A);
+    A->shape64 = stream_shape(block_n, 2);
+    int e = dev_alloc(ctx, (void **)&A->blocks, sizeof(double) * (size_t)std::max(1, nnzb) * A->ld);
+    if (!e) e = upload_array(ctx, &A->d_row_ptr, A->row_ptr, nullptr);
+    if (!e) e = upload_array(ctx, &A->d_col, A->col, nullptr);
+    if (e) { cdg_mat_destroy(ctx, A); return e; }
+    hipError_t he = hipMemsetAsync(A->blocks, 0, sizeof(double) * (size_t)std::max(1, nnzb) * A->ld, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) { cdg_mat_destroy(ctx, A); return cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he)); }
+    *out = A;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_destroy(cdg_ctx *ctx, cdg_mat *A) {
+    if (!A) return CDG_OK;
+    if (ctx) {
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx, A->blocks, sizeof(double) * (size_t)std::max(1, A->nnzb) * A->ld);
+        dev_free(ctx, A->d_row_ptr, sizeof(int32_t) * (A->row_ptr.size()));
+        dev_free(ctx, A->d_col, sizeof(int32_t) * std::max<size_t>(1, A->col.size()));
+    }
+    delete A;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_upload(cdg_ctx *ctx, cdg_mat *A, const double *blocks) {
+    if (!ctx || !A || (!blocks && A->nnzb)) return CDG_ERR_ARG;
+    if (A->nnzb == 0) { A->stamp = ++g_stamp; return CDG_OK; }
+    const size_t nn = (size_t)A->n * A->n;
+    const hipMemcpyKind kind = is_device_ptr(blocks) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    CDG_HIP_TRY(hipMemcpy2DAsync(A->blocks, sizeof(double) * A->ld, blocks, sizeof(double) * nn, sizeof(double) * nn,
+                                 A->nnzb, kind, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    A->stamp = ++g_stamp;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_download(cdg_ctx *ctx, const cdg_mat *A, double *blocks) {
+    if (!ctx || !A || (!blocks && A->nnzb)) return CDG_ERR_ARG;
+    if (A->nnzb == 0) return CDG_OK;
+    const size_t nn = (size_t)A->n * A->n;
+    const hipMemcpyKind kind = is_device_ptr(blocks) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    CDG_HIP_TRY(hipMemcpy2DAsync(blocks, sizeof(double) * nn, A->blocks, sizeof(double) * A->ld, sizeof(double) * nn,
+                                 A->nnzb, kind, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_diag, double s_off) {
+    if (!ctx || !A) return CDG_ERR_ARG;
+    int e = launch_fill_synthetic(ctx, A, seed, s_diag, s_off);
+    if (e) return e;
+    A->stamp = ++g_stamp;
+    return CDG_OK;
+}
+
+extern "C" int cdg_vec_fill_synthetic(cdg_ctx *ctx, const cdg_mat *A, uint64_t seed, double scale, double *v) {
+    if (!ctx || !A || !v) return CDG_ERR_ARG;
+    VecArg o;
+    const int64_t len = (int64_t)A->nrows * A->n;
+    int e = stage_in(ctx, 2, v, len, o);
+    if (!e) e = launch_vec_synthetic(ctx, len, seed, scale, o.dev);
+    if (!e) e = finish_out(ctx, o);
+    return e;
+}
+
+extern "C" int cdg_mat_stamp(const cdg_mat *A, uint64_t *stamp) {
+    if (!A || !stamp) return CDG_ERR_ARG;
+    *stamp = A->stamp;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_info(const cdg_mat *A, int32_t *nrows, int32_t *block_n, int32_t *nnzb) {
+    if (!A) return CDG_ERR_ARG;
+    if (nrows) *nrows = A->nrows;
+    if (block_n) *block_n = A->n;
+    if (nnzb) *nnzb = A->nnzb;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y) {
+    if (!ctx || !A || !x || !y) return CDG_ERR_ARG;
+    const int64_t len = (int64_t)A->nrows * A->n;
+    VecArg xi, yo;
+    int e = stage_in(ctx, 0, x, len, xi);
+    if (!e) e = stage_in(ctx, 1, y, len, yo);
+    if (!e) e = upload_in(ctx, xi);
+    if (!e) e = launch_mv(ctx, A, xi.dev, yo.dev);
+    if (!e) e = finish_out(ctx, yo);
+    return e;
+}
+
+extern "C" int cdg_mv_async(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y) {
+    if (!ctx || !A || !x || !y) return CDG_ERR_ARG;
+    if (!is_device_ptr(x) || !is_device_ptr(y)) return cdg_fail(ctx, CDG_ERR_ARG, "cdg_mv_async needs device pointers");
+    return launch_mv(ctx, A, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// preconditioner
+// ---------------------------------------------------------------------------
+extern "C" int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M) {
+    if (!M) return CDG_OK;
+    if (ctx) {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (DevMem &m : M->owned) dev_free(ctx, m.p, m.bytes);
+    }
+    delete M;
+    return CDG_OK;
+}
+
+extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int32_t precision, cdg_prec **out) {
+    if (!ctx || !A || !out) return CDG_ERR_ARG;
+    *out = nullptr;
+    if (type != CDG_PREC_IDENTITY && type != CDG_PREC_ILU0) return cdg_fail(ctx, CDG_ERR_ARG, "unknown preconditioner %d", type);
+    if (precision < CDG_FP64 || precision > CDG_BF16) return cdg_fail(ctx, CDG_ERR_ARG, "unknown precision %d", precision);
+    cdg_prec *M = new cdg_prec();
+    M->type = type;
+    M->precision = precision;
+    M->nrows = A->nrows;
+    M->n = A->n;
+    if (type == CDG_PREC_IDENTITY) { *out = M; return CDG_OK; }
+    if (A->struct_err != CDG_OK) {
+        delete M;
+        return cdg_fail(ctx, A->struct_err, A->struct_err == CDG_ERR_NO_TRANSPOSE
+                                                ? "lblks entry without a transposed partner"
+                                                : "block row without a diagonal block");
+    }
+    M->nlo = (int32_t)A->lo_blk.size();
+    M->nup = (int32_t)A->up_blk.size();
+    M->ld = slot_elems(A->n);
+    const size_t es = precision == CDG_FP64 ? 8 : (precision == CDG_FP32 ? 4 : 2);
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        bytes = std::max<size_t>(bytes, 16);
+        int e = dev_alloc(ctx, p, bytes);
+        if (!e) M->owned.push_back({*p, bytes});
+        return e;
+    };
+    int e = alloc(&M->L, es * (size_t)M->nlo * M->ld);
+    if (!e) e = alloc(&M->Dinv, es * (size_t)M->nrows * M->ld);
+    if (!e && precision != CDG_FP64) e = alloc(&M->U, es * (size_t)M->nup * M->ld);
+    if (!e && precision == CDG_BF16) {
+        e = alloc((void **)&M->L32, 4 * (size_t)M->nlo * M->ld);
+        if (!e) e = alloc((void **)&M->D32, 4 * (size_t)M->nrows * M->ld);
+    }
+    // structure arrays
+    std::vector<int32_t> up_idx(M->nup);
+    for (int u = 0; u < M->nup; ++u) up_idx[u] = precision == CDG_FP64 ? A->up_blk[u] : u;
+    if (!e) e = upload_array(ctx, &M->d_lo_ptr, A->lo_ptr, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lo_blk, A->lo_blk, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lo_col, A->lo_col, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lo_tr, A->lo_tr, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_ptr, A->up_ptr, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_blk, A->up_blk, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_idx, up_idx, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_col, A->up_col, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_diag, A->diag, &M->owned);
+    // sweep tasks: forward rows by forward level, backward rows by backward level
+    std::vector<int32_t> fptr;
+    std::vector<int32_t> forder = order_by_level(A->flevel, A->nflev, &fptr);
+    std::vector<int32_t> border = order_by_level(A->blevel, A->nblev, nullptr);
+    std::vector<int32_t> tasks(forder);
+    tasks.insert(tasks.end(), border.begin(), border.end());
+    if (!e) e = upload_array(ctx, &M->d_tasks, tasks, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_flags, std::vector<int32_t>(2 * (size_t)M->nrows, 0), &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_counters, std::vector<int32_t>(4, 0), &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_err, std::vector<int32_t>(1, 0), &M->owned);
+    // factor schedule: lower blocks and rows grouped by forward level
+    M->nlev = A->nflev;
+    M->lev_row_ptr = fptr;
+    std::vector<int32_t> lev_lo;
+    M->lev_lo_ptr.assign(M->nlev + 1, 0);
+    for (int l = 0; l < M->nlev; ++l) {
+        for (int k = fptr[l]; k < fptr[l + 1]; ++k) {
+            const int i = forder[k];
+            for (int q = A->lo_ptr[i]; q < A->lo_ptr[i + 1]; ++q) lev_lo.push_back(q);
+        }
+        M->lev_lo_ptr[l + 1] = (int32_t)lev_lo.size();
+    }
+    if (!e) e = upload_array(ctx, &M->d_lev_lo, lev_lo, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lev_rows, forder, &M->owned);
+    if (!e) {
+        hipError_t he = hipStreamSynchronize(ctx->stream);
+        if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
+    }
+    if (e) { cdg_prec_destroy(ctx, M); return e; }
+    const int V = precision == CDG_FP64 ? 2 : (precision == CDG_FP32 ? 4 : 8);
+    M->shape = stream_shape(A->n, V);
+    M->maxdep = 1;
+    for (int i = 0; i < A->nrows; ++i) {
+        M->maxdep = std::max(M->maxdep, A->lo_ptr[i + 1] - A->lo_ptr[i]);
+        M->maxdep = std::max(M->maxdep, A->up_ptr[i + 1] - A->up_ptr[i]);
+    }
+    const int per_cu = M->shape.T > 512 ? 2 : 4;
+    M->sweep_grid = std::max(1, std::min(2 * M->nrows, ctx->num_cus * per_cu));
+    *out = M;
+    return CDG_OK;
+}
+
+extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row) {
+    if (!ctx || !M || !A) return CDG_ERR_ARG;
+    if (bad_row) *bad_row = -1;
+    if (A->nrows != M->nrows || A->n != M->n) return cdg_fail(ctx, CDG_ERR_ARG, "matrix does not match preconditioner");
+    if (M->type == CDG_PREC_IDENTITY) { M->stamp = A->stamp; M->factored = 1; return CDG_OK; }
+    M->factored = 0;
+    int e = launch_ilu_factor(ctx, M, A, bad_row);
+    if (e) return e;
+    M->stamp = A->stamp;
+    M->factored = 1;
+    return CDG_OK;
+}
+
+static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
+    const int64_t len = (int64_t)M->nrows * M->n;
+    if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len, v, z);
+    if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
+    return launch_ilu_apply(ctx, M, A, v, z);
+}
+
+// Reads and clears the sweep's sticky status (bit 0: a dependency wait timed
+// out, bit 1: corrupt schedule).
+static int check_sweep(cdg_ctx *ctx, cdg_prec *M) {
+    if (M->type != CDG_PREC_ILU0) return CDG_OK;
+    int32_t flag = 0;
+    static const int32_t zero = 0;
+    CDG_HIP_TRY(hipMemcpyAsync(&flag, M->d_counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (flag) {
+        CDG_HIP_TRY(hipMemcpy(M->d_counters + 3, &zero, sizeof(int32_t), hipMemcpyHostToDevice));
+        return cdg_fail(ctx, CDG_ERR_HIP, "ILU sweep aborted (status %d)", flag);
+    }
+    return CDG_OK;
+}
+
+extern "C" int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
+    if (!ctx || !M || !A || !v || !z) return CDG_ERR_ARG;
+    const int64_t len = (int64_t)M->nrows * M->n;
+    VecArg vi, zo;
+    int e = stage_in(ctx, 0, v, len, vi);
+    if (!e) e = stage_in(ctx, 1, z, len, zo);
+    if (!e) e = upload_in(ctx, vi);
+    if (!e) e = prec_apply_dev(ctx, M, A, vi.dev, zo.dev);
+    if (!e) e = finish_out(ctx, zo);
+    if (!e) e = check_sweep(ctx, M);
+    return e;
+}
+
+extern "C" int cdg_prec_apply_async(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
+    if (!ctx || !M || !A || !v || !z) return CDG_ERR_ARG;
+    if (!is_device_ptr(v) || !is_device_ptr(z)) return cdg_fail(ctx, CDG_ERR_ARG, "cdg_prec_apply_async needs device pointers");
+    return prec_apply_dev(ctx, M, A, v, z);
+}
+
+extern "C" int cdg_prec_download(cdg_ctx *ctx, const cdg_prec *M, double *L, double *Dinv) {
+    if (!ctx || !M) return CDG_ERR_ARG;
+    if (M->type != CDG_PREC_ILU0 || !M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "no ILU0 factor to download");
+    const size_t nn = (size_t)M->n * M->n;
+    double *tmp = nullptr;
+    const int64_t cnt = (int64_t)std::max(M->nlo, M->nrows) * M->ld;
+    int e = dev_alloc(ctx, (void **)&tmp, sizeof(double) * std::max<int64_t>(cnt, 1));
+    if (e) return e;
+    std::vector<double> h((size_t)std::max<int64_t>(cnt, 1));
+    const size_t es = M->precision == CDG_FP64 ? 8 : (M->precision == CDG_FP32 ? 4 : 2);
+    auto fetch = [&](const void *src, int nslots, auto put) -> int {
+        if (nslots == 0) return CDG_OK;
+        int ee = launch_widen(ctx, src, M->precision, tmp, (int64_t)nslots * M->ld);
+        if (ee) return ee;
+        CDG_HIP_TRY(hipMemcpyAsync(h.data(), tmp, sizeof(double) * (size_t)nslots * M->ld, hipMemcpyDeviceToHost, ctx->stream));
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int s = 0; s < nslots; ++s) put(s, h.data() + (size_t)s * M->ld);
+        return CDG_OK;
+    };
+    (void)es;
+    // slot q of L belongs to the q-th lower block in pattern order
+    std::vector<int32_t> lo_blk_host;
+    // The prec does not keep the pattern; recover block indices from the device copy.
+    if (M->nlo) {
+        lo_blk_host.resize(M->nlo);
+        hipError_t he = hipMemcpy(lo_blk_host.data(), M->d_lo_blk, sizeof(int32_t) * M->nlo, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) { dev_free(ctx, tmp, sizeof(double) * cnt); return cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he)); }
+    }
+    if (L) {
+        e = fetch(M->L, M->nlo, [&](int q, const double *src) {
+            std::memcpy(L + (size_t)lo_blk_host[q] * nn, src, sizeof(double) * nn);
+        });
+    }
+    if (!e && Dinv) {
+        e = fetch(M->Dinv, M->nrows, [&](int i, const double *src) {
+            std::memcpy(Dinv + (size_t)i * nn, src, sizeof(double) * nn);
+        });
+    }
+    dev_free(ctx, tmp, sizeof(double) * std::max<int64_t>(cnt, 1));
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// FGMRES (type_fgmres.f90:81-310)
+// ---------------------------------------------------------------------------
+extern "C" void cdg_fgmres_default_params(cdg_fgmres_params *p) {
+    if (!p) return;
+    p->nkrylov = 2000;
+    p->tol = 1.e-8;
+    p->rtol = 1.e-10;
+    p->nmax = -1;
+    p->orthogonalization = CDG_CGS;
+    p->inner_fgmres = 1;
+    p->inner_nkrylov = 100;
+    p->inner_tol = 1.e-1;
+    p->inner_rtol = 1.e-1;
+    p->inner_nmax = 100;
+    p->inner_orthogonalization = CDG_CGS;
+    p->update_preconditioner = 1;
+}
+
+namespace {
+
+struct Workspace {
+    int m = 0;
+    int64_t len = 0;
+    double *V = nullptr, *Z = nullptr, *w = nullptr, *r0 = nullptr, *x0 = nullptr, *zr = nullptr, *dz = nullptr;
+    size_t bytes = 0;
+    void *base = nullptr;
+};
+
+int ws_alloc(cdg_ctx *ctx, Workspace &ws, int m, int64_t len, bool inner_bufs) {
+    ws.m = m;
+    ws.len = len;
+    const size_t lv = (size_t)std::max<int64_t>(len, 1);
+    const size_t nvec = 2 * (size_t)(m + 1) + 3 + (inner_bufs ? 2 : 0);
+    ws.bytes = nvec * lv * sizeof(double);
+    int e = dev_alloc(ctx, &ws.base, ws.bytes);
+    if (e) return e;
+    double *p = static_cast<double *>(ws.base);
+    ws.V = p; p += (size_t)(m + 1) * lv;
+    ws.Z = p; p += (size_t)(m + 1) * lv;
+    ws.w = p; p += lv;
+    ws.r0 = p; p += lv;
+    ws.x0 = p; p += lv;
+    if (inner_bufs) { ws.zr = p; p += lv; ws.dz = p; p += lv; }
+    return CDG_OK;
+}
+
+void ws_free(cdg_ctx *ctx, Workspace &ws) {
+    dev_free(ctx, ws.base, ws.bytes);
+    ws = Workspace();
+}
+
+struct Timers {
+    double precon = 0, mv = 0, dot = 0, norm = 0;
+};
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+struct Solver {
+    cdg_ctx *ctx;
+    const cdg_mat *A;
+    cdg_prec *M;
+    int64_t len;
+    Timers t;
+    int nreorth = 0, nrestart = 0;
+    int status = CDG_OK;
+};
+
+// fetch n device scalars from ctx->d_small into ctx->h_small
+int fetch_small(cdg_ctx *ctx, size_t n) {
+    CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CDG_OK;
+}
+
+// One orthogonalization pass of w against V_0..V_{j}: htmp -> host hcol (+=),
+// returns ||w||^2 after the pass in *nrm2.  d_small layout: [0..j] htmp, [j+1] nrm2.
+int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *hcol, double *nrm2) {
+    cdg_ctx *ctx = S.ctx;
+    const int64_t len = S.len;
+    int e = ctx_ensure_small(ctx, (size_t)jj + 8 + ws.m + 8);
+    if (e) return e;
+    double *d_h = ctx->d_small + 8 + ws.m + 8;   // beyond the combine scratch
+    if (orth == CDG_MGS) {
+        for (int i = 0; i < jj && !e; ++i) {
+            e = kv_multidot(ctx, len, ws.w, ws.V + (size_t)i * len, len, 1, d_h + i);
+            if (!e) e = kv_axpy_dev(ctx, len, ws.w, ws.V + (size_t)i * len, d_h + i);
+        }
+        if (!e) e = kv_sumsq(ctx, len, ws.w, d_h + jj);
+    } else {
+        e = kv_multidot(ctx, len, ws.w, ws.V, len, jj, d_h);
+        if (!e) e = kv_cgs_update(ctx, len, ws.w, ws.V, len, jj, d_h, d_h + jj);
+    }
+    if (e) return e;
+    CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_h, sizeof(double) * (jj + 1), hipMemcpyDeviceToHost, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < jj; ++i) hcol[i] += ctx->h_small[i];
+    *nrm2 = ctx->h_small[jj];
+    return CDG_OK;
+}
+
+int fgmres_core(Solver &S, std::vector<Workspace> &wss, int depth, const double *b, double *x, int m, double tol,
+                double rtol, int nmax, int orth, const cdg_fgmres_params *inner, int *niter_out) {
+    cdg_ctx *ctx = S.ctx;
+    const int64_t len = S.len;
+    if (m < 1) return cdg_fail(ctx, CDG_ERR_ARG, "nkrylov must be >= 1");
+    if (depth >= (int)wss.size()) return cdg_fail(ctx, CDG_ERR_ARG, "FGMRES nesting deeper than one inner level");
+    Workspace &ws = wss[depth];   // wss is never resized during a solve
+    const bool need_inner = inner != nullptr;
+    if (ws.m != m || ws.len != len || (need_inner && !ws.zr)) {
+        ws_free(ctx, ws);
+        int e = ws_alloc(ctx, ws, m, len, need_inner);
+        if (e) return e;
+    }
+    std::vector<double> h((size_t)(m + 1) * m), c(m + 1), s(m + 1), p(m + 1), y(m + 1);
+    auto H = [&](int i, int j) -> double & { return h[(size_t)j * (m + 1) + i]; };
+    int e = kv_zero(ctx, len, ws.x0);
+    if (!e) e = kv_zero(ctx, len, x);
+    if (e) return e;
+    int niter = 0;
+    double res = std::numeric_limits<double>::max(), r0norm_initial = 0.0;
+    bool conv = false;
+    while (res > tol) {
+        std::fill(h.begin(), h.end(), 0.0);
+        std::fill(p.begin(), p.end(), 0.0);
+        std::fill(c.begin(), c.end(), 0.0);
+        std::fill(s.begin(), s.end(), 0.0);
+        // r0 = b - A x0 ; v1 = r0/||r0|| ; p1 = ||r0||   (:160-163)
+        auto t0 = clk::now();
+        e = launch_mv(ctx, S.A, ws.x0, ws.r0);
+        if (!e) e = kv_sub(ctx, len, b, ws.r0, ws.r0);
+        if (!e) e = kv_sumsq(ctx, len, ws.r0, ctx->d_small);
+        if (!e) e = fetch_small(ctx, 1);
+        if (e) return e;
+        const double r0norm = std::sqrt(ctx->h_small[0]);
+        S.t.norm += secs(t0, clk::now());
+        if (!std::isfinite(r0norm)) return cdg_fail(ctx, CDG_ERR_NAN, "non-finite residual norm");
+        if (niter == 0) r0norm_initial = r0norm;
+        if (r0norm == 0.0) {
+            // exact x0: the reference divides by zero here (type_fgmres.f90:162); return x = x0
+            e = kv_copy(ctx, len, ws.x0, x);
+            if (e) return e;
+            res = 0.0;
+            break;
+        }
+        e = kv_div(ctx, len, ws.r0, r0norm, ws.V);
+        if (e) return e;
+        p[0] = r0norm;
+        int nvecs = 0;
+        for (int j = 0; j < m; ++j) {
+            nvecs++;
+            double *vj = ws.V + (size_t)j * len, *zj = ws.Z + (size_t)j * len;
+            auto ta = clk::now();
+            e = prec_apply_dev(ctx, S.M, S.A, vj, zj);
+            if (e) return e;
+            if (need_inner) {
+                // zr = v_j - A z_j ; inner solve ; z_j += dz   (:179-183)
+                e = launch_mv(ctx, S.A, zj, ws.zr);
+                if (!e) e = kv_sub(ctx, len, vj, ws.zr, ws.zr);
+                int inner_it = 0;
+                if (!e) e = fgmres_core(S, wss, depth + 1, ws.zr, ws.dz, inner->inner_nkrylov, inner->inner_tol,
+                                        inner->inner_rtol, inner->inner_nmax, inner->inner_orthogonalization, nullptr,
+                                        &inner_it);
+                if (e) return e;
+                e = kv_add(ctx, len, zj, ws.dz);
+                if (e) return e;
+            }
+            Workspace &w2 = ws;
+            CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+            auto tb = clk::now();
+            S.t.precon += secs(ta, tb);
+            // w = A z_j ; nb = ||w||   (:189-191)
+            e = launch_mv(ctx, S.A, w2.Z + (size_t)j * len, w2.w);
+            if (e) return e;
+            CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+            auto tc = clk::now();
+            S.t.mv += secs(tb, tc);
+            e = kv_sumsq(ctx, len, w2.w, ctx->d_small);
+            if (!e) e = fetch_small(ctx, 1);
+            if (e) return e;
+            const double norm_before = std::sqrt(ctx->h_small[0]);
+            auto td = clk::now();
+            S.t.norm += secs(tc, td);
+            double nrm2 = 0.0;
+            e = orthogonalize(S, w2, orth, j + 1, &H(0, j), &nrm2);
+            if (e) return e;
+            // selective re-orthogonalization, Giraud-Langou L_crit = 1   (:212-223)
+            double crit = 0.0;
+            for (int i = 0; i <= j; ++i) crit += std::fabs(H(i, j));
+            crit /= norm_before;
+            if (crit >= 1.0) {
+                e = orthogonalize(S, w2, orth, j + 1, &H(0, j), &nrm2);
+                if (e) return e;
+                S.nreorth++;
+            }
+            auto te = clk::now();
+            S.t.dot += secs(td, te);
+            // h(j+1,j) = ||w|| ; v_{j+1} = w / h(j+1,j)   (:229-231)
+            H(j + 1, j) = std::sqrt(nrm2);
+            e = kv_div(ctx, len, w2.w, H(j + 1, j), w2.V + (size_t)(j + 1) * len);
+            if (e) return e;
+            S.t.norm += secs(te, clk::now());
+            // previous Givens rotations, new rotation, rotate p   (:235-264)
+            for (int i = 0; i < j; ++i) {
+                const double hij = c[i] * H(i, j) + s[i] * H(i + 1, j);
+                const double hipj = -s[i] * H(i, j) + c[i] * H(i + 1, j);
+                H(i, j) = hij;
+                H(i + 1, j) = hipj;
+            }
+            const double gam = std::hypot(H(j, j), H(j + 1, j));
+            c[j] = H(j, j) / gam;
+            s[j] = H(j + 1, j) / gam;
+            H(j, j) = gam;
+            H(j + 1, j) = 0.0;
+            const double pj = c[j] * p[j], pjp = -s[j] * p[j];
+            p[j] = pj;
+            p[j + 1] = pjp;
+            niter++;
+            res = std::fabs(p[j + 1]);
+            const bool ca = res < tol, cr = (res / r0norm_initial) < rtol, cn = (niter >= nmax) && (nmax > 0);
+            if (!std::isfinite(res)) return cdg_fail(ctx, CDG_ERR_NAN, "non-finite FGMRES residual estimate");
+            if (ca || cr || cn) { conv = true; break; }
+        }
+        // y = H(1:k,1:k)^-1 p(1:k) by back substitution of the triangular H   (:287-288)
+        for (int i = nvecs - 1; i >= 0; --i) {
+            double acc = p[i];
+            for (int k = i + 1; k < nvecs; ++k) acc -= H(i, k) * y[k];
+            y[i] = acc / H(i, i);
+        }
+        Workspace &w3 = ws;
+        e = kv_combine(ctx, len, x, w3.x0, w3.Z, len, nvecs, y.data());
+        if (e) return e;
+        if (conv) break;
+        e = kv_copy(ctx, len, x, w3.x0);
+        if (e) return e;
+        if (depth == 0) S.nrestart++;
+    }
+    if (niter_out) *niter_out = niter;
+    return CDG_OK;
+}
+
+}  // namespace
+
+extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, const double *b, double *x,
+                                const cdg_fgmres_params *prm, cdg_fgmres_stats *st) {
+    if (!ctx || !A || !M || !b || !x || !prm) return CDG_ERR_ARG;
+    if (A->nrows != M->nrows || A->n != M->n) return cdg_fail(ctx, CDG_ERR_ARG, "matrix does not match preconditioner");
+    cdg_fgmres_stats local;
+    if (!st) st = &local;
+    std::memset(st, 0, sizeof(*st));
+    auto t_start = clk::now();
+    int e = CDG_OK;
+    // M%update(A,b) when the controller asks (type_fgmres.f90:118-122)
+    if (prm->update_preconditioner || !M->factored || M->stamp != A->stamp) {
+        auto tf = clk::now();
+        e = cdg_prec_update(ctx, M, A, nullptr);
+        if (e) return e;
+        st->factored = 1;
+        st->t_factor = secs(tf, clk::now());
+    }
+    const int64_t len = (int64_t)A->nrows * A->n;
+    VecArg bi, xo;
+    e = stage_in(ctx, 0, b, len, bi);
+    if (!e) e = stage_in(ctx, 1, x, len, xo);
+    if (!e) e = upload_in(ctx, bi);
+    if (e) return e;
+    Solver S{ctx, A, M, len};
+    std::vector<Workspace> wss(2);   // outer and inner level
+    int niter = 0;
+    e = fgmres_core(S, wss, 0, bi.dev, xo.dev, prm->nkrylov, prm->tol, prm->rtol, prm->nmax, prm->orthogonalization,
+                    prm->inner_fgmres ? prm : nullptr, &niter);
+    // residual_norm = ||b - A x||   (type_fgmres.f90:305,515)
+    if (!e) {
+        e = launch_mv(ctx, A, xo.dev, wss[0].w);
+        if (!e) e = kv_sub(ctx, len, bi.dev, wss[0].w, wss[0].w);
+        if (!e) e = kv_sumsq(ctx, len, wss[0].w, ctx->d_small);
+        if (!e) e = fetch_small(ctx, 1);
+        if (!e) st->residual_norm = std::sqrt(ctx->h_small[0]);
+    }
+    if (!e) e = check_sweep(ctx, M);
+    if (!e) e = finish_out(ctx, xo);
+    for (Workspace &w : wss) ws_free(ctx, w);
+    st->niter = niter;
+    st->nrestart = S.nrestart;
+    st->nreorth = S.nreorth;
+    st->t_precon = S.t.precon;
+    st->t_mv = S.t.mv;
+    st->t_dot = S.t.dot;
+    st->t_norm = S.t.norm;
+    st->t_total = secs(t_start, clk::now());
+    if (!e && !std::isfinite(st->residual_norm)) e = cdg_fail(ctx, CDG_ERR_NAN, "non-finite final residual");
+    return e;
+}

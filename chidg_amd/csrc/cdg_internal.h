@@ -1,0 +1,128 @@
+// Internal types of the gfx950 linear-solve library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/chidg_hip.h"
+
+#define CDG_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return cdg_fail(ctx, CDG_ERR_HIP, "%s: %s (%s:%d)", #expr, \
+                                              hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+int cdg_fail(const cdg_ctx *ctx, int code, const char *fmt, ...);
+
+// 16-bit brain float storage
+typedef uint16_t cdg_bf16;
+
+// Device memory owned by a context (freed in the owner's destructor path).
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct cdg_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    mutable std::string err;
+    int64_t mem_used = 0;
+    // reduction scratch (device) and pinned host mirror
+    double *d_part = nullptr;     // partial sums
+    size_t part_cap = 0;          // doubles
+    double *d_small = nullptr;    // small results (dots, norms)
+    double *h_small = nullptr;    // pinned host copy
+    size_t small_cap = 0;
+    // staging for host pointers passed to the ABI
+    double *d_stage[3] = {nullptr, nullptr, nullptr};
+    size_t stage_cap[3] = {0, 0, 0};
+};
+
+// Block-row streaming launch shape for one block size / element width.
+struct StreamShape {
+    int T = 0;     // threads per workgroup (V*T is a multiple of n)
+    int dc = 0;    // column advance per iteration = V*T/n
+};
+StreamShape stream_shape(int n, int V);
+
+struct cdg_mat {
+    int32_t nrows = 0, n = 0, nnzb = 0;
+    int64_t ld = 0;                    // elements per block slot (padded, multiple of 64)
+    double *blocks = nullptr;          // nnzb * ld, device
+    int32_t *d_row_ptr = nullptr, *d_col = nullptr;
+    std::vector<int32_t> row_ptr, col, kind;
+    int32_t max_row_blocks = 0;
+    uint64_t stamp = 0;
+    // kind-0 structure (type_domain_matrix.f90:295-302, 592-627)
+    std::vector<int32_t> diag, lo_ptr, lo_blk, lo_col, lo_tr, up_ptr, up_blk, up_col, up_tr;
+    int32_t struct_err = CDG_OK;       // CDG_ERR_NO_TRANSPOSE if the kind-0 pattern is not symmetric
+    std::vector<int32_t> flevel, blevel;
+    int32_t nflev = 0, nblev = 0;
+    StreamShape shape64;
+};
+
+struct cdg_prec {
+    int32_t type = CDG_PREC_IDENTITY;
+    int32_t precision = CDG_FP64;
+    int32_t nrows = 0, n = 0, nlo = 0, nup = 0;
+    int64_t ld = 0;                    // elements per factor slot
+    // factor storage (element type by precision)
+    void *L = nullptr;                 // nlo slots
+    void *U = nullptr;                 // nup slots (FP32/BF16 copies of A's upper blocks)
+    void *Dinv = nullptr;              // nrows slots
+    float *L32 = nullptr, *D32 = nullptr;   // FP32 work for the BF16 variant
+    // device index arrays
+    int32_t *d_lo_ptr = nullptr, *d_lo_blk = nullptr, *d_lo_col = nullptr, *d_lo_tr = nullptr;
+    int32_t *d_up_ptr = nullptr, *d_up_idx = nullptr, *d_up_col = nullptr, *d_up_blk = nullptr;
+    int32_t *d_diag = nullptr;
+    int32_t *d_tasks = nullptr;        // forward order then backward order (2*nrows)
+    int32_t *d_flags = nullptr;        // fdone[nrows], bdone[nrows]
+    int32_t *d_counters = nullptr;     // [0] ticket, [1] spare, [2] launch abort flag, [3] sticky status
+    int32_t *d_err = nullptr;          // first singular row (factor)
+    int32_t epoch = 0;
+    // factor schedule: per level the lower blocks (phase 1) and rows (phase 2/3)
+    std::vector<int32_t> lev_lo_ptr, lev_row_ptr;
+    int32_t *d_lev_lo = nullptr, *d_lev_rows = nullptr;
+    int32_t nlev = 0;
+    int32_t sweep_grid = 0;
+    int32_t maxdep = 1;               // max lower/upper blocks of a row
+    StreamShape shape;
+    uint64_t stamp = 0;
+    int32_t factored = 0;
+    std::vector<DevMem> owned;
+};
+
+// ---- kernel launchers (defined in the .hip files) ----------------------------
+int launch_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y);
+int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
+int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row);
+int launch_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_diag, double s_off);
+int launch_vec_synthetic(cdg_ctx *ctx, int64_t len, uint64_t seed, double scale, double *v);
+int launch_widen(cdg_ctx *ctx, const void *src, int precision, double *dst, int64_t count);
+
+// Krylov vector kernels (k_krylov.hip); all on ctx->stream, device pointers
+int kv_multidot(cdg_ctx *ctx, int64_t len, const double *w, const double *V, int64_t vstride, int j,
+                double *d_out /* j doubles */);
+int kv_sumsq(cdg_ctx *ctx, int64_t len, const double *w, double *d_out);
+// w = w - sum_i h_i V_i (i ascending), h on device; also d_nrm2 = ||w_new||^2 when non-null
+int kv_cgs_update(cdg_ctx *ctx, int64_t len, double *w, const double *V, int64_t vstride, int j,
+                  const double *d_h, double *d_nrm2);
+int kv_axpy_dev(cdg_ctx *ctx, int64_t len, double *w, const double *v, const double *d_h);
+int kv_div(cdg_ctx *ctx, int64_t len, const double *w, double h, double *out);
+int kv_sub(cdg_ctx *ctx, int64_t len, const double *b, const double *Ax, double *r);   // r = b - Ax
+int kv_add(cdg_ctx *ctx, int64_t len, double *z, const double *dz);                   // z += dz
+int kv_combine(cdg_ctx *ctx, int64_t len, double *x, const double *x0, const double *Z, int64_t zstride,
+               int k, const double *h_y /* host */);                                    // x = x0 + sum y_i Z_i
+int kv_zero(cdg_ctx *ctx, int64_t len, double *x);
+int kv_copy(cdg_ctx *ctx, int64_t len, const double *src, double *dst);
+
+int ctx_ensure_small(cdg_ctx *ctx, size_t n);
+int ctx_ensure_part(cdg_ctx *ctx, size_t n);
+int dev_alloc(cdg_ctx *ctx, void **p, size_t bytes);
+void dev_free(cdg_ctx *ctx, void *p, size_t bytes);

@@ -1,0 +1,206 @@
+// FP64 vector kernels of the FGMRES cycle (type_fgmres.f90:331-442 and the
+// chidg_vector_t operators it uses).  Reductions are two-stage with a fixed
+// order (per-workgroup partials, then one workgroup per result), so results
+// are reproducible run to run.
+#include "cdg_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int RT = 256;   // threads per workgroup
+
+int nparts(int64_t len) { return (int)std::min<int64_t>(512, std::max<int64_t>(1, (len + RT * 8 - 1) / (RT * 8))); }
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = RT / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] += sh[tid + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// part[i*nb + bx] = sum over chunk bx of V_i . w        (dot, operator_chidg_dot.f90:30-89)
+__global__ void __launch_bounds__(RT)
+k_dot_part(int64_t len, const double *__restrict__ w, const double *__restrict__ V, int64_t vstride,
+           double *__restrict__ part) {
+    __shared__ double sh[RT];
+    const int i = blockIdx.y, nb = gridDim.x;
+    const double *vi = V + (size_t)i * vstride;
+    double acc = 0.0;
+    for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)nb * RT) acc += vi[k] * w[k];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(size_t)i * nb + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(RT) k_sumsq_part(int64_t len, const double *__restrict__ w, double *__restrict__ part) {
+    __shared__ double sh[RT];
+    double acc = 0.0;
+    for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)gridDim.x * RT) acc += w[k] * w[k];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[i] = sum_b part[i*nb + b]
+__global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, int nb, double *__restrict__ out) {
+    __shared__ double sh[RT];
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RT) acc += part[(size_t)i * nb + b];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[i] = s;
+}
+
+// w = w - h_0 V_0 - h_1 V_1 - ... (sequential, as CGS's loop w = w - htmp(i)*v(i)),
+// optional partial sums of the updated w^2 for the norm that follows.
+__global__ void __launch_bounds__(RT)
+k_cgs_update(int64_t len, double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
+             const double *__restrict__ h, double *__restrict__ part) {
+    __shared__ double sh[RT];
+    double acc = 0.0;
+    for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)gridDim.x * RT) {
+        double wk = w[k];
+        for (int i0 = 0; i0 < j; i0 += 64) {
+            const int ni = min(64, j - i0);
+            for (int i = 0; i < ni; ++i) wk = wk - h[i0 + i] * V[(size_t)(i0 + i) * vstride + k];
+        }
+        w[k] = wk;
+        acc += wk * wk;
+    }
+    if (part) {
+        const double s = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_axpy_dev(int64_t len, double *__restrict__ w, const double *__restrict__ v, const double *__restrict__ h) {
+    const double a = *h;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        w[k] = w[k] - a * v[k];
+}
+
+__global__ void k_div(int64_t len, const double *__restrict__ w, double h, double *__restrict__ out) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        out[k] = w[k] / h;
+}
+
+__global__ void k_sub(int64_t len, const double *__restrict__ b, const double *__restrict__ ax, double *__restrict__ r) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        r[k] = b[k] - ax[k];
+}
+
+__global__ void k_add(int64_t len, double *__restrict__ z, const double *__restrict__ dz) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        z[k] = z[k] + dz[k];
+}
+
+// x = x0 + y_0 Z_0 + y_1 Z_1 + ... (sequential, type_fgmres.f90:292-295)
+__global__ void k_combine(int64_t len, double *__restrict__ x, const double *__restrict__ x0,
+                          const double *__restrict__ Z, int64_t zstride, int kk, const double *__restrict__ y) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x) {
+        double xk = x0[k];
+        for (int i = 0; i < kk; ++i) xk = xk + y[i] * Z[(size_t)i * zstride + k];
+        x[k] = xk;
+    }
+}
+
+__global__ void k_fill(int64_t len, double *__restrict__ x, double v) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        x[k] = v;
+}
+
+__global__ void k_copy(int64_t len, const double *__restrict__ s, double *__restrict__ d) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        d[k] = s[k];
+}
+
+dim3 egrid(int64_t len) { return dim3((unsigned)std::min<int64_t>(std::max<int64_t>(1, (len + 255) / 256), 2048)); }
+
+}  // namespace
+
+int kv_multidot(cdg_ctx *ctx, int64_t len, const double *w, const double *V, int64_t vstride, int j, double *d_out) {
+    if (j <= 0) return CDG_OK;
+    const int nb = nparts(len);
+    int e = ctx_ensure_part(ctx, (size_t)nb * j);
+    if (e) return e;
+    hipLaunchKernelGGL(k_dot_part, dim3(nb, j), dim3(RT), 0, ctx->stream, len, w, V, vstride, ctx->d_part);
+    hipLaunchKernelGGL(k_reduce, dim3(j), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_out);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_sumsq(cdg_ctx *ctx, int64_t len, const double *w, double *d_out) {
+    const int nb = nparts(len);
+    int e = ctx_ensure_part(ctx, (size_t)nb);
+    if (e) return e;
+    hipLaunchKernelGGL(k_sumsq_part, dim3(nb), dim3(RT), 0, ctx->stream, len, w, ctx->d_part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_out);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_cgs_update(cdg_ctx *ctx, int64_t len, double *w, const double *V, int64_t vstride, int j, const double *d_h,
+                  double *d_nrm2) {
+    const int nb = nparts(len);
+    int e = ctx_ensure_part(ctx, (size_t)nb);
+    if (e) return e;
+    hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(RT), 0, ctx->stream, len, w, V, vstride, j, d_h,
+                       d_nrm2 ? ctx->d_part : nullptr);
+    if (d_nrm2) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_nrm2);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_axpy_dev(cdg_ctx *ctx, int64_t len, double *w, const double *v, const double *d_h) {
+    hipLaunchKernelGGL(k_axpy_dev, egrid(len), dim3(256), 0, ctx->stream, len, w, v, d_h);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_div(cdg_ctx *ctx, int64_t len, const double *w, double h, double *out) {
+    hipLaunchKernelGGL(k_div, egrid(len), dim3(256), 0, ctx->stream, len, w, h, out);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_sub(cdg_ctx *ctx, int64_t len, const double *b, const double *ax, double *r) {
+    hipLaunchKernelGGL(k_sub, egrid(len), dim3(256), 0, ctx->stream, len, b, ax, r);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_add(cdg_ctx *ctx, int64_t len, double *z, const double *dz) {
+    hipLaunchKernelGGL(k_add, egrid(len), dim3(256), 0, ctx->stream, len, z, dz);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_combine(cdg_ctx *ctx, int64_t len, double *x, const double *x0, const double *Z, int64_t zstride, int k,
+               const double *h_y) {
+    int e = ctx_ensure_small(ctx, (size_t)k + 8);
+    if (e) return e;
+    double *d_y = ctx->d_small + 8;   // the first slots hold FGMRES scalars
+    CDG_HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * (size_t)k, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_combine, egrid(len), dim3(256), 0, ctx->stream, len, x, x0, Z, zstride, k, d_y);
+    CDG_HIP_TRY(hipGetLastError());
+    // the host array may be reused right away
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CDG_OK;
+}
+
+int kv_zero(cdg_ctx *ctx, int64_t len, double *x) {
+    if (len == 0) return CDG_OK;
+    hipLaunchKernelGGL(k_fill, egrid(len), dim3(256), 0, ctx->stream, len, x, 0.0);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_copy(cdg_ctx *ctx, int64_t len, const double *src, double *dst) {
+    if (len == 0 || src == dst) return CDG_OK;
+    hipLaunchKernelGGL(k_copy, egrid(len), dim3(256), 0, ctx->stream, len, src, dst);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}

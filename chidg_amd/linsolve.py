@@ -1,0 +1,309 @@
+"""Host-side mirror of ChiDG's linear-solve interfaces over the gfx950 C ABI.
+
+Names and semantics follow the reference so that calling code reads like
+ChiDG's own:
+
+* ``chidg_matrix`` -- ``chidg_matrix_t`` (``src/containers/type_chidg_matrix.f90:38-83``)
+  with backend ``'hip'``: ``init`` from a block pattern, ``store``/``assemble``
+  uploads the column-major blocks and stamps the matrix.
+* ``chidg_mv(A, x)`` -- ``operator_chidg_mv.f90:46-61``.
+* ``precon_ILU0`` / ``precon_identity`` -- ``init``/``update(A, b)``/``apply(A, v)``/
+  ``tear_down`` (``src/preconditioners/precon_ILU0.f90:64-470``).
+* ``fgmres`` -- ``fgmres_t`` with ``solve(A, x, b, M)``; ``niter`` and
+  ``residual_norm`` as in ``type_fgmres.f90:81-310,512-542``.
+
+Vectors are numpy float64 arrays (host) or torch float64 tensors on the GPU
+(device-resident); results come back in the kind the inputs were given in.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .pattern import Pattern
+
+_PRECISION = {"fp64": _lib.FP64, "fp32": _lib.FP32, "bf16": _lib.BF16}
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x) -> int:
+    if _is_torch(x):
+        if x.dtype.itemsize != 8 or not x.is_contiguous():
+            raise TypeError("device vectors must be contiguous float64 tensors")
+        return x.data_ptr()
+    if x.dtype != np.float64 or not x.flags.c_contiguous:
+        raise TypeError("host vectors must be contiguous float64 numpy arrays")
+    return x.ctypes.data
+
+
+def _empty_like(x, n):
+    if _is_torch(x):
+        import torch
+        return torch.empty(n, dtype=torch.float64, device=x.device)
+    return np.empty(n, dtype=np.float64)
+
+
+class Context:
+    """One HIP device + stream (``cdg_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        self._check(self.lib.cdg_ctx_create(device, C.byref(h)), "cdg_ctx_create", None)
+        self.h = h
+        self.device = device
+
+    def _check(self, code, what, h=True):
+        if code != _lib.CDG_OK:
+            msg = ""
+            if h is not None and getattr(self, "h", None) is not None:
+                msg = (self.lib.cdg_last_error(self.h) or b"").decode()
+            raise _lib.CdgError(code, what, msg)
+
+    def mem_used(self) -> int:
+        v = C.c_int64()
+        self._check(self.lib.cdg_ctx_mem_used(self.h, C.byref(v)), "cdg_ctx_mem_used")
+        return v.value
+
+    def stream_ptr(self) -> int:
+        """The library's hipStream_t (for torch.cuda.ExternalStream)."""
+        v = C.c_void_p()
+        self._check(self.lib.cdg_ctx_stream(self.h, C.byref(v)), "cdg_ctx_stream")
+        return v.value or 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cdg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class chidg_matrix:
+    """Block-sparse Jacobian resident in HBM (``chidg_matrix_t`` with backend 'hip')."""
+
+    def __init__(self, pattern: Pattern, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.pattern = pattern
+        self.n, self.nrows, self.nnzb = pattern.n, pattern.nrows, pattern.nnzb
+        rp = np.ascontiguousarray(pattern.row_ptr, dtype=np.int32)
+        co = np.ascontiguousarray(pattern.col, dtype=np.int32)
+        ki = np.ascontiguousarray(pattern.kind, dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        h = C.c_void_p()
+        self.ctx._check(self.ctx.lib.cdg_mat_create(self.ctx.h, self.nrows, self.n, rp.ctypes.data_as(ip),
+                                                    co.ctypes.data_as(ip), ki.ctypes.data_as(ip), C.byref(h)),
+                        "cdg_mat_create")
+        self.h = h
+
+    @property
+    def vec_len(self) -> int:
+        return self.nrows * self.n
+
+    def store(self, blocks) -> None:
+        """Upload all blocks (pattern order, each n*n column-major) and stamp."""
+        if _is_torch(blocks):
+            if blocks.numel() != self.nnzb * self.n * self.n:
+                raise ValueError("block array has the wrong size")
+        else:
+            blocks = np.ascontiguousarray(blocks, dtype=np.float64)
+            if blocks.size != self.nnzb * self.n * self.n:
+                raise ValueError("block array has the wrong size")
+        self.ctx._check(self.ctx.lib.cdg_mat_upload(self.ctx.h, self.h, _ptr(blocks)), "cdg_mat_upload")
+
+    assemble = store
+
+    def fill_synthetic(self, seed: int, s_diag: float, s_off: float) -> None:
+        self.ctx._check(self.ctx.lib.cdg_mat_fill_synthetic(self.ctx.h, self.h, seed, s_diag, s_off),
+                        "cdg_mat_fill_synthetic")
+
+    def synthetic_vector(self, seed: int, scale: float = 1.0, like=None):
+        v = _empty_like(like if like is not None else np.empty(0), self.vec_len)
+        self.ctx._check(self.ctx.lib.cdg_vec_fill_synthetic(self.ctx.h, self.h, seed, scale, _ptr(v)),
+                        "cdg_vec_fill_synthetic")
+        return v
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.nnzb * self.n * self.n)
+        self.ctx._check(self.ctx.lib.cdg_mat_download(self.ctx.h, self.h, _ptr(out)), "cdg_mat_download")
+        return out
+
+    @property
+    def stamp(self) -> int:
+        v = C.c_uint64()
+        self.ctx._check(self.ctx.lib.cdg_mat_stamp(self.h, C.byref(v)), "cdg_mat_stamp")
+        return v.value
+
+    def release(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.cdg_mat_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def chidg_mv(A: chidg_matrix, x, out=None):
+    """y = A x over every block class (operator_chidg_mv.f90:46-61)."""
+    y = out if out is not None else _empty_like(x, A.vec_len)
+    A.ctx._check(A.ctx.lib.cdg_mv(A.ctx.h, A.h, _ptr(x), _ptr(y)), "cdg_mv")
+    return y
+
+
+def chidg_mv_async(A: chidg_matrix, x, y) -> None:
+    """Enqueue y = A x on the library stream (device tensors only)."""
+    A.ctx._check(A.ctx.lib.cdg_mv_async(A.ctx.h, A.h, _ptr(x), _ptr(y)), "cdg_mv_async")
+
+
+class _Preconditioner:
+    kind = _lib.PREC_IDENTITY
+
+    def __init__(self, precision: str = "fp64"):
+        self.precision = precision
+        self.h = None
+        self.ctx = None
+        self.initialized = False
+        self.stamp = 0
+
+    def init(self, A: chidg_matrix):
+        self.tear_down()
+        self.ctx = A.ctx
+        h = C.c_void_p()
+        self.ctx._check(self.ctx.lib.cdg_prec_create(self.ctx.h, A.h, self.kind, _PRECISION[self.precision],
+                                                     C.byref(h)), "cdg_prec_create")
+        self.h = h
+        self.initialized = True
+        return self
+
+    def update(self, A: chidg_matrix, b=None):
+        if not self.initialized:
+            self.init(A)
+        bad = C.c_int32(-1)
+        code = self.ctx.lib.cdg_prec_update(self.ctx.h, self.h, A.h, C.byref(bad))
+        if code != _lib.CDG_OK:
+            err = _lib.CdgError(code, "cdg_prec_update", (self.ctx.lib.cdg_last_error(self.ctx.h) or b"").decode())
+            err.row = bad.value
+            raise err
+        self.stamp = A.stamp
+
+    def apply(self, A: chidg_matrix, v, out=None):
+        z = out if out is not None else _empty_like(v, A.vec_len)
+        self.ctx._check(self.ctx.lib.cdg_prec_apply(self.ctx.h, self.h, A.h, _ptr(v), _ptr(z)), "cdg_prec_apply")
+        return z
+
+    def apply_async(self, A: chidg_matrix, v, z) -> None:
+        """Enqueue z = M^-1 v on the library stream (device tensors only)."""
+        self.ctx._check(self.ctx.lib.cdg_prec_apply_async(self.ctx.h, self.h, A.h, _ptr(v), _ptr(z)),
+                        "cdg_prec_apply_async")
+
+    def factors(self, A: chidg_matrix):
+        """(L, Dinv) as FP64 host arrays, L in the matrix's block slots."""
+        L = np.zeros(A.nnzb * A.n * A.n)
+        D = np.zeros(A.nrows * A.n * A.n)
+        self.ctx._check(self.ctx.lib.cdg_prec_download(self.ctx.h, self.h, _ptr(L), _ptr(D)), "cdg_prec_download")
+        return L, D
+
+    def tear_down(self):
+        if self.h is not None and self.ctx is not None:
+            self.ctx.lib.cdg_prec_destroy(self.ctx.h, self.h)
+        self.h = None
+        self.initialized = False
+        self.stamp = 0
+
+    def __del__(self):
+        try:
+            self.tear_down()
+        except Exception:
+            pass
+
+
+class precon_identity(_Preconditioner):
+    kind = _lib.PREC_IDENTITY
+
+
+class precon_ILU0(_Preconditioner):
+    """Block ILU(0): ``precision`` 'fp64' (reference), 'fp32', or 'bf16'."""
+    kind = _lib.PREC_ILU0
+
+
+def create_preconditioner(name: str, precision: str = "fp64") -> _Preconditioner:
+    """Factory with the reference's names (mod_preconditioner.f90:49-108)."""
+    key = name.strip().lower()
+    if key == "ilu0":
+        return precon_ILU0(precision)
+    if key == "identity":
+        return precon_identity(precision)
+    raise ValueError(f"unknown preconditioner {name!r}")
+
+
+class fgmres:
+    """``fgmres_t``: restarted flexible GMRES.  Defaults are the reference's
+    (type_linear_solver.f90:21-40)."""
+
+    def __init__(self, nkrylov=2000, tol=1e-8, rtol=1e-10, nmax=-1, orthogonalization="CGS",
+                 inner_fgmres=True, inner_nkrylov=100, inner_tol=1e-1, inner_rtol=1e-1, inner_nmax=100,
+                 inner_orthogonalization="CGS"):
+        self.nkrylov, self.tol, self.rtol, self.nmax = nkrylov, tol, rtol, nmax
+        self.orthogonalization = orthogonalization
+        self.inner_fgmres, self.inner_nkrylov = inner_fgmres, inner_nkrylov
+        self.inner_tol, self.inner_rtol, self.inner_nmax = inner_tol, inner_rtol, inner_nmax
+        self.inner_orthogonalization = inner_orthogonalization
+        self.niter = 0
+        self.residual_norm = float("nan")
+        self.stats = None
+
+    def set(self, options: dict):
+        """linear_solver_t%set(options) (type_linear_solver.f90:113-131)."""
+        for k, v in options.items():
+            if not hasattr(self, k):
+                raise KeyError(k)
+            setattr(self, k, v)
+
+    def params(self, update_preconditioner: bool = True) -> _lib.FgmresParams:
+        o = {"CGS": _lib.CGS, "MGS": _lib.MGS}
+        return _lib.FgmresParams(self.nkrylov, self.tol, self.rtol, self.nmax, o[self.orthogonalization],
+                                 int(self.inner_fgmres), self.inner_nkrylov, self.inner_tol, self.inner_rtol,
+                                 self.inner_nmax, o[self.inner_orthogonalization], int(update_preconditioner))
+
+    def solve(self, A: chidg_matrix, x, b, M: _Preconditioner, solver_controller=None):
+        """Solve A x = b (x overwritten, zero initial guess).  Without a
+        controller the preconditioner is updated every call, as in the
+        reference; with one, only when the matrix stamp changed."""
+        if not M.initialized:
+            M.init(A)
+        prm = self.params(update_preconditioner=solver_controller is None)
+        st = _lib.FgmresStats()
+        A.ctx._check(A.ctx.lib.cdg_fgmres_solve(A.ctx.h, A.h, M.h, _ptr(b), _ptr(x), C.byref(prm), C.byref(st)),
+                     "cdg_fgmres_solve")
+        self.niter = st.niter
+        self.residual_norm = st.residual_norm
+        self.stats = st
+        if st.factored:
+            M.stamp = A.stamp
+        return x
+
+
+class solver_controller:
+    """Stamp-keyed refactor decision (type_solver_controller.f90:89-124)."""

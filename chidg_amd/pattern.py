@@ -131,9 +131,17 @@ def hex_box(nx: int, ny: int, nz: int, n: int, periodic=(True, True, True)) -> P
     return p
 
 
-def block_size(order: int, nfields: int = 5) -> int:
-    """``nterms = order**3`` for 'Solution Order' k (type_chidg.f90:570-572)."""
+def block_size_order(order: int, nfields: int = 5) -> int:
+    """Block size for ChiDG's 'Solution Order' k: ``nterms = k**3``
+    (type_chidg.f90:570-572)."""
     return order ** 3 * nfields
+
+
+def block_size(p: int, nfields: int = 5) -> int:
+    """Block size for polynomial degree ``p`` (P0, P1, P2, P3): the tensor
+    Legendre basis has ``(p+1)**3`` terms, i.e. Solution Order ``p+1``;
+    P1/P2/P3 with 5 fields give 40/135/320."""
+    return block_size_order(p + 1, nfields)
 
 
 def levels(p: Pattern) -> tuple[np.ndarray, np.ndarray]:

@@ -1,0 +1,164 @@
+/*
+ * chidg_hip.h -- C ABI of the MI355X (gfx950) linear-solve library for ChiDG's
+ * Newton-Krylov loop: block-sparse matvec, block ILU(0) factor/apply and
+ * restarted flexible GMRES.
+ *
+ * The entry points are what an iso_c_binding shim inside ChiDG binds (see
+ * chidg_amd/fortran/mod_chidg_hip.f90).  Each declaration names the reference
+ * interface it replaces (paths relative to the ChiDG source tree).
+ *
+ * Conventions
+ *   - Every function returns CDG_OK (0) or a CDG_ERR_* code; the message of the
+ *     last failure on a context is cdg_last_error(ctx).
+ *   - Vector / block pointers may be host memory or device memory of the
+ *     context's GPU; the library detects which.  Calls are synchronous: results
+ *     are complete when the call returns (the reference semantics).
+ *   - Blocks are dense n x n, column-major (type_densematrix.f90:60,177),
+ *     n = nterms*nfields; vectors are element-major with the reference's
+ *     intra-element order (type_densevector.f90:214,327).
+ *   - Types: double, int32_t (rk = real64, ik = int32, mod_kinds.f90.in).
+ *   - One context per host thread; calls on one context are not reentrant.
+ */
+#ifndef CHIDG_HIP_H
+#define CHIDG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (messenger.f90:152-310 signals FATAL; here they return) ---- */
+#define CDG_OK               0
+#define CDG_ERR_ARG          1   /* invalid argument / pattern                    */
+#define CDG_ERR_HIP          2   /* HIP runtime error                             */
+#define CDG_ERR_SINGULAR     3   /* singular diagonal block (mod_inv.f90:126-128) */
+#define CDG_ERR_NO_TRANSPOSE 4   /* lblks entry without transposed partner
+                                    (type_domain_matrix.f90:620)                  */
+#define CDG_ERR_OOM          5   /* device allocation failed                      */
+#define CDG_ERR_NAN          6   /* non-finite residual (type_newton.f90:197-200) */
+#define CDG_ERR_STATE        7   /* call out of order (e.g. apply before update)  */
+#define CDG_ERR_UNSUPPORTED  8   /* block size / precision not built              */
+
+/* ---- block kinds in a pattern --------------------------------------------- */
+#define CDG_KIND_LOCAL       0   /* lblks entry: interior face or diagonal        */
+#define CDG_KIND_COUPLING    1   /* chi_blks / bc_blks / hb_blks: matvec only     */
+
+/* ---- preconditioners (mod_preconditioner.f90:49-108 factory names) -------- */
+#define CDG_PREC_IDENTITY    0   /* 'identity'                                    */
+#define CDG_PREC_ILU0        1   /* 'ILU0'                                        */
+
+/* ---- storage precision of the ILU(0) factor -------------------------------- */
+#define CDG_FP64             0   /* reference precision                           */
+#define CDG_FP32             1   /* factor and apply in FP32                      */
+#define CDG_BF16             2   /* factor in FP32, stored BF16, FP32 accumulate  */
+
+#define CDG_CGS              0   /* orthogonalization = 'CGS'                     */
+#define CDG_MGS              1   /* orthogonalization = 'MGS'                     */
+
+typedef struct cdg_ctx  cdg_ctx;
+typedef struct cdg_mat  cdg_mat;
+typedef struct cdg_prec cdg_prec;
+
+/* linear_solver_t parameters (type_linear_solver.f90:21-40, set() :113-131). */
+typedef struct {
+    int32_t nkrylov;                 /* restart length                        */
+    double  tol;                     /* absolute tolerance on |p(j+1)|        */
+    double  rtol;                    /* relative to the first cycle's ||r0||  */
+    int32_t nmax;                    /* iteration cap, <= 0 unlimited         */
+    int32_t orthogonalization;       /* CDG_CGS / CDG_MGS                     */
+    int32_t inner_fgmres;            /* nested FGMRES correction on/off       */
+    int32_t inner_nkrylov;
+    double  inner_tol;
+    double  inner_rtol;
+    int32_t inner_nmax;
+    int32_t inner_orthogonalization;
+    int32_t update_preconditioner;   /* 0: refactor only when the matrix stamp
+                                        differs from the factor's stamp
+                                        (solver_controller_t, type_solver_controller.f90:89-124);
+                                        1: always (solve called without a controller,
+                                        type_fgmres.f90:118-122) */
+} cdg_fgmres_params;
+
+/* What fgmres_t%print_report exposes (type_fgmres.f90:512-542), seconds. */
+typedef struct {
+    int32_t niter;
+    double  residual_norm;           /* ||b - A x|| after the solve           */
+    int32_t nrestart;
+    int32_t nreorth;                 /* re-orthogonalized Arnoldi steps        */
+    int32_t factored;                /* 1 if the preconditioner was updated    */
+    double  t_total;
+    double  t_factor;
+    double  t_precon;
+    double  t_mv;
+    double  t_dot;
+    double  t_norm;
+} cdg_fgmres_stats;
+
+/* ---- context ---------------------------------------------------------------- */
+int         cdg_ctx_create(int32_t device, cdg_ctx **out);
+int         cdg_ctx_destroy(cdg_ctx *ctx);
+const char *cdg_last_error(const cdg_ctx *ctx);
+/* Fills the number of bytes of device memory the context currently holds. */
+int         cdg_ctx_mem_used(const cdg_ctx *ctx, int64_t *bytes);
+/* The context's HIP stream (hipStream_t) so callers can order their own work
+ * and record events around the *_async calls.  No reference counterpart. */
+int         cdg_ctx_stream(const cdg_ctx *ctx, void **stream);
+
+/* ---- matrix: chidg_matrix_t (type_chidg_matrix.f90:38-83) --------------------
+ * Pattern: per block row (element, time level) the lblks entries followed by
+ * the coupling-only entries; col[b] = parent row, kind[b] = CDG_KIND_*.
+ * Replaces chidg_matrix%init (type_chidg_matrix.f90:221-248) with backend 'hip'. */
+int cdg_mat_create(cdg_ctx *ctx, int32_t nrows, int32_t block_n, const int32_t *row_ptr,
+                   const int32_t *col, const int32_t *kind, cdg_mat **out);
+int cdg_mat_destroy(cdg_ctx *ctx, cdg_mat *A);
+/* Upload all nnzb blocks, pattern order, each n*n column-major; stamps the
+ * matrix.  Called from chidg_matrix%assemble (type_chidg_matrix.f90:864-867). */
+int cdg_mat_upload(cdg_ctx *ctx, cdg_mat *A, const double *blocks);
+int cdg_mat_download(cdg_ctx *ctx, const cdg_mat *A, double *blocks);
+/* Device-side synthetic values (bench/test input, no reference counterpart):
+ * block (r,c) = g*s_diag (+2 on r==c) for diagonal kind-0 blocks, g*s_off
+ * otherwise; g from a counter hash of (seed, block, c*n+r).  Stamps the matrix. */
+int cdg_mat_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_diag, double s_off);
+/* Synthetic vector of length nrows*n: g(seed', index)*scale (same hash). */
+int cdg_vec_fill_synthetic(cdg_ctx *ctx, const cdg_mat *A, uint64_t seed, double scale, double *v);
+int cdg_mat_stamp(const cdg_mat *A, uint64_t *stamp);
+int cdg_mat_info(const cdg_mat *A, int32_t *nrows, int32_t *block_n, int32_t *nnzb);
+
+/* y = A x over lblks, chi_blks, bc_blks, hb_blks
+ * (chidg_mv, operator_chidg_mv.f90:46-61,117-646). */
+int cdg_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y);
+/* Enqueue-only forms for device-resident callers (no reference counterpart):
+ * x, y, v, z must be device pointers; the work is queued on the context's
+ * stream and the call returns without waiting. */
+int cdg_mv_async(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y);
+
+/* ---- preconditioner: precon_ILU0_t / precon_identity -------------------------
+ * create == preconditioner%init (precon_ILU0.f90:64-90)                        */
+int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int32_t precision,
+                    cdg_prec **out);
+/* preconditioner%update(A,b) (precon_ILU0.f90:111-265).  On CDG_ERR_SINGULAR
+ * *bad_row (may be NULL) receives the first failing block row. */
+int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row);
+/* z = preconditioner%apply(A,v) (precon_ILU0.f90:286-411). */
+int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
+int cdg_prec_apply_async(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
+/* preconditioner%tear_down (precon_ILU0.f90:450-470). */
+int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
+/* Factor as FP64 for inspection: L (nnzb*n*n, lower kind-0 slots filled,
+ * others zero) and Dinv (nrows*n*n), the LD matrix of precon_ILU0_t. */
+int cdg_prec_download(cdg_ctx *ctx, const cdg_prec *M, double *L, double *Dinv);
+
+/* ---- linear solver: fgmres_t%solve(A,x,b,M) (type_fgmres.f90:81-310) --------
+ * x is overwritten (zero initial guess).  Updates M per params. */
+int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, const double *b, double *x,
+                     const cdg_fgmres_params *params, cdg_fgmres_stats *stats);
+
+/* Reference defaults of linear_solver_t (type_linear_solver.f90:21-40). */
+void cdg_fgmres_default_params(cdg_fgmres_params *params);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -122,12 +122,13 @@ def bc_mv_test_system():
     return p, blocks, x, expect
 
 
-def synthetic_system(nx, ny, nz, order, seed, periodic=(True, True, True), nfields=5,
+def synthetic_system(nx, ny, nz, degree, seed, periodic=(True, True, True), nfields=5,
                      s_off_scale=0.1):
-    """Seeded synthetic Jacobian: diag blocks 2I + G/sqrt(n), off-diagonal
-    blocks 0.1*G/sqrt(n), G ~ approx N(0,1) from the counter hash."""
+    """Seeded synthetic Jacobian at polynomial degree ``degree`` (P1: n=40,
+    P2: 135, P3: 320): diag blocks 2I + G/sqrt(n), off-diagonal blocks
+    0.1*G/sqrt(n), G ~ approx N(0,1) from the counter hash."""
     import pyoracle as po
-    n = pt.block_size(order, nfields)
+    n = pt.block_size(degree, nfields)
     p = pt.hex_box(nx, ny, nz, n, periodic)
     blocks = po.fill_synthetic(p, seed, 1.0 / np.sqrt(n), s_off_scale / np.sqrt(n))
     return p, blocks
