@@ -447,19 +447,14 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     if (!e) e = upload_array(ctx, &M->d_up_idx, up_idx, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_up_col, A->up_col, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_diag, A->diag, &M->owned);
-    // sweep tasks: forward rows by forward level, backward rows by backward level
-    std::vector<int32_t> fptr;
+    // forward levels (factor schedule and forward sweep), backward levels
+    std::vector<int32_t> fptr, bptr;
     std::vector<int32_t> forder = order_by_level(A->flevel, A->nflev, &fptr);
-    std::vector<int32_t> border = order_by_level(A->blevel, A->nblev, nullptr);
-    std::vector<int32_t> tasks(forder);
-    tasks.insert(tasks.end(), border.begin(), border.end());
-    if (!e) e = upload_array(ctx, &M->d_tasks, tasks, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_flags, std::vector<int32_t>(2 * (size_t)M->nrows, 0), &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_counters, std::vector<int32_t>(4, 0), &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_err, std::vector<int32_t>(1, 0), &M->owned);
-    // factor schedule: lower blocks and rows grouped by forward level
+    std::vector<int32_t> border = order_by_level(A->blevel, A->nblev, &bptr);
     M->nlev = A->nflev;
     M->lev_row_ptr = fptr;
+    M->nblev = A->nblev;
+    M->blev_row_ptr = bptr;
     std::vector<int32_t> lev_lo;
     M->lev_lo_ptr.assign(M->nlev + 1, 0);
     for (int l = 0; l < M->nlev; ++l) {
@@ -471,20 +466,28 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     }
     if (!e) e = upload_array(ctx, &M->d_lev_lo, lev_lo, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_lev_rows, forder, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_blev_rows, border, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_err, std::vector<int32_t>(1, 0), &M->owned);
+    if (!e) e = alloc((void **)&M->d_t, sizeof(double) * (size_t)M->nrows * M->n);
     if (!e) {
         hipError_t he = hipStreamSynchronize(ctx->stream);
         if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
     }
     if (e) { cdg_prec_destroy(ctx, M); return e; }
-    const int V = precision == CDG_FP64 ? 2 : (precision == CDG_FP32 ? 4 : 8);
-    M->shape = stream_shape(A->n, V);
     M->maxdep = 1;
     for (int i = 0; i < A->nrows; ++i) {
         M->maxdep = std::max(M->maxdep, A->lo_ptr[i + 1] - A->lo_ptr[i]);
         M->maxdep = std::max(M->maxdep, A->up_ptr[i + 1] - A->up_ptr[i]);
     }
-    const int per_cu = M->shape.T > 512 ? 2 : 4;
-    M->sweep_grid = std::max(1, std::min(2 * M->nrows, ctx->num_cus * per_cu));
+    // slices of about 40 rows; threads = whole multiples of the slice length
+    M->slices = std::max(1, (A->n + 39) / 40);
+    M->slice_len = (A->n + M->slices - 1) / M->slices;
+    M->slice_threads = M->slice_len * std::max(1, 512 / M->slice_len);
+    const size_t lds = 8 * ((size_t)M->maxdep * A->n + M->slice_threads);
+    if (lds > 64 * 1024) {
+        cdg_prec_destroy(ctx, M);
+        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply would need %zu bytes of LDS", lds);
+    }
     *out = M;
     return CDG_OK;
 }
@@ -509,21 +512,6 @@ static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const dou
     return launch_ilu_apply(ctx, M, A, v, z);
 }
 
-// Reads and clears the sweep's sticky status (bit 0: a dependency wait timed
-// out, bit 1: corrupt schedule).
-static int check_sweep(cdg_ctx *ctx, cdg_prec *M) {
-    if (M->type != CDG_PREC_ILU0) return CDG_OK;
-    int32_t flag = 0;
-    static const int32_t zero = 0;
-    CDG_HIP_TRY(hipMemcpyAsync(&flag, M->d_counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (flag) {
-        CDG_HIP_TRY(hipMemcpy(M->d_counters + 3, &zero, sizeof(int32_t), hipMemcpyHostToDevice));
-        return cdg_fail(ctx, CDG_ERR_HIP, "ILU sweep aborted (status %d)", flag);
-    }
-    return CDG_OK;
-}
-
 extern "C" int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
     if (!ctx || !M || !A || !v || !z) return CDG_ERR_ARG;
     const int64_t len = (int64_t)M->nrows * M->n;
@@ -533,7 +521,6 @@ extern "C" int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const
     if (!e) e = upload_in(ctx, vi);
     if (!e) e = prec_apply_dev(ctx, M, A, vi.dev, zo.dev);
     if (!e) e = finish_out(ctx, zo);
-    if (!e) e = check_sweep(ctx, M);
     return e;
 }
 
@@ -865,7 +852,6 @@ extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, con
         if (!e) e = fetch_small(ctx, 1);
         if (!e) st->residual_norm = std::sqrt(ctx->h_small[0]);
     }
-    if (!e) e = check_sweep(ctx, M);
     if (!e) e = finish_out(ctx, xo);
     for (Workspace &w : wss) ws_free(ctx, w);
     st->niter = niter;

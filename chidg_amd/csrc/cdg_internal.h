@@ -81,18 +81,21 @@ struct cdg_prec {
     int32_t *d_lo_ptr = nullptr, *d_lo_blk = nullptr, *d_lo_col = nullptr, *d_lo_tr = nullptr;
     int32_t *d_up_ptr = nullptr, *d_up_idx = nullptr, *d_up_col = nullptr, *d_up_blk = nullptr;
     int32_t *d_diag = nullptr;
-    int32_t *d_tasks = nullptr;        // forward order then backward order (2*nrows)
-    int32_t *d_flags = nullptr;        // fdone[nrows], bdone[nrows]
-    int32_t *d_counters = nullptr;     // [0] ticket, [1] spare, [2] launch abort flag, [3] sticky status
     int32_t *d_err = nullptr;          // first singular row (factor)
-    int32_t epoch = 0;
-    // factor schedule: per level the lower blocks (phase 1) and rows (phase 2/3)
+    double *d_t = nullptr;             // backward-sweep stage A result (nrows*n)
+    // forward levels: rows grouped by level (factor and forward sweep); lower
+    // blocks of those rows grouped the same way (factor phase 1)
     std::vector<int32_t> lev_lo_ptr, lev_row_ptr;
     int32_t *d_lev_lo = nullptr, *d_lev_rows = nullptr;
     int32_t nlev = 0;
-    int32_t sweep_grid = 0;
+    // backward levels (backward sweep)
+    std::vector<int32_t> blev_row_ptr;
+    int32_t *d_blev_rows = nullptr;
+    int32_t nblev = 0;
     int32_t maxdep = 1;               // max lower/upper blocks of a row
-    StreamShape shape;
+    // apply launch shape: each row's output is cut into `slices` pieces of
+    // `slice_len` rows; one workgroup of `slice_threads` threads per piece
+    int32_t slices = 1, slice_len = 1, slice_threads = 64;
     uint64_t stamp = 0;
     int32_t factored = 0;
     std::vector<DevMem> owned;

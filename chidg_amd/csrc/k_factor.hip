@@ -123,21 +123,36 @@ k_fac_p2(const double *__restrict__ Ab, int64_t lda, const T *__restrict__ L, T 
         }
 }
 
-// phase 3: in-place inverse of D_i.  Thread (tr, tc) of a TR x TR grid keeps
-// rows tr + TR*a and columns tc + TR*b (a, b < R) in registers.  Pivot rows are
-// not swapped physically: step k picks the unused row p with the largest
-// |a(p,k)|, and the final scatter inv(rowstep[r], pivrow[m]) = S(r, m) undoes
-// the implicit permutation.
-template <typename T, int TR, int R>
-__global__ void __launch_bounds__(TR *TR)
+// phase 3: in-place inverse of D_i, one workgroup of 4 waves (one per SIMD)
+// per block.  Thread (tr, tc) of the 16 x 16 thread grid keeps rows
+// tr + 16a and columns tc + 16b (a, b < R) in registers.  Gauss-Jordan with
+// partial pivoting and two barriers per pivot step:
+//   1. the owners of column k publish it;
+//   2. every wave finds the pivot row p itself (same data, same deterministic
+//      reduction, so all waves agree; ties go to the lower row index) and the
+//      owners of row p publish it;
+//   3. everyone applies  S -= c_r * prow  (c_p = 0) and rewrites column k.
+// Rows are never swapped; the final scatter inv(rowstep[r], pivrow[m]) undoes
+// the implicit permutation.  The pivot row is not scaled in place: row r keeps
+// its unscaled values and the factor sigma_r = 1/pivot is applied when the
+// result is written (later eliminations are linear in the row, so they act on
+// the unscaled values consistently).  Register indices are compile-time: the
+// pivot loop runs k = 16*kb + kt with kb unrolled, and the pivot row is
+// published through one LDS buffer per register row, so no register array is
+// indexed at run time.
+template <typename T, int R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err) {
+    constexpr int TR = 16;
     constexpr int NMAX = TR * R;
-    __shared__ T colv[2][NMAX], rowv[2][NMAX];
-    __shared__ int32_t used[NMAX], pivrow[NMAX], rowstep[NMAX];
-    __shared__ int32_t piv_sh, bad_sh;
+    constexpr int UR = (NMAX + 63) / 64;   // rows per lane in the pivot search
+    __shared__ T colv[2][NMAX];
+    __shared__ T rowv[2][R][NMAX];
+    __shared__ T sig[NMAX];
+    __shared__ int32_t pivrow[NMAX], rowstep[NMAX];
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
-    const int tid = threadIdx.x, tr = tid % TR, tc = tid / TR;
+    const int tid = threadIdx.x, tr = tid % TR, tc = tid / TR, lane = tid & 63;
     T S[R][R];
 #pragma unroll
     for (int a = 0; a < R; ++a)
@@ -146,86 +161,86 @@ k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n
             const int r = tr + TR * a, m = tc + TR * b;
             S[a][b] = (r < n && m < n) ? M[(size_t)m * n + r] : T(0);
         }
-    for (int i = tid; i < NMAX; i += TR * TR) used[i] = 0;
-    if (tid == 0) bad_sh = 0;
+    for (int i = tid; i < 2 * NMAX; i += 256) (&colv[0][0])[i] = T(0);
+    for (int i = tid; i < 2 * R * NMAX; i += 256) (&rowv[0][0][0])[i] = T(0);
+    bool used[UR];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) used[u] = false;
+    int bad = 0;
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        const int buf = k & 1;
-        const int kb = k / TR, kt = k - kb * TR;
-        if (tc == kt) {
 #pragma unroll
-            for (int a = 0; a < R; ++a) {
-                const int r = tr + TR * a;
-                T v = 0;
+    for (int kb = 0; kb < R; ++kb) {
+        const int kt_end = min(TR, n - kb * TR);
+        for (int kt = 0; kt < kt_end; ++kt) {
+            const int k = kb * TR + kt;
+            const int buf = k & 1;
+            if (tc == kt) {   // padding rows hold zeros, so no bounds test is needed
 #pragma unroll
-                for (int b = 0; b < R; ++b) v = (b == kb) ? S[a][b] : v;
-                if (r < n) colv[buf][r] = v;
+                for (int a = 0; a < R; ++a) colv[buf][tr + TR * a] = S[a][kb];
             }
-        }
-        __syncthreads();
-        if (tid < 64) {
+            __syncthreads();
+            // pivot: largest |a(r,k)| over unused rows, lowest index on ties
             T best = T(-1);
-            int bi = n;
-            for (int r = tid; r < n; r += 64)
-                if (!used[r]) {
+            int bi = 0x7fffffff, fu = 0x7fffffff;
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                const int r = lane + 64 * u;
+                if (r < n && !used[u]) {
                     const T v = fabs(colv[buf][r]);
                     if (v > best) { best = v; bi = r; }
+                    fu = min(fu, r);
                 }
+            }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 const T ob = __shfl_xor(best, off, 64);
                 const int oi = __shfl_xor(bi, off, 64);
                 if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+                fu = min(fu, __shfl_xor(fu, off, 64));
             }
-            if (tid == 0) {
-                if (!(best > T(0)) || bi >= n) {   // zero or non-finite pivot column
-                    bad_sh = 1;
-                    for (int r = 0; r < n; ++r)
-                        if (!used[r]) { bi = r; break; }
-                }
-                piv_sh = bi;
-                used[bi] = 1;
-                pivrow[k] = bi;
-                rowstep[bi] = k;
+            int p = bi;
+            if (!(best > T(0)) || p >= n) { bad = 1; p = fu; }   // zero or non-finite pivot column
+            p = __builtin_amdgcn_readfirstlane(p);
+#pragma unroll
+            for (int u = 0; u < UR; ++u)
+                if (lane + 64 * u == p) used[u] = true;
+            const int pa = p / TR, pt = p - pa * TR;
+            const T rinv = T(1) / colv[buf][p];
+            if (tid == 0) { pivrow[k] = p; rowstep[p] = k; sig[p] = rinv; }
+            if (tr == pt) {
+#pragma unroll
+                for (int a = 0; a < R; ++a)
+                    if (a == pa) {
+#pragma unroll
+                        for (int b = 0; b < R; ++b) rowv[buf][a][tc + TR * b] = S[a][b];
+                    }
             }
-        }
-        __syncthreads();
-        const int p = piv_sh;
-        const int pa = p / TR, pt = p - pa * TR;
-        if (tr == pt) {
+            __syncthreads();
+            T prow[R], cr[R];
 #pragma unroll
-            for (int b = 0; b < R; ++b) {
-                const int m = tc + TR * b;
-                T v = 0;
+            for (int b = 0; b < R; ++b) prow[b] = rowv[buf][pa][tc + TR * b] * rinv;
 #pragma unroll
-                for (int a = 0; a < R; ++a) v = (a == pa) ? S[a][b] : v;
-                if (m < n) rowv[buf][m] = v;
-            }
-        }
-        __syncthreads();
-        const T rinv = T(1) / rowv[buf][k];
+            for (int a = 0; a < R; ++a) cr[a] = (tr + TR * a == p) ? T(0) : colv[buf][tr + TR * a];
 #pragma unroll
-        for (int b = 0; b < R; ++b) {
-            const int m = tc + TR * b;
-            if (m >= n) continue;
-            const T prow = (m == k ? T(1) : rowv[buf][m]) * rinv;
+            for (int a = 0; a < R; ++a)
+#pragma unroll
+                for (int b = 0; b < R; ++b) S[a][b] -= cr[a] * prow[b];
+            // column k: -c_r/pivot, and 1 (times sigma_p = 1/pivot) in row p
 #pragma unroll
             for (int a = 0; a < R; ++a) {
-                const int r = tr + TR * a;
-                if (r >= n) continue;
-                if (r == p) S[a][b] = prow;
-                else S[a][b] = (m == k ? T(0) : S[a][b]) - colv[buf][r] * prow;
+                const T v = (tr + TR * a == p) ? T(1) : -cr[a] * rinv;
+                S[a][kb] = (tc == kt) ? v : S[a][kb];
             }
         }
     }
     __syncthreads();
-    if (tid == 0 && bad_sh) atomicMin(err, blk);
+    if (tid == 0 && bad) atomicMin(err, blk);
 #pragma unroll
     for (int a = 0; a < R; ++a)
 #pragma unroll
         for (int b = 0; b < R; ++b) {
             const int r = tr + TR * a, m = tc + TR * b;
-            if (r < n && m < n) M[(size_t)pivrow[m] * n + rowstep[r]] = S[a][b];
+            if (r < n && m < n) M[(size_t)pivrow[m] * n + rowstep[r]] = sig[r] * S[a][b];
         }
 }
 
@@ -258,18 +273,12 @@ __global__ void k_f32_to_bf16(const float *__restrict__ src, cdg_bf16 *__restric
 
 template <typename T>
 int launch_inv(cdg_ctx *ctx, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n, int32_t *err) {
-    dim3 grid(nrows_lvl);
-    if (n <= 48) {
-        hipLaunchKernelGGL((k_fac_inv<T, 16, 3>), grid, dim3(256), 0, ctx->stream, D, ld, rows, n, err);
-    } else if (n <= 96) {
-        hipLaunchKernelGGL((k_fac_inv<T, 16, 6>), grid, dim3(256), 0, ctx->stream, D, ld, rows, n, err);
-    } else if (n <= 160) {
-        hipLaunchKernelGGL((k_fac_inv<T, 32, 5>), grid, dim3(1024), 0, ctx->stream, D, ld, rows, n, err);
-    } else if (n <= 192) {
-        hipLaunchKernelGGL((k_fac_inv<T, 32, 6>), grid, dim3(1024), 0, ctx->stream, D, ld, rows, n, err);
-    } else {
-        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "block inverse for n=%d not built", n);
-    }
+    dim3 grid(nrows_lvl), block(256);
+    if (n <= 48) hipLaunchKernelGGL((k_fac_inv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+    else if (n <= 96) hipLaunchKernelGGL((k_fac_inv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+    else if (n <= 144) hipLaunchKernelGGL((k_fac_inv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+    else if (n <= 192) hipLaunchKernelGGL((k_fac_inv<T, 12>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+    else return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "block inverse for n=%d not built", n);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }

@@ -18,10 +18,8 @@ namespace {
 constexpr int kMaxThreads = 1024;
 
 typedef double nat_f64x2 __attribute__((ext_vector_type(2)));
-typedef float nat_f32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t nat_u32x4 __attribute__((ext_vector_type(4)));
 
-// ---- 16-byte vector loads converted to the accumulation type ----------------
+// ---- 16-byte vector loads ---------------------------------------------------
 template <typename S, typename Acc> struct Ld16;
 
 template <> struct Ld16<double, double> {
@@ -31,25 +29,10 @@ template <> struct Ld16<double, double> {
         o[0] = t.x; o[1] = t.y;
     }
 };
-template <> struct Ld16<float, float> {
-    static constexpr int V = 4;
-    __device__ static __forceinline__ void load(const float *p, float (&o)[4]) {
-        const nat_f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const nat_f32x4 *>(p));
-        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
-    }
-};
-template <> struct Ld16<cdg_bf16, float> {
-    static constexpr int V = 8;
-    __device__ static __forceinline__ void load(const cdg_bf16 *p, float (&o)[8]) {
-        const nat_u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const nat_u32x4 *>(p));
-        const uint32_t w[4] = {t.x, t.y, t.z, t.w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            o[2 * i] = __uint_as_float(w[i] << 16);
-            o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
-        }
-    }
-};
+template <typename S, typename Acc> __device__ __forceinline__ Acc to_acc(S v) { return static_cast<Acc>(v); }
+template <> __device__ __forceinline__ float to_acc<cdg_bf16, float>(cdg_bf16 v) {
+    return __uint_as_float(static_cast<uint32_t>(v) << 16);
+}
 
 // Per-thread view of the flat element walk.
 template <int V> struct Walk {
@@ -149,141 +132,79 @@ k_mv(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ 
 }
 
 // ============================================================================
-// K3: ILU(0) apply as one persistent dataflow kernel (precon_ILU0.f90:286-411)
-//   task t <  nrows : forward row  tasks[t]:  z_i = v_i - sum_j L_ij z_j
-//   task t >= nrows : backward row tasks[t]:  z_i = Dinv_i (z_i - sum_k U_ik z_k)
-// Tasks are taken in topological order from an atomic counter; a task waits
-// only on rows of earlier tasks (fdone/bdone flags == epoch), so the sweep has
-// no level barriers and needs no co-residency guarantee.  Every wait is
-// bounded: on timeout the kernel records the fact and drains.
+// K3: ILU(0) apply (precon_ILU0.f90:286-411), level-synchronous.
+//   forward,  level l:  z_i = v_i - sum_j L_ij z_j            (rows of level l)
+//   backward, level l:  t_i = z_i - sum_k U_ik z_k   (stage A)
+//                       z_i = Dinv_i t_i             (stage B)
+// Rows of one wavefront level are independent, so a level is one launch whose
+// workgroups each own one (row, output slice) pair: the slice's rows of every
+// block are read column by column (a column's slice is contiguous), every
+// thread keeps one fixed output row and a strided set of columns, and the
+// partial sums meet in LDS in a fixed order.  No inter-workgroup waiting.
 // ============================================================================
-struct SweepArgs {
-    const void *L;           // nlo slots
-    const void *U;           // upper slots (A's blocks for FP64)
-    const void *Dinv;        // nrows slots
-    int64_t ld, ldu;
-    const int32_t *lo_ptr, *lo_col;          // L slot = lower index q
-    const int32_t *up_ptr, *up_idx, *up_col; // U slot = up_idx[u]
-    const int32_t *tasks;
-    int32_t *fdone, *bdone, *counters;
-    int32_t epoch;
-    const double *v;
-    double *z;
-    int n, nrows, dc, maxdep;
+struct SliceArgs {
+    const void *B;            // block storage (L, U, or Dinv)
+    int64_t ld;               // elements per block slot
+    const int32_t *rows;      // rows of this level
+    const int32_t *ptr;       // per-row block range (lower or upper lists); null: the row's own block
+    const int32_t *idx;       // block slot of entry d (null: slot = d)
+    const int32_t *col;       // parent row of entry d
+    const double *x;          // vector the parents are read from
+    const double *base;       // result = base_i - sum (null: result = sum)
+    double *out;              // result vector
+    int n, S, len;            // block size, slices per row, slice length
+    int maxdep;
 };
 
-constexpr int kSpinLimit = 1 << 21;   // polls (each followed by s_sleep) before giving up
-
-// Waits for *f == epoch.  Gives up after kSpinLimit polls, or at once when
-// another wait of this launch has already given up (*abort != 0), so a broken
-// dependency drains the grid instead of hanging it.
-__device__ __forceinline__ bool wait_flag(int32_t *f, int32_t epoch, int32_t *abort) {
-    for (int it = 0; it < kSpinLimit; ++it) {
-        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return true;
-        if ((it & 255) == 255 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-        __builtin_amdgcn_s_sleep(2);
-    }
-    return false;
-}
-
-// Next task index, or -1 when the tickets are exhausted or the schedule
-// entry is out of range (flagged in the status words).
-__device__ __forceinline__ int take_ticket(const SweepArgs &a) {
-    const int t = atomicAdd(&a.counters[0], 1);
-    if (t < 0 || t >= 2 * a.nrows) return -1;
-    const int r = a.tasks[t];
-    if (r < 0 || r >= a.nrows) {
-        atomicOr(&a.counters[2], 2);
-        atomicOr(&a.counters[3], 2);
-        return -1;
-    }
-    return t;
-}
-
 template <typename S, typename Acc>
-__global__ void __launch_bounds__(kMaxThreads) k_ilu_sweep(SweepArgs a) {
-    constexpr int V = Ld16<S, Acc>::V;
+__global__ void __launch_bounds__(kMaxThreads) k_slice_gemv(SliceArgs a) {
     extern __shared__ unsigned char smem_raw[];
-    Acc *xs = reinterpret_cast<Acc *>(smem_raw);           // maxdep * n
-    Acc *red = xs + (size_t)a.maxdep * a.n;                 // V*T
-    Acc *tv = red + (size_t)V * blockDim.x;                 // n
-    __shared__ int32_t task_sh;
-    const int T = blockDim.x, tid = threadIdx.x, n = a.n, nn = n * n;
-    const Walk<V> w = make_walk<V>(tid, T, n, a.dc);
-    const int parts = V * T / n;
-    const S *Lp = static_cast<const S *>(a.L);
-    const S *Up = static_cast<const S *>(a.U);
-    const S *Dp = static_cast<const S *>(a.Dinv);
-    // Loop shape matters: the tid-0 work (publish the finished row, take the
-    // next ticket) sits in one block followed by an unconditional barrier, so
-    // the back-edge is convergent.  With tid-0 blocks at both the end and the
-    // start of the body, the structurizer merges them across the back-edge and
-    // the other lanes re-enter the next barrier before tid 0 has published the
-    // row and taken a ticket: the workgroup never drains.
-    if (tid == 0) task_sh = take_ticket(a);
-    __syncthreads();
-    for (;;) {
-        const int task = __builtin_amdgcn_readfirstlane(task_sh);
-        if (task < 0) break;   // tickets exhausted (or a corrupt schedule: flagged, drained)
-        const bool fwd = task < a.nrows;
-        const int row = __builtin_amdgcn_readfirstlane(a.tasks[task]);
-        const int d0 = fwd ? a.lo_ptr[row] : a.up_ptr[row];
-        const int d1 = fwd ? a.lo_ptr[row + 1] : a.up_ptr[row + 1];
-        const int32_t *dcol = fwd ? a.lo_col : a.up_col;
-        if (tid == 0) {
-            bool ok = true;
-            if (!fwd) ok = wait_flag(&a.fdone[row], a.epoch, &a.counters[2]);
-            for (int d = d0; d < d1 && ok; ++d)
-                ok = wait_flag(fwd ? &a.fdone[dcol[d]] : &a.bdone[dcol[d]], a.epoch, &a.counters[2]);
-            if (!ok) { atomicOr(&a.counters[2], 1); atomicOr(&a.counters[3], 1); }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        __syncthreads();
-        const int nd = d1 - d0;
-        for (int idx = tid; idx < nd * n; idx += T) {
-            const int k = idx / n, c = idx - k * n;
-            xs[idx] = static_cast<Acc>(a.z[(size_t)dcol[d0 + k] * n + c]);
-        }
-        __syncthreads();
-        Acc acc[V];
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = 0;
-        for (int k = 0; k < nd; ++k) {
-            const S *blk = fwd ? Lp + (size_t)(d0 + k) * a.ld : Up + (size_t)a.up_idx[d0 + k] * a.ldu;
-            stream_block<S, Acc>(blk, xs + k * n, nn, w, acc);
-        }
-        park<Acc, V>(red, tid, acc);
-        __syncthreads();
-        double *zi = a.z + (size_t)row * n;
-        if (fwd) {
-            const double *vi = a.v + (size_t)row * n;
-            for (int r = tid; r < n; r += T) zi[r] = vi[r] - static_cast<double>(row_sum(red, r, n, parts));
-        } else {
-            for (int r = tid; r < n; r += T) tv[r] = static_cast<Acc>(zi[r]) - row_sum(red, r, n, parts);
-            __syncthreads();
-#pragma unroll
-            for (int v = 0; v < V; ++v) acc[v] = 0;
-            stream_block<S, Acc>(Dp + (size_t)row * a.ld, tv, nn, w, acc);
-            park<Acc, V>(red, tid, acc);
-            __syncthreads();
-            for (int r = tid; r < n; r += T) zi[r] = static_cast<double>(row_sum(red, r, n, parts));
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (tid == 0) {
-            __hip_atomic_store(fwd ? &a.fdone[row] : &a.bdone[row], a.epoch, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            task_sh = take_ticket(a);
-        }
-        __syncthreads();
+    const int T = blockDim.x, tid = threadIdx.x, n = a.n, len = a.len;
+    Acc *xs = reinterpret_cast<Acc *>(smem_raw);            // maxdep * n
+    Acc *red = xs + (size_t)a.maxdep * n;                    // T
+    const int row = a.rows[blockIdx.x / a.S];
+    const int slice = blockIdx.x - (blockIdx.x / a.S) * a.S;
+    const int r0 = slice * len;
+    const int r1 = min(n, r0 + len);
+    int d0 = 0, d1 = 1;
+    if (a.ptr) { d0 = a.ptr[row]; d1 = a.ptr[row + 1]; }
+    const int nd = d1 - d0;
+    for (int e = tid; e < nd * n; e += T) {
+        const int k = e / n, c = e - k * n;
+        const int parent = a.ptr ? a.col[d0 + k] : row;
+        xs[e] = static_cast<Acc>(a.x[(size_t)parent * n + c]);
     }
-}
-
-// Arms the sweep: ticket counter and this launch's abort flag to zero (a
-// kernel rather than a memset so the pair can be captured in a graph).
-// counters[3] is the sticky status the host reads and clears.
-__global__ void k_sweep_arm(int32_t *counters) {
-    if (threadIdx.x < 3) counters[threadIdx.x] = 0;
+    __syncthreads();
+    const int rr = tid % len, cg = tid / len, ncg = T / len;
+    const S *Bp = static_cast<const S *>(a.B);
+    Acc acc = 0;
+    if (cg < ncg && r0 + rr < r1) {
+        for (int k = 0; k < nd; ++k) {
+            const int slot = a.ptr ? (a.idx ? a.idx[d0 + k] : d0 + k) : row;
+            const S *blk = Bp + (size_t)slot * a.ld + r0 + rr;
+            const Acc *xk = xs + k * n;
+            int c = cg;
+            for (; c + 3 * ncg < n; c += 4 * ncg) {
+                const Acc b0 = to_acc<S, Acc>(blk[(size_t)c * n]);
+                const Acc b1 = to_acc<S, Acc>(blk[(size_t)(c + ncg) * n]);
+                const Acc b2 = to_acc<S, Acc>(blk[(size_t)(c + 2 * ncg) * n]);
+                const Acc b3 = to_acc<S, Acc>(blk[(size_t)(c + 3 * ncg) * n]);
+                acc += b0 * xk[c];
+                acc += b1 * xk[c + ncg];
+                acc += b2 * xk[c + 2 * ncg];
+                acc += b3 * xk[c + 3 * ncg];
+            }
+            for (; c < n; c += ncg) acc += to_acc<S, Acc>(blk[(size_t)c * n]) * xk[c];
+        }
+    }
+    red[tid] = acc;
+    __syncthreads();
+    if (tid < r1 - r0) {
+        Acc s = 0;
+        for (int g = 0; g < ncg; ++g) s += red[g * len + tid];
+        const size_t o = (size_t)row * n + r0 + tid;
+        a.out[o] = a.base ? a.base[o] - static_cast<double>(s) : static_cast<double>(s);
+    }
 }
 
 // Identity preconditioner / widening copy: dst[k] = (double) src[k]
@@ -335,38 +256,61 @@ int launch_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y) {
     return CDG_OK;
 }
 
-int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
-    if (M->nrows == 0) return CDG_OK;
-    SweepArgs a;
-    a.L = M->L;
-    a.Dinv = M->Dinv;
-    a.ld = M->ld;
-    if (M->precision == CDG_FP64) { a.U = A->blocks; a.ldu = A->ld; }
-    else { a.U = M->U; a.ldu = M->ld; }
-    a.lo_ptr = M->d_lo_ptr; a.lo_col = M->d_lo_col;
-    a.up_ptr = M->d_up_ptr; a.up_idx = M->d_up_idx; a.up_col = M->d_up_col;
-    a.tasks = M->d_tasks;
-    a.fdone = M->d_flags; a.bdone = M->d_flags + M->nrows;
-    a.counters = M->d_counters;
-    a.epoch = ++M->epoch;
-    a.v = v; a.z = z;
-    a.n = M->n; a.nrows = M->nrows; a.dc = M->shape.dc;
+namespace {
+
+template <typename S, typename Acc>
+int launch_slice(cdg_ctx *ctx, const cdg_prec *M, SliceArgs a, int nrows_lvl) {
+    if (nrows_lvl == 0) return CDG_OK;
+    a.n = M->n;
+    a.S = M->slices;
+    a.len = M->slice_len;
     a.maxdep = M->maxdep;
-    const int T = M->shape.T;
-    size_t accsz = M->precision == CDG_FP64 ? sizeof(double) : sizeof(float);
-    int V = M->precision == CDG_FP64 ? 2 : (M->precision == CDG_FP32 ? 4 : 8);
-    size_t lds = accsz * ((size_t)a.maxdep * M->n + (size_t)V * T + (size_t)M->n);
-    if (lds > 64 * 1024) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU sweep needs %zu bytes of LDS", lds);
-    dim3 grid(M->sweep_grid), block(T);
-    hipLaunchKernelGGL(k_sweep_arm, dim3(1), dim3(64), 0, ctx->stream, M->d_counters);
-    if (M->precision == CDG_FP64)
-        hipLaunchKernelGGL((k_ilu_sweep<double, double>), grid, block, lds, ctx->stream, a);
-    else if (M->precision == CDG_FP32)
-        hipLaunchKernelGGL((k_ilu_sweep<float, float>), grid, block, lds, ctx->stream, a);
-    else
-        hipLaunchKernelGGL((k_ilu_sweep<cdg_bf16, float>), grid, block, lds, ctx->stream, a);
+    const int T = M->slice_threads;
+    const size_t lds = sizeof(Acc) * ((size_t)a.maxdep * M->n + T);
+    hipLaunchKernelGGL((k_slice_gemv<S, Acc>), dim3(nrows_lvl * M->slices), dim3(T), lds, ctx->stream, a);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
+}
+
+template <typename S, typename Acc>
+int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
+    const void *U = M->precision == CDG_FP64 ? static_cast<const void *>(A->blocks) : M->U;
+    const int64_t ldu = M->precision == CDG_FP64 ? A->ld : M->ld;
+    for (int l = 0; l < M->nlev; ++l) {
+        const int r0 = M->lev_row_ptr[l], r1 = M->lev_row_ptr[l + 1];
+        SliceArgs f{};
+        f.B = M->L; f.ld = M->ld; f.rows = M->d_lev_rows + r0;
+        f.ptr = M->d_lo_ptr; f.idx = nullptr; f.col = M->d_lo_col;
+        f.x = z; f.base = v; f.out = z;
+        int e = launch_slice<S, Acc>(ctx, M, f, r1 - r0);
+        if (e) return e;
+    }
+    for (int l = 0; l < M->nblev; ++l) {
+        const int r0 = M->blev_row_ptr[l], r1 = M->blev_row_ptr[l + 1];
+        SliceArgs u{};
+        u.B = U; u.ld = ldu; u.rows = M->d_blev_rows + r0;
+        u.ptr = M->d_up_ptr; u.idx = M->d_up_idx; u.col = M->d_up_col;
+        u.x = z; u.base = z; u.out = M->d_t;
+        int e = launch_slice<S, Acc>(ctx, M, u, r1 - r0);
+        if (e) return e;
+        SliceArgs d{};
+        d.B = M->Dinv; d.ld = M->ld; d.rows = M->d_blev_rows + r0;
+        d.ptr = nullptr; d.idx = nullptr; d.col = nullptr;
+        d.x = M->d_t; d.base = nullptr; d.out = z;
+        e = launch_slice<S, Acc>(ctx, M, d, r1 - r0);
+        if (e) return e;
+    }
+    return CDG_OK;
+}
+
+}  // namespace
+
+int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
+    if (M->nrows == 0) return CDG_OK;
+    if (v == z) return cdg_fail(ctx, CDG_ERR_ARG, "ILU apply needs distinct input and output vectors");
+    if (M->precision == CDG_FP64) return sweep<double, double>(ctx, M, A, v, z);
+    if (M->precision == CDG_FP32) return sweep<float, float>(ctx, M, A, v, z);
+    return sweep<cdg_bf16, float>(ctx, M, A, v, z);
 }
 
 int launch_widen(cdg_ctx *ctx, const void *src, int precision, double *dst, int64_t count) {
