@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -291,7 +292,7 @@ extern "C" int cdg_mat_create(cdg_ctx *ctx, int32_t nrows, int32_t block_n, cons
     A->col.assign(col, col + nnzb);
     A->kind.assign(kind, kind + nnzb);
     analyze(A);
-    A->shape64 = stream_shape(block_n, 2);
+    A->shape64 = stream_shape(block_n, 2, 1024);
     int e = dev_alloc(ctx, (void **)&A->blocks, sizeof(double) * (size_t)std::max(1, nnzb) * A->ld);
     if (!e) e = upload_array(ctx, &A->d_row_ptr, A->row_ptr, nullptr);
     if (!e) e = upload_array(ctx, &A->d_col, A->col, nullptr);
@@ -468,7 +469,6 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     if (!e) e = upload_array(ctx, &M->d_lev_rows, forder, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_blev_rows, border, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_err, std::vector<int32_t>(1, 0), &M->owned);
-    if (!e) e = alloc((void **)&M->d_t, sizeof(double) * (size_t)M->nrows * M->n);
     if (!e) {
         hipError_t he = hipStreamSynchronize(ctx->stream);
         if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
@@ -479,14 +479,29 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
         M->maxdep = std::max(M->maxdep, A->lo_ptr[i + 1] - A->lo_ptr[i]);
         M->maxdep = std::max(M->maxdep, A->up_ptr[i + 1] - A->up_ptr[i]);
     }
-    // slices of about 40 rows; threads = whole multiples of the slice length
-    M->slices = std::max(1, (A->n + 39) / 40);
-    M->slice_len = (A->n + M->slices - 1) / M->slices;
-    M->slice_threads = M->slice_len * std::max(1, 512 / M->slice_len);
-    const size_t lds = 8 * ((size_t)M->maxdep * A->n + M->slice_threads);
-    if (lds > 64 * 1024) {
+    // Column slices (multiples of 8 columns, so every slice starts 16-byte
+    // aligned for all element widths), threads per workgroup and loads in
+    // flight.  CDG_APPLY_SLICE_W / CDG_APPLY_THREADS / CDG_APPLY_UNROLL
+    // override the defaults (used by the tuning sweep).
+    int sw = 40, target_t = 256, unroll = 4;   // measured best on 16^3 P2 (n = 135)
+    if (const char *e = std::getenv("CDG_APPLY_SLICE_W")) sw = std::max(8, std::atoi(e));
+    if (const char *e = std::getenv("CDG_APPLY_THREADS")) target_t = std::max(64, std::atoi(e));
+    if (const char *e = std::getenv("CDG_APPLY_UNROLL")) unroll = std::atoi(e) == 8 ? 8 : 4;
+    M->unroll = unroll;
+    M->slices = std::max(1, (A->n + sw - 1) / sw);
+    M->slice_w = ((A->n + M->slices - 1) / M->slices + 7) / 8 * 8;
+    M->slices = (A->n + M->slice_w - 1) / M->slice_w;
+    M->shape[CDG_FP64] = stream_shape(A->n, 2, target_t);
+    M->shape[CDG_FP32] = stream_shape(A->n, 4, target_t);
+    M->shape[CDG_BF16] = stream_shape(A->n, 8, target_t);
+    const int V = precision == CDG_FP64 ? 2 : (precision == CDG_FP32 ? 4 : 8);
+    const size_t accsz = precision == CDG_FP64 ? 8 : 4;
+    const size_t lds = accsz * ((size_t)M->maxdep * M->slice_w + (size_t)V * M->shape[precision].T);
+    M->part_len = (int64_t)M->nrows * M->slices * M->n;
+    if (!e) e = alloc((void **)&M->d_part, sizeof(double) * 3 * (size_t)M->part_len);
+    if (e || lds > 64 * 1024 || M->shape[precision].T == 0) {
         cdg_prec_destroy(ctx, M);
-        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply would need %zu bytes of LDS", lds);
+        return e ? e : cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply needs %zu bytes of LDS", lds);
     }
     *out = M;
     return CDG_OK;

@@ -49,7 +49,9 @@ struct StreamShape {
     int T = 0;     // threads per workgroup (V*T is a multiple of n)
     int dc = 0;    // column advance per iteration = V*T/n
 };
-StreamShape stream_shape(int n, int V);
+// Launch shape of the fixed-row walk; T is a multiple of n/gcd(n,V) chosen
+// for lane utilization, preferring sizes near `target`.
+StreamShape stream_shape(int n, int V, int target);
 
 struct cdg_mat {
     int32_t nrows = 0, n = 0, nnzb = 0;
@@ -82,7 +84,8 @@ struct cdg_prec {
     int32_t *d_up_ptr = nullptr, *d_up_idx = nullptr, *d_up_col = nullptr, *d_up_blk = nullptr;
     int32_t *d_diag = nullptr;
     int32_t *d_err = nullptr;          // first singular row (factor)
-    double *d_t = nullptr;             // backward-sweep stage A result (nrows*n)
+    double *d_part = nullptr;          // apply partial sums: 3 arrays of part_len doubles
+    int64_t part_len = 0;              // nrows * slices * n
     // forward levels: rows grouped by level (factor and forward sweep); lower
     // blocks of those rows grouped the same way (factor phase 1)
     std::vector<int32_t> lev_lo_ptr, lev_row_ptr;
@@ -93,9 +96,10 @@ struct cdg_prec {
     int32_t *d_blev_rows = nullptr;
     int32_t nblev = 0;
     int32_t maxdep = 1;               // max lower/upper blocks of a row
-    // apply launch shape: each row's output is cut into `slices` pieces of
-    // `slice_len` rows; one workgroup of `slice_threads` threads per piece
-    int32_t slices = 1, slice_len = 1, slice_threads = 64;
+    // apply launch shape: each block's columns are cut into `slices` ranges of
+    // `slice_w` columns; shape[precision] is the fixed-row walk for that element
+    int32_t slices = 1, slice_w = 1, unroll = 4;
+    StreamShape shape[3];
     uint64_t stamp = 0;
     int32_t factored = 0;
     std::vector<DevMem> owned;

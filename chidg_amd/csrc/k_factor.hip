@@ -43,31 +43,68 @@ constexpr int KC = 16;       // K chunk staged in LDS
 constexpr int GW = 3;        // waves per workgroup
 constexpr int GT = 64 * GW;  // threads per workgroup
 
-// acc += A[tm-tile rows, :] * B[:, tn-tile cols]; A, B column-major n x n.
+constexpr int TMP = TM + 1;  // LDS row pitch: the padding spreads column-wise writes over the banks
+constexpr int LPT = KC * TM / GT;   // elements per thread per operand tile and chunk (4)
+
+// Loads chunk kc of the A strip (rows r0.., K = kc..kc+KC) and the B strip
+// (K = kc.., cols c0..) into registers, zero outside n.
+template <typename T, typename SA, typename SB>
+__device__ __forceinline__ void tile_fetch(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0, int c0,
+                                           int kc, T (&ra)[LPT], T (&rb)[LPT]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+        const int idx = tid + GT * i;
+        const int kk = idx / TM, rr = idx - kk * TM;
+        const int gr = r0 + rr, gk = kc + kk;
+        ra[i] = (gr < n && gk < n) ? static_cast<T>(A[(size_t)gk * n + gr]) : T(0);
+        const int cc = idx / KC, kb = idx - cc * KC;
+        const int gc = c0 + cc, gk2 = kc + kb;
+        rb[i] = (gc < n && gk2 < n) ? static_cast<T>(B[(size_t)gc * n + gk2]) : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_store(T (*As)[TMP], T (*Bs)[TMP], const T (&ra)[LPT], const T (&rb)[LPT]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+        const int idx = tid + GT * i;
+        const int kk = idx / TM, rr = idx - kk * TM;
+        As[kk][rr] = ra[i];
+        const int cc = idx / KC, kb = idx - cc * KC;
+        Bs[kb][cc] = rb[i];
+    }
+}
+
+// acc += A[r0.., :] * B[:, c0..] over the full K = n; A, B column-major n x n.
+// Two LDS buffers: chunk c+1 is fetched into registers while chunk c feeds
+// the MFMAs, so one barrier per chunk and the global latency is hidden.
 template <typename T, typename SA, typename SB>
 __device__ __forceinline__ void tile_mma(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0,
-                                         int c0, T (*As)[TM], T (*Bs)[TM], typename Mfma<T>::acc_t *acc) {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+                                         int c0, T (*As)[KC][TMP], T (*Bs)[KC][TMP], typename Mfma<T>::acc_t *acc) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    T ra[LPT], rb[LPT];
+    tile_fetch<T, SA, SB>(A, B, n, r0, c0, 0, ra, rb);
+    __syncthreads();   // the previous tile_mma call may still read buffer 0
+    tile_store<T>(As[0], Bs[0], ra, rb);
+    __syncthreads();
+    int cur = 0;
     for (int kc = 0; kc < n; kc += KC) {
-        for (int idx = tid; idx < KC * TM; idx += GT) {
-            const int kk = idx / TM, rr = idx - kk * TM;
-            const int gr = r0 + rr, gk = kc + kk;
-            As[kk][rr] = (gr < n && gk < n) ? static_cast<T>(A[(size_t)gk * n + gr]) : T(0);
-            const int cc = idx / KC, kb = idx - cc * KC;
-            const int gc = c0 + cc, gk2 = kc + kb;
-            Bs[kb][cc] = (gc < n && gk2 < n) ? static_cast<T>(B[(size_t)gc * n + gk2]) : T(0);
-        }
-        __syncthreads();
+        const bool more = kc + KC < n;
+        if (more) tile_fetch<T, SA, SB>(A, B, n, r0, c0, kc + KC, ra, rb);
 #pragma unroll
         for (int s = 0; s < KC / 4; ++s) {
-            const T a = As[4 * s + (lane >> 4)][16 * wv + (lane & 15)];
+            const T a = As[cur][4 * s + (lane >> 4)][16 * wv + (lane & 15)];
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
-                const T b = Bs[4 * s + (lane >> 4)][16 * t + (lane & 15)];
+                const T b = Bs[cur][4 * s + (lane >> 4)][16 * t + (lane & 15)];
                 acc[t] = Mfma<T>::run(a, b, acc[t]);
             }
         }
+        if (more) tile_store<T>(As[cur ^ 1], Bs[cur ^ 1], ra, rb);
         __syncthreads();
+        cur ^= 1;
     }
 }
 
@@ -77,7 +114,7 @@ __global__ void __launch_bounds__(GT)
 k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ jobs, const int32_t *__restrict__ lo_blk, const int32_t *__restrict__ lo_col,
          int n, int tiles) {
-    __shared__ T As[KC][TM], Bs[KC][TM];
+    __shared__ T As[2][KC][TMP], Bs[2][KC][TMP];
     const int q = jobs[blockIdx.y];
     const int tm = blockIdx.x % tiles, tn = blockIdx.x / tiles;
     typename Mfma<T>::acc_t acc[3];
@@ -102,7 +139,7 @@ __global__ void __launch_bounds__(GT)
 k_fac_p2(const double *__restrict__ Ab, int64_t lda, const T *__restrict__ L, T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ rows, const int32_t *__restrict__ lo_ptr, const int32_t *__restrict__ lo_tr,
          const int32_t *__restrict__ diag, int n, int tiles) {
-    __shared__ T As[KC][TM], Bs[KC][TM];
+    __shared__ T As[2][KC][TMP], Bs[2][KC][TMP];
     const int i = rows[blockIdx.y];
     const int tm = blockIdx.x % tiles, tn = blockIdx.x / tiles;
     typename Mfma<T>::acc_t acc[3];
@@ -123,36 +160,57 @@ k_fac_p2(const double *__restrict__ Ab, int64_t lda, const T *__restrict__ L, T 
         }
 }
 
+// DPP moves inside a 16-lane row: xor 1, xor 2 (quad permutes), rotate 4, 8.
+template <int CTRL> __device__ __forceinline__ int dpp_i32(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double x) {
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, int2{dpp_i32<CTRL>(v.x), dpp_i32<CTRL>(v.y)});
+}
+template <int CTRL> __device__ __forceinline__ float dpp(float x) {
+    return __builtin_bit_cast(float, dpp_i32<CTRL>(__builtin_bit_cast(int, x)));
+}
+
+// (value, index) max with the lower index on ties, and a min of `fu`, over the
+// 16 lanes of a DPP row; every lane of the row ends with the result.
+template <typename T, int CTRL>
+__device__ __forceinline__ void argmax_step(T &best, int &bi, int &fu) {
+    const T ob = dpp<CTRL>(best);
+    const int oi = dpp_i32<CTRL>(bi);
+    fu = min(fu, dpp_i32<CTRL>(fu));
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+}
+
 // phase 3: in-place inverse of D_i, one workgroup of 4 waves (one per SIMD)
 // per block.  Thread (tr, tc) of the 16 x 16 thread grid keeps rows
-// tr + 16a and columns tc + 16b (a, b < R) in registers.  Gauss-Jordan with
-// partial pivoting and two barriers per pivot step:
-//   1. the owners of column k publish it;
-//   2. every wave finds the pivot row p itself (same data, same deterministic
-//      reduction, so all waves agree; ties go to the lower row index) and the
-//      owners of row p publish it;
+// tr + 16a and columns tc + 16b (a, b < R) in registers; the 16 owners of a
+// column are one 16-lane DPP row.  Gauss-Jordan with partial pivoting and two
+// barriers per pivot step:
+//   1. the owners of column k publish it and reduce it among themselves
+//      (largest |a(r,k)| over unused rows, lower row index on ties);
+//   2. everyone reads the pivot row p; its owners publish it;
 //   3. everyone applies  S -= c_r * prow  (c_p = 0) and rewrites column k.
 // Rows are never swapped; the final scatter inv(rowstep[r], pivrow[m]) undoes
 // the implicit permutation.  The pivot row is not scaled in place: row r keeps
-// its unscaled values and the factor sigma_r = 1/pivot is applied when the
-// result is written (later eliminations are linear in the row, so they act on
-// the unscaled values consistently).  Register indices are compile-time: the
+// its unscaled values and sigma_r = 1/pivot is applied when the result is
+// written (later eliminations are linear in the row, so they act on the
+// unscaled values consistently).  Register indices are compile-time: the
 // pivot loop runs k = 16*kb + kt with kb unrolled, and the pivot row is
-// published through one LDS buffer per register row, so no register array is
-// indexed at run time.
+// published through one LDS buffer per register row.
 template <typename T, int R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err) {
     constexpr int TR = 16;
     constexpr int NMAX = TR * R;
-    constexpr int UR = (NMAX + 63) / 64;   // rows per lane in the pivot search
     __shared__ T colv[2][NMAX];
     __shared__ T rowv[2][R][NMAX];
     __shared__ T sig[NMAX];
     __shared__ int32_t pivrow[NMAX], rowstep[NMAX];
+    __shared__ int32_t piv_sh[2], bad_sh;
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
-    const int tid = threadIdx.x, tr = tid % TR, tc = tid / TR, lane = tid & 63;
+    const int tid = threadIdx.x, tr = tid % TR, tc = tid / TR;
     T S[R][R];
 #pragma unroll
     for (int a = 0; a < R; ++a)
@@ -161,12 +219,11 @@ k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n
             const int r = tr + TR * a, m = tc + TR * b;
             S[a][b] = (r < n && m < n) ? M[(size_t)m * n + r] : T(0);
         }
-    for (int i = tid; i < 2 * NMAX; i += 256) (&colv[0][0])[i] = T(0);
     for (int i = tid; i < 2 * R * NMAX; i += 256) (&rowv[0][0][0])[i] = T(0);
-    bool used[UR];
+    if (tid == 0) bad_sh = 0;
+    bool used[R];
 #pragma unroll
-    for (int u = 0; u < UR; ++u) used[u] = false;
-    int bad = 0;
+    for (int a = 0; a < R; ++a) used[a] = (tr + TR * a >= n);   // padding rows never pivot
     __syncthreads();
 #pragma unroll
     for (int kb = 0; kb < R; ++kb) {
@@ -174,36 +231,33 @@ k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n
         for (int kt = 0; kt < kt_end; ++kt) {
             const int k = kb * TR + kt;
             const int buf = k & 1;
-            if (tc == kt) {   // padding rows hold zeros, so no bounds test is needed
+            if (tc == kt) {
+                T best = T(-1);
+                int bi = 0x7fffffff, fu = 0x7fffffff;
 #pragma unroll
-                for (int a = 0; a < R; ++a) colv[buf][tr + TR * a] = S[a][kb];
-            }
-            __syncthreads();
-            // pivot: largest |a(r,k)| over unused rows, lowest index on ties
-            T best = T(-1);
-            int bi = 0x7fffffff, fu = 0x7fffffff;
-#pragma unroll
-            for (int u = 0; u < UR; ++u) {
-                const int r = lane + 64 * u;
-                if (r < n && !used[u]) {
-                    const T v = fabs(colv[buf][r]);
-                    if (v > best) { best = v; bi = r; }
-                    fu = min(fu, r);
+                for (int a = 0; a < R; ++a) {
+                    const int r = tr + TR * a;
+                    const T v = S[a][kb];
+                    colv[buf][r] = v;   // padding rows hold zeros
+                    if (!used[a]) {
+                        const T av = fabs(v);
+                        if (av > best) { best = av; bi = r; }
+                        fu = min(fu, r);
+                    }
+                }
+                argmax_step<T, 0xB1>(best, bi, fu);    // quad_perm [1,0,3,2]
+                argmax_step<T, 0x4E>(best, bi, fu);    // quad_perm [2,3,0,1]
+                argmax_step<T, 0x124>(best, bi, fu);   // row_ror:4
+                argmax_step<T, 0x128>(best, bi, fu);   // row_ror:8
+                if (tr == 0) {
+                    if (!(best > T(0)) || bi >= n) { bad_sh = 1; bi = fu; }   // zero or non-finite column
+                    piv_sh[buf] = bi;
                 }
             }
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(piv_sh[buf]);
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-                fu = min(fu, __shfl_xor(fu, off, 64));
-            }
-            int p = bi;
-            if (!(best > T(0)) || p >= n) { bad = 1; p = fu; }   // zero or non-finite pivot column
-            p = __builtin_amdgcn_readfirstlane(p);
-#pragma unroll
-            for (int u = 0; u < UR; ++u)
-                if (lane + 64 * u == p) used[u] = true;
+            for (int a = 0; a < R; ++a) used[a] = used[a] || (tr + TR * a == p);
             const int pa = p / TR, pt = p - pa * TR;
             const T rinv = T(1) / colv[buf][p];
             if (tid == 0) { pivrow[k] = p; rowstep[p] = k; sig[p] = rinv; }
@@ -234,7 +288,7 @@ k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n
         }
     }
     __syncthreads();
-    if (tid == 0 && bad) atomicMin(err, blk);
+    if (tid == 0 && bad_sh) atomicMin(err, blk);
 #pragma unroll
     for (int a = 0; a < R; ++a)
 #pragma unroll

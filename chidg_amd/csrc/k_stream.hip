@@ -29,10 +29,27 @@ template <> struct Ld16<double, double> {
         o[0] = t.x; o[1] = t.y;
     }
 };
-template <typename S, typename Acc> __device__ __forceinline__ Acc to_acc(S v) { return static_cast<Acc>(v); }
-template <> __device__ __forceinline__ float to_acc<cdg_bf16, float>(cdg_bf16 v) {
-    return __uint_as_float(static_cast<uint32_t>(v) << 16);
-}
+typedef float nat_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nat_u32x4 __attribute__((ext_vector_type(4)));
+template <> struct Ld16<float, float> {
+    static constexpr int V = 4;
+    __device__ static __forceinline__ void load(const float *p, float (&o)[4]) {
+        const nat_f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const nat_f32x4 *>(p));
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+    }
+};
+template <> struct Ld16<cdg_bf16, float> {
+    static constexpr int V = 8;
+    __device__ static __forceinline__ void load(const cdg_bf16 *p, float (&o)[8]) {
+        const nat_u32x4 t = __builtin_nontemporal_load(reinterpret_cast<const nat_u32x4 *>(p));
+        const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            o[2 * i] = __uint_as_float(w[i] << 16);
+            o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+        }
+    }
+};
 
 // Per-thread view of the flat element walk.
 template <int V> struct Walk {
@@ -52,19 +69,20 @@ template <int V> __device__ __forceinline__ Walk<V> make_walk(int tid, int T, in
     return w;
 }
 
-// acc[v] += sum over this thread's elements of blk(e) * xs(col(e)).
-template <typename S, typename Acc>
+// acc[v] += sum over this thread's elements of blk(e) * xs(col(e)); U loads
+// of 16 bytes in flight per thread.
+template <typename S, typename Acc, int U = 4>
 __device__ __forceinline__ void stream_block(const S *__restrict__ blk, const Acc *__restrict__ xs, int nn,
                                              const Walk<Ld16<S, Acc>::V> &w, Acc *acc) {
     constexpr int V = Ld16<S, Acc>::V;
     const int mfull = nn / w.step;           // steps where every element is in range
     int m = 0;
-    for (; m + 4 <= mfull; m += 4) {
-        Acc a[4][V];
+    for (; m + U <= mfull; m += U) {
+        Acc a[U][V];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) Ld16<S, Acc>::load(blk + w.e0 + (m + u) * w.step, a[u]);
+        for (int u = 0; u < U; ++u) Ld16<S, Acc>::load(blk + w.e0 + (m + u) * w.step, a[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int v = 0; v < V; ++v) acc[v] += a[u][v] * xs[w.c0[v] + (m + u) * w.dc];
     }
@@ -136,74 +154,96 @@ k_mv(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ 
 //   forward,  level l:  z_i = v_i - sum_j L_ij z_j            (rows of level l)
 //   backward, level l:  t_i = z_i - sum_k U_ik z_k   (stage A)
 //                       z_i = Dinv_i t_i             (stage B)
-// Rows of one wavefront level are independent, so a level is one launch whose
-// workgroups each own one (row, output slice) pair: the slice's rows of every
-// block are read column by column (a column's slice is contiguous), every
-// thread keeps one fixed output row and a strided set of columns, and the
-// partial sums meet in LDS in a fixed order.  No inter-workgroup waiting.
+// Rows of one wavefront level are independent, so a level is one launch.  Its
+// workgroups own (row, column slice) pairs: a slice of columns of a
+// column-major block is one contiguous range, streamed with the same
+// fixed-row 16-byte walk as the matvec.  Each workgroup writes the partial
+// n-vector of its slice; vectors are kept in the implicit form
+//     u = base + sign1 * sum_s part1[s] + sign2 * sum_s part2[s]
+// and rebuilt by their consumers (only the consumer's column slice) in a
+// fixed order, so results are deterministic.  A last elementwise pass
+// materializes z.
 // ============================================================================
-struct SliceArgs {
+struct VecView {
+    const double *base;       // may be null (0)
+    const double *p1, *p2;    // partial arrays [nrows][S][n], may be null
+    double s1, s2;            // signs of the partial sums
+};
+
+__device__ __forceinline__ double view_at(const VecView &u, int row, int c, int n, int S) {
+    double x = u.base ? u.base[(size_t)row * n + c] : 0.0;
+    if (u.p1) {
+        double t = 0.0;
+        for (int s = 0; s < S; ++s) t += u.p1[((size_t)row * S + s) * n + c];
+        x += u.s1 * t;
+    }
+    if (u.p2) {
+        double t = 0.0;
+        for (int s = 0; s < S; ++s) t += u.p2[((size_t)row * S + s) * n + c];
+        x += u.s2 * t;
+    }
+    return x;
+}
+
+struct KSliceArgs {
     const void *B;            // block storage (L, U, or Dinv)
     int64_t ld;               // elements per block slot
     const int32_t *rows;      // rows of this level
-    const int32_t *ptr;       // per-row block range (lower or upper lists); null: the row's own block
+    const int32_t *ptr;       // per-row entry range (lower or upper lists); null: the row's own block
     const int32_t *idx;       // block slot of entry d (null: slot = d)
     const int32_t *col;       // parent row of entry d
-    const double *x;          // vector the parents are read from
-    const double *base;       // result = base_i - sum (null: result = sum)
-    double *out;              // result vector
-    int n, S, len;            // block size, slices per row, slice length
-    int maxdep;
+    VecView x;                // vector the parents' slices are read from
+    double *part;             // output partials [nrows][S][n]
+    int n, S, w, maxdep, dc;  // block size, slices, slice width (columns), max entries, walk step
 };
 
-template <typename S, typename Acc>
-__global__ void __launch_bounds__(kMaxThreads) k_slice_gemv(SliceArgs a) {
+template <typename S, typename Acc, int U>
+__global__ void __launch_bounds__(kMaxThreads) k_kslice(KSliceArgs a) {
+    constexpr int V = Ld16<S, Acc>::V;
     extern __shared__ unsigned char smem_raw[];
-    const int T = blockDim.x, tid = threadIdx.x, n = a.n, len = a.len;
-    Acc *xs = reinterpret_cast<Acc *>(smem_raw);            // maxdep * n
-    Acc *red = xs + (size_t)a.maxdep * n;                    // T
+    const int T = blockDim.x, tid = threadIdx.x, n = a.n;
+    Acc *xs = reinterpret_cast<Acc *>(smem_raw);             // maxdep * w
+    Acc *red = xs + (size_t)a.maxdep * a.w;                   // V*T
     const int row = a.rows[blockIdx.x / a.S];
-    const int slice = blockIdx.x - (blockIdx.x / a.S) * a.S;
-    const int r0 = slice * len;
-    const int r1 = min(n, r0 + len);
+    const int sl = blockIdx.x - (blockIdx.x / a.S) * a.S;
+    const int c0 = sl * a.w;
+    const int c1 = min(n, c0 + a.w);
+    const int wc = c1 - c0;
     int d0 = 0, d1 = 1;
     if (a.ptr) { d0 = a.ptr[row]; d1 = a.ptr[row + 1]; }
     const int nd = d1 - d0;
-    for (int e = tid; e < nd * n; e += T) {
-        const int k = e / n, c = e - k * n;
+    for (int e = tid; e < nd * wc; e += T) {
+        const int k = e / wc, c = e - k * wc;
         const int parent = a.ptr ? a.col[d0 + k] : row;
-        xs[e] = static_cast<Acc>(a.x[(size_t)parent * n + c]);
+        xs[k * a.w + c] = static_cast<Acc>(view_at(a.x, parent, c0 + c, n, a.S));
     }
     __syncthreads();
-    const int rr = tid % len, cg = tid / len, ncg = T / len;
+    const Walk<V> wk = make_walk<V>(tid, T, n, a.dc);
     const S *Bp = static_cast<const S *>(a.B);
-    Acc acc = 0;
-    if (cg < ncg && r0 + rr < r1) {
-        for (int k = 0; k < nd; ++k) {
-            const int slot = a.ptr ? (a.idx ? a.idx[d0 + k] : d0 + k) : row;
-            const S *blk = Bp + (size_t)slot * a.ld + r0 + rr;
-            const Acc *xk = xs + k * n;
-            int c = cg;
-            for (; c + 3 * ncg < n; c += 4 * ncg) {
-                const Acc b0 = to_acc<S, Acc>(blk[(size_t)c * n]);
-                const Acc b1 = to_acc<S, Acc>(blk[(size_t)(c + ncg) * n]);
-                const Acc b2 = to_acc<S, Acc>(blk[(size_t)(c + 2 * ncg) * n]);
-                const Acc b3 = to_acc<S, Acc>(blk[(size_t)(c + 3 * ncg) * n]);
-                acc += b0 * xk[c];
-                acc += b1 * xk[c + ncg];
-                acc += b2 * xk[c + 2 * ncg];
-                acc += b3 * xk[c + 3 * ncg];
-            }
-            for (; c < n; c += ncg) acc += to_acc<S, Acc>(blk[(size_t)c * n]) * xk[c];
-        }
+    Acc acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
+    // the slice is the flat range [c0*n, c1*n) of each block; column index
+    // inside the slice = global column - c0, so xs is read at (col - c0)
+    for (int k = 0; k < nd; ++k) {
+        const int slot = a.ptr ? (a.idx ? a.idx[d0 + k] : d0 + k) : row;
+        stream_block<S, Acc, U>(Bp + (size_t)slot * a.ld + (size_t)c0 * n, xs + k * a.w, wc * n, wk, acc);
     }
-    red[tid] = acc;
+#pragma unroll
+    for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
     __syncthreads();
-    if (tid < r1 - r0) {
-        Acc s = 0;
-        for (int g = 0; g < ncg; ++g) s += red[g * len + tid];
-        const size_t o = (size_t)row * n + r0 + tid;
-        a.out[o] = a.base ? a.base[o] - static_cast<double>(s) : static_cast<double>(s);
+    const int parts = V * T / n;
+    double *out = a.part + ((size_t)row * a.S + sl) * n;
+    for (int r = tid; r < n; r += T) out[r] = static_cast<double>(row_sum(red, r, n, parts));
+}
+
+// z = sum_s part[s]  (materializes the backward result)
+__global__ void k_sum_parts(const double *__restrict__ part, int S, int n, int64_t len, double *__restrict__ z) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / n, c = e - row * n;
+        double t = 0.0;
+        for (int s = 0; s < S; ++s) t += part[(row * S + s) * n + c];
+        z[e] = t;
     }
 }
 
@@ -222,23 +262,23 @@ __global__ void k_widen<cdg_bf16>(const cdg_bf16 *__restrict__ src, double *__re
 }  // namespace
 
 // ---- launch shape -------------------------------------------------------------
-StreamShape stream_shape(int n, int V) {
+StreamShape stream_shape(int n, int V, int target) {
     int g = n;
-    {   // g = n / gcd(n, V)
+    {   // g = n / gcd(n, V): T must be a multiple of g for V*T % n == 0
         int a = n, b = V;
         while (b) { int t = a % b; a = b; b = t; }
         g = n / a;
     }
     StreamShape best;
-    double best_score = -1.0;
+    double best_score = -1e30;
     for (int T = g; T <= kMaxThreads; T += g) {
         if (T < 64 && T + g <= kMaxThreads) continue;
         const int waves = (T + 63) / 64;
         const double util = (double)T / (64.0 * waves);
-        const double score = util - 1e-4 * std::abs(T - 640) / 64.0;
+        const double score = util - 0.03 * std::abs(T - target) / 64.0;
         if (score > best_score) { best_score = score; best.T = T; }
     }
-    if (best.T == 0) best.T = g <= kMaxThreads ? g : 0;
+    if (best.T == 0 && g <= kMaxThreads) best.T = g;
     best.dc = best.T ? V * best.T / n : 0;
     return best;
 }
@@ -259,15 +299,20 @@ int launch_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y) {
 namespace {
 
 template <typename S, typename Acc>
-int launch_slice(cdg_ctx *ctx, const cdg_prec *M, SliceArgs a, int nrows_lvl) {
+int launch_kslice(cdg_ctx *ctx, const cdg_prec *M, KSliceArgs a, int nrows_lvl) {
     if (nrows_lvl == 0) return CDG_OK;
+    constexpr int V = Ld16<S, Acc>::V;
+    const StreamShape &sh = M->shape[M->precision];
     a.n = M->n;
     a.S = M->slices;
-    a.len = M->slice_len;
+    a.w = M->slice_w;
     a.maxdep = M->maxdep;
-    const int T = M->slice_threads;
-    const size_t lds = sizeof(Acc) * ((size_t)a.maxdep * M->n + T);
-    hipLaunchKernelGGL((k_slice_gemv<S, Acc>), dim3(nrows_lvl * M->slices), dim3(T), lds, ctx->stream, a);
+    a.dc = sh.dc;
+    const size_t lds = sizeof(Acc) * ((size_t)a.maxdep * a.w + (size_t)V * sh.T);
+    if (M->unroll == 8)
+        hipLaunchKernelGGL((k_kslice<S, Acc, 8>), dim3(nrows_lvl * M->slices), dim3(sh.T), lds, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_kslice<S, Acc, 4>), dim3(nrows_lvl * M->slices), dim3(sh.T), lds, ctx->stream, a);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
@@ -276,30 +321,38 @@ template <typename S, typename Acc>
 int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
     const void *U = M->precision == CDG_FP64 ? static_cast<const void *>(A->blocks) : M->U;
     const int64_t ldu = M->precision == CDG_FP64 ? A->ld : M->ld;
+    double *P = M->d_part, *QA = M->d_part + M->part_len, *PB = M->d_part + 2 * M->part_len;
+    const VecView zf{v, P, nullptr, -1.0, 0.0};      // forward result  v - sum P
+    const VecView zb{nullptr, PB, nullptr, 1.0, 0.0};  // backward result sum PB
+    const VecView t{v, P, QA, -1.0, -1.0};            // stage A result v - sum P - sum QA
     for (int l = 0; l < M->nlev; ++l) {
         const int r0 = M->lev_row_ptr[l], r1 = M->lev_row_ptr[l + 1];
-        SliceArgs f{};
+        KSliceArgs f{};
         f.B = M->L; f.ld = M->ld; f.rows = M->d_lev_rows + r0;
         f.ptr = M->d_lo_ptr; f.idx = nullptr; f.col = M->d_lo_col;
-        f.x = z; f.base = v; f.out = z;
-        int e = launch_slice<S, Acc>(ctx, M, f, r1 - r0);
+        f.x = zf; f.part = P;
+        int e = launch_kslice<S, Acc>(ctx, M, f, r1 - r0);
         if (e) return e;
     }
     for (int l = 0; l < M->nblev; ++l) {
         const int r0 = M->blev_row_ptr[l], r1 = M->blev_row_ptr[l + 1];
-        SliceArgs u{};
+        KSliceArgs u{};
         u.B = U; u.ld = ldu; u.rows = M->d_blev_rows + r0;
         u.ptr = M->d_up_ptr; u.idx = M->d_up_idx; u.col = M->d_up_col;
-        u.x = z; u.base = z; u.out = M->d_t;
-        int e = launch_slice<S, Acc>(ctx, M, u, r1 - r0);
+        u.x = zb; u.part = QA;
+        int e = launch_kslice<S, Acc>(ctx, M, u, r1 - r0);
         if (e) return e;
-        SliceArgs d{};
+        KSliceArgs d{};
         d.B = M->Dinv; d.ld = M->ld; d.rows = M->d_blev_rows + r0;
         d.ptr = nullptr; d.idx = nullptr; d.col = nullptr;
-        d.x = M->d_t; d.base = nullptr; d.out = z;
-        e = launch_slice<S, Acc>(ctx, M, d, r1 - r0);
+        d.x = t; d.part = PB;
+        e = launch_kslice<S, Acc>(ctx, M, d, r1 - r0);
         if (e) return e;
     }
+    const int64_t len = (int64_t)M->nrows * M->n;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<int64_t>((len + 255) / 256, 4096)), dim3(256), 0,
+                       ctx->stream, PB, M->slices, M->n, len, z);
+    CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
 
@@ -307,7 +360,6 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
 
 int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
     if (M->nrows == 0) return CDG_OK;
-    if (v == z) return cdg_fail(ctx, CDG_ERR_ARG, "ILU apply needs distinct input and output vectors");
     if (M->precision == CDG_FP64) return sweep<double, double>(ctx, M, A, v, z);
     if (M->precision == CDG_FP32) return sweep<float, float>(ctx, M, A, v, z);
     return sweep<cdg_bf16, float>(ctx, M, A, v, z);
