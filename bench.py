@@ -163,11 +163,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kreps", type=int, default=20, help="kernel timing repetitions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one system per GPU; "
               "sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0")   # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 512 MiB device-to-device (read 512 MiB + write 512 MiB each)")
+        del src, dst
     p, g = gpu_run(args)
     f32, f64 = g["fp32"], g["fp64"]
     cpu = None

@@ -1,0 +1,306 @@
+!> Fortran binding of the gfx950 linear-solve library (include/chidg_hip.h).
+!!
+!! This is the iso_c_binding layer a ChiDG build with backend 'hip' uses in
+!! place of the native containers on the Newton-Krylov linear solve:
+!!   hip_matrix_t   <- chidg_matrix_t   (src/containers/type_chidg_matrix.f90:38-83)
+!!   hip_mv         <- chidg_mv         (src/containers/operator_chidg_mv.f90:46-61)
+!!   hip_precon_t   <- precon_ILU0_t / precon_identity_t
+!!                     init/update/apply/tear_down (src/preconditioners/precon_ILU0.f90:64-470)
+!!   hip_fgmres_t   <- fgmres_t with solve(A,x,b,M) (src/linear_solvers/type_fgmres.f90:81-310)
+!! Blocks are the reference's densematrix_t%mat: n x n column-major real(8);
+!! vectors are contiguous element-major real(8) arrays.  Every call aborts
+!! through hip_check on a non-zero status, the way chidg_signal(FATAL, ...)
+!! ends the reference (src/tools/messenger.f90:295-304).
+module mod_chidg_hip
+    use iso_c_binding
+    implicit none
+    private
+
+    integer(c_int32_t), parameter, public :: HIP_KIND_LOCAL = 0, HIP_KIND_COUPLING = 1
+    integer(c_int32_t), parameter, public :: HIP_PREC_IDENTITY = 0, HIP_PREC_ILU0 = 1
+    integer(c_int32_t), parameter, public :: HIP_FP64 = 0, HIP_FP32 = 1, HIP_BF16 = 2
+    integer(c_int32_t), parameter, public :: HIP_CGS = 0, HIP_MGS = 1
+
+    ! cdg_fgmres_params / cdg_fgmres_stats (same layout as the C structs)
+    type, bind(c), public :: hip_fgmres_params_t
+        integer(c_int32_t) :: nkrylov
+        real(c_double)     :: tol
+        real(c_double)     :: rtol
+        integer(c_int32_t) :: nmax
+        integer(c_int32_t) :: orthogonalization
+        integer(c_int32_t) :: inner_fgmres
+        integer(c_int32_t) :: inner_nkrylov
+        real(c_double)     :: inner_tol
+        real(c_double)     :: inner_rtol
+        integer(c_int32_t) :: inner_nmax
+        integer(c_int32_t) :: inner_orthogonalization
+        integer(c_int32_t) :: update_preconditioner
+    end type hip_fgmres_params_t
+
+    type, bind(c), public :: hip_fgmres_stats_t
+        integer(c_int32_t) :: niter
+        real(c_double)     :: residual_norm
+        integer(c_int32_t) :: nrestart
+        integer(c_int32_t) :: nreorth
+        integer(c_int32_t) :: factored
+        real(c_double)     :: t_total, t_factor, t_precon, t_mv, t_dot, t_norm
+    end type hip_fgmres_stats_t
+
+    interface
+        integer(c_int) function cdg_ctx_create(device, ctx) bind(c, name='cdg_ctx_create')
+            import :: c_int, c_int32_t, c_ptr
+            integer(c_int32_t), value :: device
+            type(c_ptr) :: ctx
+        end function
+        integer(c_int) function cdg_ctx_destroy(ctx) bind(c, name='cdg_ctx_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        type(c_ptr) function cdg_last_error(ctx) bind(c, name='cdg_last_error')
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function cdg_mat_create(ctx, nrows, n, row_ptr, col, kind, mat) bind(c, name='cdg_mat_create')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nrows, n
+            integer(c_int32_t), intent(in) :: row_ptr(*), col(*), kind(*)
+            type(c_ptr) :: mat
+        end function
+        integer(c_int) function cdg_mat_destroy(ctx, mat) bind(c, name='cdg_mat_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, mat
+        end function
+        integer(c_int) function cdg_mat_upload(ctx, mat, blocks) bind(c, name='cdg_mat_upload')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, mat
+            real(c_double), intent(in) :: blocks(*)
+        end function
+        integer(c_int) function cdg_mv(ctx, mat, x, y) bind(c, name='cdg_mv')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, mat
+            real(c_double), intent(in) :: x(*)
+            real(c_double), intent(inout) :: y(*)
+        end function
+        integer(c_int) function cdg_prec_create(ctx, mat, ptype, precision, prec) bind(c, name='cdg_prec_create')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx, mat
+            integer(c_int32_t), value :: ptype, precision
+            type(c_ptr) :: prec
+        end function
+        integer(c_int) function cdg_prec_update(ctx, prec, mat, bad_row) bind(c, name='cdg_prec_update')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx, prec, mat
+            integer(c_int32_t) :: bad_row
+        end function
+        integer(c_int) function cdg_prec_apply(ctx, prec, mat, v, z) bind(c, name='cdg_prec_apply')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, prec, mat
+            real(c_double), intent(in) :: v(*)
+            real(c_double), intent(inout) :: z(*)
+        end function
+        integer(c_int) function cdg_prec_destroy(ctx, prec) bind(c, name='cdg_prec_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, prec
+        end function
+        integer(c_int) function cdg_prec_download(ctx, prec, L, Dinv) bind(c, name='cdg_prec_download')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, prec
+            real(c_double), intent(inout) :: L(*), Dinv(*)
+        end function
+        integer(c_int) function cdg_fgmres_solve(ctx, mat, prec, b, x, params, stats) bind(c, name='cdg_fgmres_solve')
+            import :: c_int, c_ptr, c_double, hip_fgmres_params_t, hip_fgmres_stats_t
+            type(c_ptr), value :: ctx, mat, prec
+            real(c_double), intent(in) :: b(*)
+            real(c_double), intent(inout) :: x(*)
+            type(hip_fgmres_params_t), intent(in) :: params
+            type(hip_fgmres_stats_t), intent(inout) :: stats
+        end function
+        subroutine cdg_fgmres_default_params(params) bind(c, name='cdg_fgmres_default_params')
+            import :: hip_fgmres_params_t
+            type(hip_fgmres_params_t), intent(inout) :: params
+        end subroutine
+        integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function
+    end interface
+
+    !> One device + stream (cdg_ctx).
+    type, public :: hip_context_t
+        type(c_ptr) :: h = c_null_ptr
+    contains
+        procedure :: init => ctx_init
+        procedure :: release => ctx_release
+    end type
+
+    !> Block-sparse matrix resident on the device (chidg_matrix_t, backend 'hip').
+    type, public :: hip_matrix_t
+        type(c_ptr) :: h = c_null_ptr
+        type(c_ptr) :: ctx = c_null_ptr
+        integer(c_int32_t) :: nrows = 0, n = 0, nnzb = 0
+    contains
+        procedure :: init => mat_init
+        procedure :: store => mat_store       ! upload + stamp: chidg_matrix%assemble
+        procedure :: release => mat_release
+    end type
+
+    !> Preconditioner (precon_ILU0_t / precon_identity_t).
+    type, public :: hip_precon_t
+        type(c_ptr) :: h = c_null_ptr
+        type(c_ptr) :: ctx = c_null_ptr
+        integer(c_int32_t) :: ptype = HIP_PREC_ILU0
+        integer(c_int32_t) :: precision = HIP_FP64
+        logical :: initialized = .false.
+    contains
+        procedure :: init => prec_init
+        procedure :: update => prec_update
+        procedure :: apply => prec_apply
+        procedure :: tear_down => prec_tear_down
+    end type
+
+    !> fgmres_t: parameters as in linear_solver_t (type_linear_solver.f90:21-40).
+    type, public :: hip_fgmres_t
+        type(hip_fgmres_params_t) :: params
+        integer(c_int32_t) :: niter = 0
+        real(c_double) :: residual_norm = 0.d0
+        type(hip_fgmres_stats_t) :: stats
+    contains
+        procedure :: init => fgmres_init
+        procedure :: solve => fgmres_solve
+    end type
+
+    public :: hip_mv, hip_check
+
+contains
+
+    subroutine hip_check(status, what, ctx)
+        integer(c_int), intent(in) :: status
+        character(*), intent(in) :: what
+        type(c_ptr), intent(in) :: ctx
+        character(len=512) :: msg
+        type(c_ptr) :: cmsg
+        character(kind=c_char), pointer :: chars(:)
+        integer :: i, nchar
+        if (status == 0) return
+        msg = ''
+        if (c_associated(ctx)) then
+            cmsg = cdg_last_error(ctx)
+            nchar = int(c_strlen(cmsg))
+            call c_f_pointer(cmsg, chars, [max(nchar, 1)])
+            do i = 1, min(nchar, len(msg))
+                msg(i:i) = chars(i)
+            end do
+        end if
+        write(*, '(a, a, a, i0, a, a)') 'FATAL: ', what, ' failed with status ', status, ': ', trim(msg)
+        error stop 1
+    end subroutine hip_check
+
+    subroutine ctx_init(self, device)
+        class(hip_context_t), intent(inout) :: self
+        integer, intent(in), optional :: device
+        integer(c_int32_t) :: dev
+        dev = 0
+        if (present(device)) dev = int(device, c_int32_t)
+        call hip_check(cdg_ctx_create(dev, self%h), 'cdg_ctx_create', c_null_ptr)
+    end subroutine
+
+    subroutine ctx_release(self)
+        class(hip_context_t), intent(inout) :: self
+        integer(c_int) :: st
+        if (c_associated(self%h)) st = cdg_ctx_destroy(self%h)
+        self%h = c_null_ptr
+    end subroutine
+
+    !> Pattern in ChiDG row order: per block row the lblks entries then the
+    !! coupling-only entries; row_ptr is 0-based (C), col holds 0-based parents.
+    subroutine mat_init(self, ctx, nrows, n, row_ptr, col, kind)
+        class(hip_matrix_t), intent(inout) :: self
+        type(hip_context_t), intent(in) :: ctx
+        integer, intent(in) :: nrows, n
+        integer(c_int32_t), intent(in) :: row_ptr(:), col(:), kind(:)
+        self%ctx = ctx%h
+        self%nrows = int(nrows, c_int32_t)
+        self%n = int(n, c_int32_t)
+        self%nnzb = row_ptr(nrows + 1)
+        call hip_check(cdg_mat_create(ctx%h, self%nrows, self%n, row_ptr, col, kind, self%h), 'cdg_mat_create', ctx%h)
+    end subroutine
+
+    subroutine mat_store(self, blocks)
+        class(hip_matrix_t), intent(inout) :: self
+        real(c_double), intent(in) :: blocks(:)
+        call hip_check(cdg_mat_upload(self%ctx, self%h, blocks), 'cdg_mat_upload', self%ctx)
+    end subroutine
+
+    subroutine mat_release(self)
+        class(hip_matrix_t), intent(inout) :: self
+        integer(c_int) :: st
+        if (c_associated(self%h)) st = cdg_mat_destroy(self%ctx, self%h)
+        self%h = c_null_ptr
+    end subroutine
+
+    !> y = A x (chidg_mv)
+    function hip_mv(A, x) result(y)
+        type(hip_matrix_t), intent(in) :: A
+        real(c_double), intent(in) :: x(:)
+        real(c_double), allocatable :: y(:)
+        allocate(y(size(x)))
+        call hip_check(cdg_mv(A%ctx, A%h, x, y), 'cdg_mv', A%ctx)
+    end function
+
+    subroutine prec_init(self, A, ptype, precision)
+        class(hip_precon_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        integer, intent(in), optional :: ptype, precision
+        if (present(ptype)) self%ptype = int(ptype, c_int32_t)
+        if (present(precision)) self%precision = int(precision, c_int32_t)
+        self%ctx = A%ctx
+        call hip_check(cdg_prec_create(A%ctx, A%h, self%ptype, self%precision, self%h), 'cdg_prec_create', A%ctx)
+        self%initialized = .true.
+    end subroutine
+
+    subroutine prec_update(self, A)
+        class(hip_precon_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        integer(c_int32_t) :: bad
+        call hip_check(cdg_prec_update(A%ctx, self%h, A%h, bad), 'cdg_prec_update', A%ctx)
+    end subroutine
+
+    function prec_apply(self, A, v) result(z)
+        class(hip_precon_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        real(c_double), intent(in) :: v(:)
+        real(c_double), allocatable :: z(:)
+        allocate(z(size(v)))
+        call hip_check(cdg_prec_apply(A%ctx, self%h, A%h, v, z), 'cdg_prec_apply', A%ctx)
+    end function
+
+    subroutine prec_tear_down(self)
+        class(hip_precon_t), intent(inout) :: self
+        integer(c_int) :: st
+        if (c_associated(self%h)) st = cdg_prec_destroy(self%ctx, self%h)
+        self%h = c_null_ptr
+        self%initialized = .false.
+    end subroutine
+
+    subroutine fgmres_init(self)
+        class(hip_fgmres_t), intent(inout) :: self
+        call cdg_fgmres_default_params(self%params)
+    end subroutine
+
+    !> solve(A,x,b,M): x overwritten (zero initial guess); M updated first
+    !! unless the controller semantics (params%update_preconditioner = 0)
+    !! find its factor current.
+    subroutine fgmres_solve(self, A, x, b, M)
+        class(hip_fgmres_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        real(c_double), intent(inout) :: x(:)
+        real(c_double), intent(in) :: b(:)
+        type(hip_precon_t), intent(inout) :: M
+        if (.not. M%initialized) call M%init(A)
+        call hip_check(cdg_fgmres_solve(A%ctx, A%h, M%h, b, x, self%params, self%stats), 'cdg_fgmres_solve', A%ctx)
+        self%niter = self%stats%niter
+        self%residual_norm = self%stats%residual_norm
+    end subroutine
+
+end module mod_chidg_hip
