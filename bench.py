@@ -33,6 +33,12 @@ NFIELDS = 5
 SEED_A = 20261202
 SEED_B = 1
 HBM_PEAK_GBS = 8000.0
+# The FP32 ILU(0) apply, one call = 138 k_kslice launches + 1 k_sum_parts.
+APPLY_KERNEL = "ILU(0) apply FP32: 138 x k_kslice<float,float,4> + k_sum_parts (one call)"
+# HBM bytes per apply call from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), the
+# factors being what the same counters read for a 512 MiB device copy
+# (profiles/hbm_traffic_16x16x16_p2.md).
+APPLY_TRAFFIC_FP32 = 2.236e9
 
 
 def sig(v, digits=5):
@@ -185,10 +191,11 @@ def main():
     if not args.no_cpu:
         cpu, x_cpu, n_cpu = cpu_baseline(p, f64["niter"])
     roof_ms = f32["apply_ms"] / 1e3
-    roof = {"bound": "hbm", "kernel": "k_ilu_sweep<float,float> (FP32 ILU(0) apply)",
+    roof = {"bound": "hbm", "kernel": APPLY_KERNEL,
             "achieved": round(f32["apply_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(f32["apply_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None, "algorithmic_bytes": ilu_apply_bytes(p, 4), "avg_launch_ms": round(1e3 * roof_ms, 4)}
+            "traffic": APPLY_TRAFFIC_FP32, "algorithmic_bytes": ilu_apply_bytes(p, 4),
+            "avg_launch_ms": round(1e3 * roof_ms, 4)}
     line = {
         "metric": "P2 16^3 Newton-Krylov linear solves/s (FP32-ILU vs FP64), ILU apply GB/s, p99 latency",
         "value": round(f32["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
