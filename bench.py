@@ -92,7 +92,7 @@ def gpu_run(args):
     bnorm = float(torch.linalg.norm(b))
     out = {}
     for prec in ("fp32", "fp64"):
-        M = precon_ILU0(prec).init(A)
+        M = precon_ILU0(prec, schedule=args.schedule).init(A)
         ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False, orthogonalization="CGS")
         x = torch.zeros_like(b)
         for _ in range(args.warmup):
@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kreps", type=int, default=20, help="kernel timing repetitions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--schedule", choices=["levels", "dataflow"], default=None,
+                    help="ILU(0) apply schedule (default: the library's)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()

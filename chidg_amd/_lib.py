@@ -24,6 +24,7 @@ EXPORTS = [
     "cdg_mat_create", "cdg_mat_destroy", "cdg_mat_upload", "cdg_mat_download",
     "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
     "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
+    "cdg_prec_set_schedule",
     "cdg_fgmres_solve", "cdg_fgmres_default_params",
 ]
 

@@ -392,6 +392,45 @@ extern "C" int cdg_mv_async(cdg_ctx *ctx, const cdg_mat *A, const double *x, dou
 // ---------------------------------------------------------------------------
 // preconditioner
 // ---------------------------------------------------------------------------
+// Task list, counters and partial storage of the dataflow sweep (once per
+// preconditioner).  Tickets: forward levels ascending; per backward level its
+// BA tasks then its BB tasks, so every input of a task has a smaller ticket.
+static int df_setup(cdg_ctx *ctx, cdg_prec *M) {
+    if (M->d_tasks) return CDG_OK;
+    int e = df_configure(ctx, M);
+    if (e) return e;
+    const int S = M->df_S;
+    std::vector<int32_t> tasks;
+    tasks.reserve((size_t)3 * M->nrows * S);
+    for (int l = 0; l < M->nlev; ++l)
+        for (int k = M->lev_row_ptr[l]; k < M->lev_row_ptr[l + 1]; ++k)
+            for (int sl = 0; sl < S; ++sl) tasks.push_back(((M->h_lev_rows[k] * S + sl) << 2) | 0);
+    for (int l = 0; l < M->nblev; ++l)
+        for (int stg = 1; stg <= 2; ++stg)
+            for (int k = M->blev_row_ptr[l]; k < M->blev_row_ptr[l + 1]; ++k)
+                for (int sl = 0; sl < S; ++sl) tasks.push_back(((M->h_blev_rows[k] * S + sl) << 2) | stg);
+    M->df_ntasks = (int32_t)tasks.size();
+    const int64_t need = 3 * (int64_t)M->nrows * S * M->n;
+    if (need > 3 * M->part_len) {      // partial storage shared with the level schedule
+        double *np = nullptr;
+        e = dev_alloc(ctx, (void **)&np, sizeof(double) * (size_t)need);
+        if (e) return e;
+        for (DevMem &m : M->owned)
+            if (m.p == M->d_part) { dev_free(ctx, m.p, m.bytes); m.p = np; m.bytes = sizeof(double) * (size_t)need; }
+        M->d_part = np;
+        M->part_len = need / 3;
+    }
+    e = upload_array(ctx, &M->d_tasks, tasks, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_cnt, std::vector<int32_t>((size_t)3 * M->nrows, 0), &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_dferr, std::vector<int32_t>(1, 0), &M->owned);
+    if (!e) {
+        e = dev_alloc(ctx, (void **)&M->d_ticket, sizeof(unsigned long long));
+        if (!e) M->owned.push_back({M->d_ticket, sizeof(unsigned long long)});
+    }
+    if (!e) e = df_reset(ctx, M);
+    return e;
+}
+
 extern "C" int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M) {
     if (!M) return CDG_OK;
     if (ctx) {
@@ -498,13 +537,55 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     const size_t accsz = precision == CDG_FP64 ? 8 : 4;
     const size_t lds = accsz * ((size_t)M->maxdep * M->slice_w + (size_t)V * M->shape[precision].T);
     M->part_len = (int64_t)M->nrows * M->slices * M->n;
-    if (!e) e = alloc((void **)&M->d_part, sizeof(double) * 3 * (size_t)M->part_len);
     if (e || lds > 64 * 1024 || M->shape[precision].T == 0) {
         cdg_prec_destroy(ctx, M);
         return e ? e : cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply needs %zu bytes of LDS", lds);
     }
+    // apply schedule (CDG_APPLY_MODE=levels|dataflow overrides the default)
+    M->apply_mode = CDG_APPLY_LEVELS;
+    if (const char *m = std::getenv("CDG_APPLY_MODE")) {
+        if (!std::strcmp(m, "dataflow")) M->apply_mode = CDG_APPLY_DATAFLOW;
+        else if (!std::strcmp(m, "levels")) M->apply_mode = CDG_APPLY_LEVELS;
+    }
+    M->h_lev_rows = forder;
+    M->h_blev_rows = border;
+    const int64_t part_doubles = 3 * M->part_len;
+    if (!e) e = alloc((void **)&M->d_part, sizeof(double) * (size_t)part_doubles);
+    if (!e && M->apply_mode == CDG_APPLY_DATAFLOW) e = df_setup(ctx, M);
+    if (!e) {
+        hipError_t he = hipStreamSynchronize(ctx->stream);
+        if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
+    }
+    if (e) { cdg_prec_destroy(ctx, M); return e; }
     *out = M;
     return CDG_OK;
+}
+
+extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule) {
+    if (!ctx || !M) return CDG_ERR_ARG;
+    if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW)
+        return cdg_fail(ctx, CDG_ERR_ARG, "unknown apply schedule %d", schedule);
+    if (M->type != CDG_PREC_ILU0) return CDG_OK;
+    if (schedule == CDG_SCHEDULE_DATAFLOW) {
+        int e = df_setup(ctx, M);
+        if (e) return e;
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW : CDG_APPLY_LEVELS;
+    return CDG_OK;
+}
+
+// A dataflow sweep that gave up waiting leaves its counters unusable: report
+// it and reset them (called from the synchronizing entry points).
+static int df_check(cdg_ctx *ctx, cdg_prec *M) {
+    if (M->apply_mode != CDG_APPLY_DATAFLOW || !M->d_dferr) return CDG_OK;
+    int32_t err = 0;
+    CDG_HIP_TRY(hipMemcpyAsync(&err, M->d_dferr, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
+    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (!err) return CDG_OK;
+    int e = df_reset(ctx, M);
+    if (e) return e;
+    return cdg_fail(ctx, CDG_ERR_HIP, "ILU(0) dataflow sweep gave up waiting for a dependency; counters reset");
 }
 
 extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row) {
@@ -524,6 +605,7 @@ static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const dou
     const int64_t len = (int64_t)M->nrows * M->n;
     if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len, v, z);
     if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
+    if (M->apply_mode == CDG_APPLY_DATAFLOW) return launch_ilu_apply_df(ctx, M, A, v, z);
     return launch_ilu_apply(ctx, M, A, v, z);
 }
 
@@ -536,6 +618,7 @@ extern "C" int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const
     if (!e) e = upload_in(ctx, vi);
     if (!e) e = prec_apply_dev(ctx, M, A, vi.dev, zo.dev);
     if (!e) e = finish_out(ctx, zo);
+    if (!e) e = df_check(ctx, M);
     return e;
 }
 
@@ -878,5 +961,9 @@ extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, con
     st->t_norm = S.t.norm;
     st->t_total = secs(t_start, clk::now());
     if (!e && !std::isfinite(st->residual_norm)) e = cdg_fail(ctx, CDG_ERR_NAN, "non-finite final residual");
+    if (e == CDG_ERR_NAN) {
+        const int e2 = df_check(ctx, M);
+        if (e2) return e2;
+    }
     return e;
 }

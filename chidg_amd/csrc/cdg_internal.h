@@ -49,9 +49,9 @@ struct StreamShape {
     int T = 0;     // threads per workgroup (V*T is a multiple of n)
     int dc = 0;    // column advance per iteration = V*T/n
 };
-// Launch shape of the fixed-row walk; T is a multiple of n/gcd(n,V) chosen
-// for lane utilization, preferring sizes near `target`.
-StreamShape stream_shape(int n, int V, int target);
+// Launch shape of the fixed-row walk; T <= tmax is a multiple of n/gcd(n,V)
+// chosen for lane utilization, preferring sizes near `target`.
+StreamShape stream_shape(int n, int V, int target, int tmax = 1024);
 
 struct cdg_mat {
     int32_t nrows = 0, n = 0, nnzb = 0;
@@ -95,11 +95,22 @@ struct cdg_prec {
     std::vector<int32_t> blev_row_ptr;
     int32_t *d_blev_rows = nullptr;
     int32_t nblev = 0;
+    std::vector<int32_t> h_lev_rows, h_blev_rows;   // host copies of the level orders
     int32_t maxdep = 1;               // max lower/upper blocks of a row
     // apply launch shape: each block's columns are cut into `slices` ranges of
     // `slice_w` columns; shape[precision] is the fixed-row walk for that element
     int32_t slices = 1, slice_w = 1, unroll = 4;
     StreamShape shape[3];
+    // apply schedule: CDG_APPLY_LEVELS (one launch per level and stage) or
+    // CDG_APPLY_DATAFLOW (one persistent launch, k_dfapply.hip)
+    int32_t apply_mode = 0;
+    int32_t df_T = 0, df_dc = 0, df_rounds = 0, df_w = 0, df_S = 0, df_grid = 0, df_ntasks = 0;
+    size_t df_lds = 0;
+    int32_t *d_tasks = nullptr;        // task codes in ticket order
+    int32_t *d_cnt = nullptr;          // [3][nrows] finished slices (cumulative)
+    unsigned long long *d_ticket = nullptr;
+    int32_t *d_dferr = nullptr;        // raised by a sweep that gave up waiting
+    int32_t df_epoch = 0;              // calls since the counters were zeroed
     uint64_t stamp = 0;
     int32_t factored = 0;
     std::vector<DevMem> owned;
@@ -108,6 +119,10 @@ struct cdg_prec {
 // ---- kernel launchers (defined in the .hip files) ----------------------------
 int launch_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y);
 int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
+int launch_ilu_apply_df(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
+int df_configure(cdg_ctx *ctx, cdg_prec *M);
+int df_reset(cdg_ctx *ctx, cdg_prec *M);
+enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1 };
 int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row);
 int launch_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_diag, double s_off);
 int launch_vec_synthetic(cdg_ctx *ctx, int64_t len, uint64_t seed, double scale, double *v);

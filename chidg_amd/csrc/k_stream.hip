@@ -262,7 +262,7 @@ __global__ void k_widen<cdg_bf16>(const cdg_bf16 *__restrict__ src, double *__re
 }  // namespace
 
 // ---- launch shape -------------------------------------------------------------
-StreamShape stream_shape(int n, int V, int target) {
+StreamShape stream_shape(int n, int V, int target, int tmax) {
     int g = n;
     {   // g = n / gcd(n, V): T must be a multiple of g for V*T % n == 0
         int a = n, b = V;
@@ -271,14 +271,15 @@ StreamShape stream_shape(int n, int V, int target) {
     }
     StreamShape best;
     double best_score = -1e30;
-    for (int T = g; T <= kMaxThreads; T += g) {
-        if (T < 64 && T + g <= kMaxThreads) continue;
+    tmax = std::min(tmax, kMaxThreads);
+    for (int T = g; T <= tmax; T += g) {
+        if (T < 64 && T + g <= tmax) continue;
         const int waves = (T + 63) / 64;
         const double util = (double)T / (64.0 * waves);
         const double score = util - 0.03 * std::abs(T - target) / 64.0;
         if (score > best_score) { best_score = score; best.T = T; }
     }
-    if (best.T == 0 && g <= kMaxThreads) best.T = g;
+    if (best.T == 0 && g <= tmax) best.T = g;
     best.dc = best.T ? V * best.T / n : 0;
     return best;
 }

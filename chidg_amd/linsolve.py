@@ -180,8 +180,9 @@ def chidg_mv_async(A: chidg_matrix, x, y) -> None:
 class _Preconditioner:
     kind = _lib.PREC_IDENTITY
 
-    def __init__(self, precision: str = "fp64"):
+    def __init__(self, precision: str = "fp64", schedule: str | None = None):
         self.precision = precision
+        self.schedule = schedule
         self.h = None
         self.ctx = None
         self.initialized = False
@@ -195,7 +196,16 @@ class _Preconditioner:
                                                      C.byref(h)), "cdg_prec_create")
         self.h = h
         self.initialized = True
+        if self.schedule is not None:
+            self.set_schedule(self.schedule)
         return self
+
+    def set_schedule(self, schedule: str):
+        """ILU(0) apply schedule: 'levels' (one launch per wavefront level) or
+        'dataflow' (one persistent launch)."""
+        code = {"levels": 0, "dataflow": 1}[schedule]
+        self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
+        self.schedule = schedule
 
     def update(self, A: chidg_matrix, b=None):
         if not self.initialized:

@@ -145,6 +145,14 @@ int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v,
 int cdg_prec_apply_async(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 /* preconditioner%tear_down (precon_ILU0.f90:450-470). */
 int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
+/* Schedule of the ILU(0) apply (no reference counterpart: the reference's
+ * apply is one sequential loop over rows, precon_ILU0.f90:286-411).  Both give
+ * the same result for every schedule-independent input order:
+ *   CDG_SCHEDULE_LEVELS    one launch per wavefront level and stage
+ *   CDG_SCHEDULE_DATAFLOW  one persistent launch, row-to-row dependencies  */
+#define CDG_SCHEDULE_LEVELS   0
+#define CDG_SCHEDULE_DATAFLOW 1
+int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule);
 /* Factor as FP64 for inspection: L (nnzb*n*n, lower kind-0 slots filled,
  * others zero) and Dinv (nrows*n*n), the LD matrix of precon_ILU0_t. */
 int cdg_prec_download(cdg_ctx *ctx, const cdg_prec *M, double *L, double *Dinv);

@@ -1,0 +1,106 @@
+"""The ILU(0) apply under its two schedules (one launch per wavefront level,
+or one persistent dataflow launch) against the oracle, against each other,
+and across repeated calls (the dataflow counters carry over between calls)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from chidg_amd import pattern as pt
+from chidg_amd.linsolve import Context, chidg_matrix, fgmres, precon_ILU0
+from helpers import ilu_test_system, synthetic_system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def upload(ctx, p, blocks):
+    A = chidg_matrix(p, ctx)
+    A.store(blocks)
+    return A
+
+
+def test_dataflow_known_answer(ctx):
+    """test_ILU0_mod.pf:294-332 through the dataflow sweep (1x1 blocks)."""
+    p, blocks, b = ilu_test_system()
+    A = upload(ctx, p, blocks)
+    M = precon_ILU0(schedule="dataflow").init(A)
+    M.update(A, b)
+    for _ in range(3):
+        np.testing.assert_allclose(M.apply(A, b), [43 / 7, 18 / 7, 3 / 14], rtol=1e-14)
+
+
+# (dims, degree, precision, tolerance against the FP64 oracle)
+CASES = [((3, 3, 3), 1, "fp64", 1e-12), ((4, 3, 2), 2, "fp64", 1e-12), ((5, 2, 2), 0, "fp64", 1e-12),
+         ((4, 4, 4), 1, "fp32", 1e-5), ((3, 3, 3), 2, "fp32", 1e-5), ((3, 3, 3), 2, "bf16", 5e-2)]
+
+
+@pytest.mark.parametrize("dims,degree,precision,tol", CASES)
+def test_dataflow_matches_oracle_and_levels(ctx, dims, degree, precision, tol):
+    p, blocks = synthetic_system(*dims, degree, seed=17)
+    A = upload(ctx, p, blocks)
+    Ml = precon_ILU0(precision, schedule="levels").init(A)
+    Md = precon_ILU0(precision, schedule="dataflow").init(A)
+    Ml.update(A)
+    Md.update(A)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    for seed in (3, 4):
+        v = po.fill_vector(p.vec_len, seed)
+        z_ref = po.ilu0_apply(p, blocks, L_ref, D_ref, v)
+        zd = Md.apply(A, v)
+        assert rel(zd, z_ref) < tol
+        # the two schedules differ only in how partial sums are grouped
+        assert rel(zd, Ml.apply(A, v)) < (1e-13 if precision == "fp64" else tol)
+        # repeated calls: bitwise identical (fixed-order sums, counters carried over)
+        assert np.array_equal(Md.apply(A, v), zd)
+
+
+def test_dataflow_fgmres_config1(ctx):
+    """4^3 periodic P1, FGMRES(30)+ILU0 to rtol 1e-8 with the dataflow sweep."""
+    p, blocks = synthetic_system(4, 4, 4, 1, seed=20261202)
+    b = po.fill_vector(p.vec_len, 1)
+    A = upload(ctx, p, blocks)
+    ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
+    x = np.zeros(p.vec_len)
+    ls.solve(A, x, b, precon_ILU0(schedule="dataflow"))
+    x_ref, n_ref, _, _ = po.fgmres(p, blocks, b, nkrylov=30, tol=1e-30, rtol=1e-8)
+    assert abs(ls.niter - n_ref) <= 1
+    assert rel(x, x_ref) < 1e-8
+
+
+def test_dataflow_full_size(ctx):
+    """16^3 periodic P2 (4096 block rows): dataflow and level sweeps agree, and
+    an FGMRES(30) solve with the FP32 dataflow sweep reaches rtol 1e-8."""
+    import torch
+    n = pt.block_size(2)
+    p = pt.hex_box(16, 16, 16, n)
+    A = chidg_matrix(p, ctx)
+    A.fill_synthetic(20261202, 1 / np.sqrt(n), 0.1 / np.sqrt(n))
+    dev = torch.empty(0, dtype=torch.float64, device="cuda:0")
+    v = A.synthetic_vector(1, like=dev)
+    for precision, tol in (("fp64", 1e-13), ("fp32", 1e-6)):
+        Ml = precon_ILU0(precision, schedule="levels").init(A)
+        Md = precon_ILU0(precision, schedule="dataflow").init(A)
+        Ml.update(A)
+        Md.update(A)
+        zl = Ml.apply(A, v)
+        zd = Md.apply(A, v)
+        assert float(torch.linalg.norm(zd - zl) / torch.linalg.norm(zl)) < tol
+        assert torch.equal(Md.apply(A, v), zd)
+        Ml.tear_down()
+        if precision == "fp32":
+            x = torch.zeros_like(v)
+            ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
+            ls.solve(A, x, v, Md)
+            assert ls.residual_norm / float(torch.linalg.norm(v)) < 1e-8
+        Md.tear_down()
+    A.release()
