@@ -12,6 +12,7 @@
 #include "cdg_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace {
 
@@ -298,6 +299,262 @@ k_fac_inv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n
         }
 }
 
+// ---- blocked Gauss-Jordan inverse -------------------------------------------
+// DPP with a row mask (lanes outside the mask keep their own value).
+template <int CTRL, int RM> __device__ __forceinline__ int dpp_rm_i32(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM> __device__ __forceinline__ double dpp_rm(double x) {
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, int2{dpp_rm_i32<CTRL, RM>(v.x), dpp_rm_i32<CTRL, RM>(v.y)});
+}
+template <int CTRL, int RM> __device__ __forceinline__ float dpp_rm(float x) {
+    return __builtin_bit_cast(float, dpp_rm_i32<CTRL, RM>(__builtin_bit_cast(int, x)));
+}
+template <typename T, int CTRL, int RM>
+__device__ __forceinline__ void wargmax_step(T &best, int &bi, int &fu) {
+    const T ob = dpp_rm<CTRL, RM>(best);
+    const int oi = dpp_rm_i32<CTRL, RM>(bi);
+    fu = min(fu, dpp_rm_i32<CTRL, RM>(fu));
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+}
+// (value, index) maximum over the wave, lower index on ties, and the minimum
+// of fu; the result is complete in lane 63.
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T &best, int &bi, int &fu) {
+    wargmax_step<T, 0xB1, 0xF>(best, bi, fu);    // quad_perm [1,0,3,2]
+    wargmax_step<T, 0x4E, 0xF>(best, bi, fu);    // quad_perm [2,3,0,1]
+    wargmax_step<T, 0x124, 0xF>(best, bi, fu);   // row_ror:4
+    wargmax_step<T, 0x128, 0xF>(best, bi, fu);   // row_ror:8  (every lane: its row's result)
+    wargmax_step<T, 0x142, 0xA>(best, bi, fu);   // row_bcast:15 into rows 1, 3
+    wargmax_step<T, 0x143, 0xC>(best, bi, fu);   // row_bcast:31 into rows 2, 3
+}
+// Lanes of one wave exchanging data through LDS: without this the compiler
+// may order another lane's load before the writing lane's store (each lane's
+// program alone allows it).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ double readlane(double x, int l) {
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(v.x, l), __builtin_amdgcn_readlane(v.y, l)});
+}
+__device__ __forceinline__ float readlane(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
+// Dinv_i = inv(D_i) for blocks of n <= 16*NT, one workgroup of 4 waves per
+// block.  The block lives in MFMA accumulator tiles (16 x 16 tile t =
+// (ti, tj) held by wave t % 4).  Columns are processed in panels of 16:
+//   1. the panel's tiles are written to LDS and wave 0 runs Gauss-Jordan with
+//      partial pivoting on the n x 16 panel in registers (pivot: largest
+//      |a(r,k)| over unused rows, lower row on ties, found with DPP); rows are
+//      never swapped.  The panel then holds C = [W on the pivot rows P,
+//      -A(r,P) W elsewhere], W the inverse of the panel's pivot block;
+//   2. every other column is updated at once, X += (C - E_P) X(P,:), one
+//      rank-16 product on MFMA (E_P: ones at (p_j, j));
+//   3. the panel's tiles take C.
+// This is the unblocked in-place Gauss-Jordan (same pivots) reorganized; the
+// final scatter inv(rowstep(r), pivrow(m)) = S(r, m) undoes the implicit row
+// permutation.
+template <typename T, int NT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err) {
+    constexpr int NP = 16 * NT;                  // padded size
+    constexpr int TPW = (NT * NT + 3) / 4;       // tiles per wave
+    constexpr int NR = (NP + 63) / 64;           // panel rows per lane of wave 0
+    constexpr int GP = 17;                       // LDS pitch of the panel (bank spread)
+    constexpr int XPP = NP + 1;                  // LDS pitch of the pivot rows
+    __shared__ T G[NP * GP];                     // panel in, C - E_P out
+    __shared__ T XP[16 * XPP];                   // pivot rows of the other columns
+    __shared__ int32_t rowtag[NP];               // 1 + step at which the row was pivot (0: not yet)
+    __shared__ int32_t pivrow[NP];
+    __shared__ int32_t bad_sh;
+    __shared__ T prow_sh[16];
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int blk = rows[blockIdx.x];
+    T *M = D + (size_t)blk * ld;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    acc_t acc[TPW];
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        const int t = wv + 4 * s;
+        const int ti = t / NT, tj = t - ti * NT;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + (lane & 15);
+            acc[s][r] = (t < NT * NT && row < n && col < n) ? M[(size_t)col * n + row] : T(0);
+        }
+    }
+    for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
+    for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
+    if (tid == 0) bad_sh = 0;
+    for (int kb = 0; kb < NT; ++kb) {
+        const int k0 = 16 * kb;
+        const int w = min(16, n - k0);
+        // 1. panel tiles to LDS
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            const int t = wv + 4 * s;
+            const int ti = t / NT, tj = t - ti * NT;
+            if (t < NT * NT && tj == kb) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) G[(16 * ti + Mfma<T>::row(lane, r)) * GP + (lane & 15)] = acc[s][r];
+            }
+        }
+        __syncthreads();
+        // 2. panel Gauss-Jordan (wave 0)
+        if (wv == 0) {
+            T pr[NR][16];
+            bool used[NR];
+#pragma unroll
+            for (int a = 0; a < NR; ++a) {
+                const int r = lane + 64 * a;
+                used[a] = r >= n || (r < NP && rowtag[r] != 0);
+#pragma unroll
+                for (int c = 0; c < 16; ++c) pr[a][c] = r < NP ? G[r * GP + c] : T(0);
+            }
+            bool bad = false;
+            // The panel's columns rotate left by one per step, so the column
+            // being eliminated always sits at register position 0 and the
+            // loop body is not unrolled (code size: one step, not sixteen).
+            // After 16 rotations the columns are back in place.
+#pragma unroll 1
+            for (int j = 0; j < 16; ++j) {
+                if (j < w) {
+                    T best = T(-1);
+                    int bi = 0x7fffffff, fu = 0x7fffffff;
+#pragma unroll
+                    for (int a = 0; a < NR; ++a) {
+                        const int r = lane + 64 * a;
+                        if (!used[a]) {
+                            const T av = fabs(pr[a][0]);
+                            if (av > best) { best = av; bi = r; }
+                            fu = min(fu, r);
+                        }
+                    }
+                    wave_argmax<T>(best, bi, fu);
+                    const T bbest = readlane(best, 63);
+                    int p = __builtin_amdgcn_readlane(bi, 63);
+                    if (!(bbest > T(0)) || p >= n) { bad = true; p = __builtin_amdgcn_readlane(fu, 63); }
+                    const int pa = p >> 6, pl = p & 63;
+                    // the pivot row through LDS
+                    if (lane == pl) {
+#pragma unroll
+                        for (int a = 0; a < NR; ++a)
+                            if (a == pa) {
+#pragma unroll
+                                for (int c = 0; c < 16; ++c) prow_sh[c] = pr[a][c];
+                            }
+                    }
+                    wave_sync();
+                    T prow[16];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) prow[c] = prow_sh[c];
+                    const T rinv = T(1) / prow[0];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) prow[c] *= rinv;
+#pragma unroll
+                    for (int a = 0; a < NR; ++a) {
+                        const bool isp = (a == pa) && (lane == pl);
+                        const T cr = isp ? T(0) : pr[a][0];
+#pragma unroll
+                        for (int c = 1; c < 16; ++c) pr[a][c] -= cr * prow[c];
+                        pr[a][0] = -cr * rinv;
+                        used[a] = used[a] || isp;
+                    }
+                    if (lane == pl) {
+#pragma unroll
+                        for (int a = 0; a < NR; ++a)
+                            if (a == pa) {
+#pragma unroll
+                                for (int c = 1; c < 16; ++c) pr[a][c] = prow[c];
+                                pr[a][0] = rinv;
+                            }
+                    }
+                    if (lane == 0) { rowtag[p] = k0 + j + 1; pivrow[k0 + j] = p; }
+                    wave_sync();   // prow_sh is rewritten by the next step
+                }
+#pragma unroll
+                for (int a = 0; a < NR; ++a) {
+                    const T first = pr[a][0];
+#pragma unroll
+                    for (int c = 0; c < 15; ++c) pr[a][c] = pr[a][c + 1];
+                    pr[a][15] = first;
+                }
+            }
+            // C - E_P back to LDS (the ones of E_P sit at (p_j, j))
+#pragma unroll
+            for (int a = 0; a < NR; ++a) {
+                const int r = lane + 64 * a;
+                if (r < NP) {
+                    const int tg = rowtag[r] - 1 - k0;     // panel step of this row, if any
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) G[r * GP + c] = pr[a][c] - ((tg == c) ? T(1) : T(0));
+                }
+            }
+            if (bad && lane == 0) bad_sh = 1;
+        }
+        __syncthreads();
+        // 3. pivot rows of the other columns
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            const int t = wv + 4 * s;
+            const int ti = t / NT, tj = t - ti * NT;
+            if (t < NT * NT && tj != kb) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * ti + Mfma<T>::row(lane, r);
+                    const int j = rowtag[row] - 1 - k0;
+                    if (j >= 0 && j < 16) XP[j * XPP + 16 * tj + (lane & 15)] = acc[s][r];
+                }
+            }
+        }
+        __syncthreads();
+        // 4. X += (C - E_P) X(P,:) on MFMA; the panel tiles take C
+#pragma unroll
+        for (int s = 0; s < TPW; ++s) {
+            const int t = wv + 4 * s;
+            const int ti = t / NT, tj = t - ti * NT;
+            if (t < NT * NT) {
+                if (tj != kb) {
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const T a = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
+                        const T b = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
+                        acc[s] = Mfma<T>::run(a, b, acc[s]);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * ti + Mfma<T>::row(lane, r);
+                        const int c = lane & 15;
+                        const int tg = rowtag[row] - 1 - k0;
+                        acc[s][r] = G[row * GP + c] + ((tg == c) ? T(1) : T(0));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && bad_sh) atomicMin(err, blk);
+#pragma unroll
+    for (int s = 0; s < TPW; ++s) {
+        const int t = wv + 4 * s;
+        const int ti = t / NT, tj = t - ti * NT;
+        if (t < NT * NT) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + (lane & 15);
+                if (row < n && col < n) M[(size_t)pivrow[col] * n + (rowtag[row] - 1)] = acc[s][r];
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ cdg_bf16 to_bf16(float f) {
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
@@ -328,6 +585,14 @@ __global__ void k_f32_to_bf16(const float *__restrict__ src, cdg_bf16 *__restric
 template <typename T>
 int launch_inv(cdg_ctx *ctx, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n, int32_t *err) {
     dim3 grid(nrows_lvl), block(256);
+    static const int mode = [] { const char *e = std::getenv("CDG_INV"); return e ? std::atoi(e) : 0; }();
+    if (mode == 1 && n <= 144) {
+        if (n <= 48) hipLaunchKernelGGL((k_fac_binv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+        else if (n <= 96) hipLaunchKernelGGL((k_fac_binv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+        else hipLaunchKernelGGL((k_fac_binv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+        CDG_HIP_TRY(hipGetLastError());
+        return CDG_OK;
+    }
     if (n <= 48) hipLaunchKernelGGL((k_fac_inv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
     else if (n <= 96) hipLaunchKernelGGL((k_fac_inv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
     else if (n <= 144) hipLaunchKernelGGL((k_fac_inv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
