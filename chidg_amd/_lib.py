@@ -25,8 +25,23 @@ EXPORTS = [
     "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
     "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
     "cdg_prec_set_schedule",
+    "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
+    "cdg_server_last_error", "cdg_server_destroy",
     "cdg_fgmres_solve", "cdg_fgmres_default_params",
 ]
+
+
+class Request(C.Structure):
+    """cdg_request: one synthetic parameter-sweep system."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("order", C.c_int32),
+                ("periodic", C.c_int32), ("precision", C.c_int32), ("seed_A", C.c_uint64), ("seed_b", C.c_uint64),
+                ("s_diag", C.c_double), ("s_off", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("niter", C.c_int32), ("residual_norm", C.c_double),
+                ("t_queue", C.c_double), ("t_run", C.c_double), ("t_latency", C.c_double),
+                ("t_factor", C.c_double), ("slot", C.c_int32)]
 
 
 class FgmresParams(C.Structure):
@@ -95,6 +110,14 @@ def load() -> C.CDLL:
         "cdg_prec_download": ([vp, vp, dp, dp], C.c_int),
         "cdg_fgmres_solve": ([vp, vp, vp, dp, dp, C.POINTER(FgmresParams), C.POINTER(FgmresStats)], C.c_int),
         "cdg_fgmres_default_params": ([C.POINTER(FgmresParams)], None),
+        "cdg_prec_set_schedule": ([vp, vp, C.c_int32], C.c_int),
+        "cdg_server_create": ([C.c_int32, C.c_int32, C.c_int64, C.POINTER(FgmresParams), pp], C.c_int),
+        "cdg_server_submit": ([vp, C.POINTER(Request), dp, C.POINTER(C.c_uint64)], C.c_int),
+        "cdg_server_wait": ([vp, C.c_uint64, C.POINTER(Result)], C.c_int),
+        "cdg_server_footprint": ([C.POINTER(Request), C.POINTER(FgmresParams), C.POINTER(C.c_int64)], C.c_int),
+        "cdg_server_info": ([vp, ip, ip, ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "cdg_server_last_error": ([vp], C.c_char_p),
+        "cdg_server_destroy": ([vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchidg_hip.so")
-SOURCES = ["cdg_api.hip", "k_stream.hip", "k_dfapply.hip", "k_factor.hip", "k_krylov.hip", "k_synth.hip"]
+SOURCES = ["cdg_api.hip", "k_stream.hip", "k_dfapply.hip", "k_factor.hip", "k_krylov.hip", "k_synth.hip", "cdg_serve.hip"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
          "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -44,6 +44,13 @@ def build(verbose: bool = False) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    # every symbol of this library must resolve inside it (or in the HIP / C++ runtimes)
+    r = subprocess.run(["nm", "-D", "--undefined-only", tmp], capture_output=True, text=True)
+    own = [l.split()[-1] for l in r.stdout.splitlines() if l.split() and "@" not in l.split()[-1]
+           and not l.split()[-1].startswith(("hip", "__hip", "_ITM", "__gmon", "__cxa", "_Unwind", "__amd"))]
+    if own:
+        os.remove(tmp)
+        raise RuntimeError(f"unresolved symbols in {LIB}: {own}")
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -27,23 +27,68 @@ int cdg_fail(const cdg_ctx *ctx, int code, const char *fmt, ...) {
     return code;
 }
 
+// Device memory of a context.  Freed blocks stay cached in the context and
+// are handed out again to later requests of up to their size (at most twice
+// the request): every use of a context's memory is ordered on its one
+// stream, and hipFree would synchronize the whole device, stalling the other
+// contexts' work.  The cache is released when the context is destroyed, or
+// when an allocation fails.
+static void ctx_flush_cache(cdg_ctx *ctx) {
+    if (ctx->cache.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->cache) (void)hipFree(kv.second);
+    ctx->cache.clear();
+    ctx->mem_cached = 0;
+}
+
 int dev_alloc(cdg_ctx *ctx, void **p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) return CDG_OK;
+    bytes = (bytes + 255) / 256 * 256;
+    auto it = ctx->cache.lower_bound(bytes);
+    if (it != ctx->cache.end() && it->first <= 2 * bytes) {
+        *p = it->second;
+        ctx->alloc_size[*p] = it->first;
+        ctx->mem_cached -= (int64_t)it->first;
+        ctx->mem_used += (int64_t)it->first;
+        ctx->cache.erase(it);
+        return CDG_OK;
+    }
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx_flush_cache(ctx);
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
         return cdg_fail(ctx, CDG_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
     }
+    ctx->alloc_size[*p] = bytes;
     ctx->mem_used += (int64_t)bytes;
     return CDG_OK;
 }
 
-void dev_free(cdg_ctx *ctx, void *p, size_t bytes) {
+void dev_free(cdg_ctx *ctx, void *p, size_t) {
     if (!p) return;
-    (void)hipFree(p);
+    auto it = ctx->alloc_size.find(p);
+    if (it == ctx->alloc_size.end()) return;
+    const size_t bytes = it->second;
+    ctx->alloc_size.erase(it);
     ctx->mem_used -= (int64_t)bytes;
+    ctx->mem_cached += (int64_t)bytes;
+    ctx->cache.emplace(bytes, p);
+}
+
+int ctx_norm2(cdg_ctx *ctx, int64_t len, const double *v, double *out) {
+    int e = kv_sumsq(ctx, len, v, ctx->d_small);
+    if (!e) {
+        CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        *out = std::sqrt(ctx->h_small[0]);
+    }
+    return e;
 }
 
 int ctx_ensure_part(cdg_ctx *ctx, size_t n) {
@@ -128,7 +173,11 @@ int finish_out(cdg_ctx *ctx, const VecArg &a) {
     return CDG_OK;
 }
 
+}  // namespace
+
 int64_t slot_elems(int n) { return ((int64_t)n * n + 63) / 64 * 64; }
+
+namespace {
 
 // Allocates and fills a device array.  Waits for the copy: callers pass
 // temporaries, and an asynchronous copy from pageable memory may read its
@@ -245,6 +294,9 @@ extern "C" int cdg_ctx_destroy(cdg_ctx *ctx) {
     dev_free(ctx, ctx->d_small, ctx->small_cap * sizeof(double));
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     for (int s = 0; s < 3; ++s) dev_free(ctx, ctx->d_stage[s], ctx->stage_cap[s] * sizeof(double));
+    for (auto &kv : ctx->alloc_size) (void)hipFree(kv.first);   // blocks of objects never destroyed
+    ctx->alloc_size.clear();
+    ctx_flush_cache(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CDG_OK;

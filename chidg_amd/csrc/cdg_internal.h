@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/chidg_hip.h"
@@ -32,7 +34,10 @@ struct cdg_ctx {
     int num_cus = 0;
     hipStream_t stream = nullptr;
     mutable std::string err;
-    int64_t mem_used = 0;
+    int64_t mem_used = 0;              // bytes held by live objects
+    int64_t mem_cached = 0;            // bytes freed by objects, kept for reuse
+    std::unordered_map<void *, size_t> alloc_size;
+    std::multimap<size_t, void *> cache;
     // reduction scratch (device) and pinned host mirror
     double *d_part = nullptr;     // partial sums
     size_t part_cap = 0;          // doubles
@@ -79,6 +84,7 @@ struct cdg_prec {
     void *U = nullptr;                 // nup slots (FP32/BF16 copies of A's upper blocks)
     void *Dinv = nullptr;              // nrows slots
     float *L32 = nullptr, *D32 = nullptr;   // FP32 work for the BF16 variant
+    void *inv_w = nullptr;             // working slots of the global-memory block inverse (n > 192)
     // device index arrays
     int32_t *d_lo_ptr = nullptr, *d_lo_blk = nullptr, *d_lo_col = nullptr, *d_lo_tr = nullptr;
     int32_t *d_up_ptr = nullptr, *d_up_idx = nullptr, *d_up_col = nullptr, *d_up_blk = nullptr;
@@ -145,6 +151,8 @@ int kv_zero(cdg_ctx *ctx, int64_t len, double *x);
 int kv_copy(cdg_ctx *ctx, int64_t len, const double *src, double *dst);
 
 int ctx_ensure_small(cdg_ctx *ctx, size_t n);
+int ctx_norm2(cdg_ctx *ctx, int64_t len, const double *v, double *out);   // ||v|| (synchronizes)
+int64_t slot_elems(int n);   // elements per block slot: n*n rounded up to 64
 int ctx_ensure_part(cdg_ctx *ctx, size_t n);
 int dev_alloc(cdg_ctx *ctx, void **p, size_t bytes);
 void dev_free(cdg_ctx *ctx, void *p, size_t bytes);

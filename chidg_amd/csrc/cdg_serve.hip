@@ -1,0 +1,286 @@
+// Request loop for parameter-sweep solves on one GPU (no reference
+// counterpart: ChiDG solves one system per MPI rank).
+//
+// A server owns `nslots` worker threads, each with its own context (HIP
+// stream, reduction scratch, cached device memory).  Requests queue in
+// arrival order; a free worker admits the request at the head of the queue
+// as soon as its device-memory footprint fits the budget next to the
+// requests already running, so requests that arrive while others run start
+// as slots and memory free up, and their work overlaps on the device (one
+// stream per slot).  A request's Jacobian, ILU(0) factor and Krylov basis
+// stay on the device for the whole solve; its result depends only on the
+// request (every kernel sums in a fixed order), not on what runs beside it.
+#include "cdg_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// Uniform hex box in ChiDG row order: element i fastest; per row the interior
+// neighbours in face order XI_MIN, XI_MAX, ETA_MIN, ETA_MAX, ZETA_MIN, ZETA_MAX,
+// then the diagonal, then periodic wrap neighbours as coupling-only blocks
+// (a donor already present is not added twice).  Same order as
+// chidg_amd/pattern.py hex_box (type_domain_matrix.f90:134,253-374).
+void hex_box(int nx, int ny, int nz, int periodic, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
+             std::vector<int32_t> &kind) {
+    const int dims[3] = {nx, ny, nz};
+    const int stride[3] = {1, nx, nx * ny};
+    const int nrows = nx * ny * nz;
+    row_ptr.assign(nrows + 1, 0);
+    col.clear();
+    kind.clear();
+    for (int e = 0; e < nrows; ++e) {
+        const int idx[3] = {e % nx, (e / nx) % ny, e / (nx * ny)};
+        std::vector<int> chi;
+        for (int d = 0; d < 3; ++d)
+            for (int side = -1; side <= 1; side += 2) {
+                const int q = idx[d] + side;
+                if (q >= 0 && q < dims[d]) {
+                    col.push_back(e + side * stride[d]);
+                    kind.push_back(CDG_KIND_LOCAL);
+                } else if (periodic & (1 << d)) {
+                    const int wrap = (q + dims[d]) % dims[d];
+                    const int donor = e + (wrap - idx[d]) * stride[d];
+                    if (std::find(chi.begin(), chi.end(), donor) == chi.end()) chi.push_back(donor);
+                }
+            }
+        col.push_back(e);
+        kind.push_back(CDG_KIND_LOCAL);
+        for (int dnr : chi) { col.push_back(dnr); kind.push_back(CDG_KIND_COUPLING); }
+        row_ptr[e + 1] = (int32_t)col.size();
+    }
+}
+
+int block_n(const cdg_request &r) { return (r.order + 1) * (r.order + 1) * (r.order + 1) * 5; }
+
+struct Job {
+    uint64_t ticket;
+    cdg_request req;
+    double *x_out;
+    int64_t footprint;
+    clk::time_point t_submit;
+};
+
+struct Slot {
+    cdg_result res{};
+    bool done = false;
+};
+
+}  // namespace
+
+struct cdg_server {
+    int device = 0;
+    int64_t budget = 0;
+    cdg_fgmres_params params{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job> queue;
+    std::unordered_map<uint64_t, Slot> results;
+    uint64_t next_ticket = 1;
+    int64_t in_use = 0;
+    int active = 0, peak_active = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::string err;
+};
+
+extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_params *params, int64_t *bytes) {
+    if (!req || !params || !bytes) return CDG_ERR_ARG;
+    if (req->nx <= 0 || req->ny <= 0 || req->nz <= 0 || req->order < 0 || req->order > 3) return CDG_ERR_ARG;
+    std::vector<int32_t> rp, col, kind;
+    hex_box(req->nx, req->ny, req->nz, req->periodic, rp, col, kind);
+    const int64_t nrows = (int64_t)rp.size() - 1, nnzb = (int64_t)col.size();
+    int64_t nloc = 0;
+    for (int32_t k : kind) nloc += (k == CDG_KIND_LOCAL);
+    const int64_t n = block_n(*req), ld = slot_elems((int)n), len = nrows * n;
+    const int64_t nlo = (nloc - nrows) / 2;
+    const int64_t es = req->precision == CDG_FP64 ? 8 : (req->precision == CDG_FP32 ? 4 : 2);
+    int64_t f = 8 * nnzb * ld;                                   // A
+    f += es * (nlo + nrows) * ld;                                // L, Dinv
+    if (req->precision != CDG_FP64) f += es * nlo * ld;         // U copy
+    if (req->precision == CDG_BF16) f += 4 * (nlo + nrows) * ld; // FP32 factor work
+    if (n > 192) f += 8 * nrows * ld;                            // inverse working slots (bound)
+    f += 3 * 8 * nrows * 8 * n;                                  // apply partial sums (<= 8 slices)
+    const int64_t m = std::max(1, params->nkrylov);
+    f += 8 * len * (2 * (m + 1) + 6);                            // V, Z, b, x, work vectors
+    if (params->inner_fgmres) f += 8 * len * (2 * (std::max(1, params->inner_nkrylov) + 1) + 4);
+    *bytes = f + f / 20 + (64ll << 20);
+    return CDG_OK;
+}
+
+namespace {
+
+// One request, start to finish, on the worker's context.
+int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result &res) {
+    const cdg_request &r = job.req;
+    std::vector<int32_t> rp, col, kind;
+    hex_box(r.nx, r.ny, r.nz, r.periodic, rp, col, kind);
+    const int nrows = (int)rp.size() - 1, n = block_n(r);
+    const int64_t len = (int64_t)nrows * n;
+    cdg_mat *A = nullptr;
+    cdg_prec *M = nullptr;
+    double *b = nullptr, *x = nullptr;
+    int e = cdg_mat_create(ctx, nrows, n, rp.data(), col.data(), kind.data(), &A);
+    if (!e) e = cdg_mat_fill_synthetic(ctx, A, r.seed_A, r.s_diag, r.s_off);
+    if (!e) e = dev_alloc(ctx, (void **)&b, sizeof(double) * len);
+    if (!e) e = dev_alloc(ctx, (void **)&x, sizeof(double) * len);
+    if (!e) e = launch_vec_synthetic(ctx, len, r.seed_b, 1.0, b);
+    if (!e) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, r.precision, &M);
+    cdg_fgmres_stats st{};
+    if (!e) e = cdg_fgmres_solve(ctx, A, M, b, x, &params, &st);
+    double bn = 0.0;
+    if (!e) e = ctx_norm2(ctx, len, b, &bn);
+    if (!e && job.x_out) {
+        if (hipMemcpyAsync(job.x_out, x, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            e = cdg_fail(ctx, CDG_ERR_HIP, "copying the solution out failed");
+    }
+    res.niter = st.niter;
+    res.residual_norm = bn > 0 ? st.residual_norm / bn : st.residual_norm;
+    res.t_factor = st.t_factor;
+    if (M) cdg_prec_destroy(ctx, M);
+    if (A) cdg_mat_destroy(ctx, A);
+    dev_free(ctx, b, 0);
+    dev_free(ctx, x, 0);
+    return e;
+}
+
+void worker(cdg_server *s, int slot) {
+    cdg_ctx *ctx = nullptr;
+    const int ce = cdg_ctx_create(s->device, &ctx);
+    for (;;) {
+        Job job;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                if (s->stop && s->queue.empty()) return true;
+                if (s->queue.empty()) return false;
+                const int64_t f = s->queue.front().footprint;
+                return s->in_use + f <= s->budget || s->active == 0;
+            });
+            if (s->queue.empty()) break;
+            job = s->queue.front();
+            s->queue.pop_front();
+            s->in_use += job.footprint;
+            s->active += 1;
+            s->peak_active = std::max(s->peak_active, s->active);
+        }
+        cdg_result res{};
+        res.slot = slot;
+        const clk::time_point t_start = clk::now();
+        res.t_queue = secs(job.t_submit, t_start);
+        int e;
+        if (ce) e = ce;
+        else if (job.footprint > s->budget) e = CDG_ERR_OOM;
+        else e = run_job(ctx, s->params, job, res);
+        const clk::time_point t_end = clk::now();
+        res.status = e;
+        res.t_run = secs(t_start, t_end);
+        res.t_latency = secs(job.t_submit, t_end);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->in_use -= job.footprint;
+            s->active -= 1;
+            Slot &sl = s->results[job.ticket];
+            sl.res = res;
+            sl.done = true;
+            if (e && ctx) s->err = ctx->err;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+    if (ctx) cdg_ctx_destroy(ctx);
+}
+
+}  // namespace
+
+extern "C" int cdg_server_create(int32_t device, int32_t nslots, int64_t hbm_budget, const cdg_fgmres_params *params,
+                                 cdg_server **out) {
+    if (!out || nslots < 1 || nslots > 64) return CDG_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { (void)hipGetLastError(); return CDG_ERR_HIP; }
+    if (device < 0 || device >= ndev) return CDG_ERR_ARG;
+    cdg_server *s = new cdg_server();
+    s->device = device;
+    if (params) s->params = *params;
+    else cdg_fgmres_default_params(&s->params);
+    if (hbm_budget <= 0) {
+        size_t fr = 0, tot = 0;
+        (void)hipSetDevice(device);
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete s; return CDG_ERR_HIP; }
+        hbm_budget = (int64_t)(0.8 * (double)fr);
+    }
+    s->budget = hbm_budget;
+    for (int i = 0; i < nslots; ++i) s->workers.emplace_back(worker, s, i);
+    *out = s;
+    return CDG_OK;
+}
+
+extern "C" int cdg_server_submit(cdg_server *s, const cdg_request *req, double *x_out, uint64_t *ticket) {
+    if (!s || !req || !ticket) return CDG_ERR_ARG;
+    Job job;
+    job.req = *req;
+    job.x_out = x_out;
+    job.t_submit = clk::now();
+    int e = cdg_server_footprint(req, &s->params, &job.footprint);
+    if (e) return e;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return CDG_ERR_STATE;
+        job.ticket = s->next_ticket++;
+        s->results[job.ticket] = Slot{};
+        s->queue.push_back(job);
+        *ticket = job.ticket;
+    }
+    s->cv_work.notify_one();
+    return CDG_OK;
+}
+
+extern "C" int cdg_server_wait(cdg_server *s, uint64_t ticket, cdg_result *res) {
+    if (!s || !res) return CDG_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->results.find(ticket);
+    if (it == s->results.end()) return CDG_ERR_ARG;
+    s->cv_done.wait(lk, [&] { return s->results[ticket].done; });
+    *res = s->results[ticket].res;
+    s->results.erase(ticket);
+    return CDG_OK;
+}
+
+extern "C" int cdg_server_info(cdg_server *s, int32_t *queued, int32_t *active, int32_t *peak_active,
+                               int64_t *in_use, int64_t *budget) {
+    if (!s) return CDG_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (queued) *queued = (int32_t)s->queue.size();
+    if (active) *active = s->active;
+    if (peak_active) *peak_active = s->peak_active;
+    if (in_use) *in_use = s->in_use;
+    if (budget) *budget = s->budget;
+    return CDG_OK;
+}
+
+extern "C" const char *cdg_server_last_error(cdg_server *s) { return s ? s->err.c_str() : "null server"; }
+
+extern "C" int cdg_server_destroy(cdg_server *s) {
+    if (!s) return CDG_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (std::thread &t : s->workers) t.join();
+    delete s;
+    return CDG_OK;
+}

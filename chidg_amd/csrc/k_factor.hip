@@ -345,6 +345,107 @@ __device__ __forceinline__ float readlane(float x, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }
 
+// Gauss-Jordan with partial pivoting on one 16-column panel, run by one
+// wave.  On entry G[r*gp + c] holds the panel (rows r < NP, zero beyond n and
+// beyond the panel width w) and rowtag[r] != 0 marks rows pivoted in earlier
+// panels.  On exit G holds C - E_P (C = [W on the pivot rows, -A(r,P) W
+// elsewhere], W the inverse of the pivot block), rowtag[p] = 1 + step and
+// pivrow[step] = p for this panel's pivots.  The pivot is the largest |a(r,k)|
+// over unused rows, lower row on ties (DPP argmax); rows are never swapped.
+// The panel's columns rotate left by one per step, so the column being
+// eliminated always sits at register position 0 and the loop body is not
+// unrolled; after 16 rotations the columns are back in place.
+template <typename T, int NR>
+__device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, int w, int32_t *rowtag,
+                                         int32_t *pivrow, T *prow_sh) {
+    const int lane = threadIdx.x & 63;
+    T pr[NR][16];
+    bool used[NR];
+#pragma unroll
+    for (int a = 0; a < NR; ++a) {
+        const int r = lane + 64 * a;
+        used[a] = r >= n || (r < NP && rowtag[r] != 0);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) pr[a][c] = r < NP ? G[r * gp + c] : T(0);
+    }
+    bool bad = false;
+#pragma unroll 1
+    for (int j = 0; j < 16; ++j) {
+        if (j < w) {
+            T best = T(-1);
+            int bi = 0x7fffffff, fu = 0x7fffffff;
+#pragma unroll
+            for (int a = 0; a < NR; ++a) {
+                const int r = lane + 64 * a;
+                if (!used[a]) {
+                    const T av = fabs(pr[a][0]);
+                    if (av > best) { best = av; bi = r; }
+                    fu = min(fu, r);
+                }
+            }
+            wave_argmax<T>(best, bi, fu);
+            const T bbest = readlane(best, 63);
+            int p = __builtin_amdgcn_readlane(bi, 63);
+            if (!(bbest > T(0)) || p >= n) { bad = true; p = __builtin_amdgcn_readlane(fu, 63); }
+            const int pa = p >> 6, pl = p & 63;
+            // the pivot row through LDS
+            if (lane == pl) {
+#pragma unroll
+                for (int a = 0; a < NR; ++a)
+                    if (a == pa) {
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) prow_sh[c] = pr[a][c];
+                    }
+            }
+            wave_sync();
+            T prow[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) prow[c] = prow_sh[c];
+            const T rinv = T(1) / prow[0];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) prow[c] *= rinv;
+#pragma unroll
+            for (int a = 0; a < NR; ++a) {
+                const bool isp = (a == pa) && (lane == pl);
+                const T cr = isp ? T(0) : pr[a][0];
+#pragma unroll
+                for (int c = 1; c < 16; ++c) pr[a][c] -= cr * prow[c];
+                pr[a][0] = -cr * rinv;
+                used[a] = used[a] || isp;
+            }
+            if (lane == pl) {
+#pragma unroll
+                for (int a = 0; a < NR; ++a)
+                    if (a == pa) {
+#pragma unroll
+                        for (int c = 1; c < 16; ++c) pr[a][c] = prow[c];
+                        pr[a][0] = rinv;
+                    }
+            }
+            if (lane == 0) { rowtag[p] = k0 + j + 1; pivrow[k0 + j] = p; }
+            wave_sync();   // prow_sh is rewritten by the next step; rowtag is read below
+        }
+#pragma unroll
+        for (int a = 0; a < NR; ++a) {
+            const T first = pr[a][0];
+#pragma unroll
+            for (int c = 0; c < 15; ++c) pr[a][c] = pr[a][c + 1];
+            pr[a][15] = first;
+        }
+    }
+    // C - E_P back to LDS (the ones of E_P sit at (p_j, j))
+#pragma unroll
+    for (int a = 0; a < NR; ++a) {
+        const int r = lane + 64 * a;
+        if (r < NP) {
+            const int tg = rowtag[r] - 1 - k0;     // panel step of this row, if any
+#pragma unroll
+            for (int c = 0; c < 16; ++c) G[r * gp + c] = pr[a][c] - ((tg == c) ? T(1) : T(0));
+        }
+    }
+    return bad;
+}
+
 // Dinv_i = inv(D_i) for blocks of n <= 16*NT, one workgroup of 4 waves per
 // block.  The block lives in MFMA accumulator tiles (16 x 16 tile t =
 // (ti, tj) held by wave t % 4).  Columns are processed in panels of 16:
@@ -408,94 +509,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
         __syncthreads();
         // 2. panel Gauss-Jordan (wave 0)
         if (wv == 0) {
-            T pr[NR][16];
-            bool used[NR];
-#pragma unroll
-            for (int a = 0; a < NR; ++a) {
-                const int r = lane + 64 * a;
-                used[a] = r >= n || (r < NP && rowtag[r] != 0);
-#pragma unroll
-                for (int c = 0; c < 16; ++c) pr[a][c] = r < NP ? G[r * GP + c] : T(0);
-            }
-            bool bad = false;
-            // The panel's columns rotate left by one per step, so the column
-            // being eliminated always sits at register position 0 and the
-            // loop body is not unrolled (code size: one step, not sixteen).
-            // After 16 rotations the columns are back in place.
-#pragma unroll 1
-            for (int j = 0; j < 16; ++j) {
-                if (j < w) {
-                    T best = T(-1);
-                    int bi = 0x7fffffff, fu = 0x7fffffff;
-#pragma unroll
-                    for (int a = 0; a < NR; ++a) {
-                        const int r = lane + 64 * a;
-                        if (!used[a]) {
-                            const T av = fabs(pr[a][0]);
-                            if (av > best) { best = av; bi = r; }
-                            fu = min(fu, r);
-                        }
-                    }
-                    wave_argmax<T>(best, bi, fu);
-                    const T bbest = readlane(best, 63);
-                    int p = __builtin_amdgcn_readlane(bi, 63);
-                    if (!(bbest > T(0)) || p >= n) { bad = true; p = __builtin_amdgcn_readlane(fu, 63); }
-                    const int pa = p >> 6, pl = p & 63;
-                    // the pivot row through LDS
-                    if (lane == pl) {
-#pragma unroll
-                        for (int a = 0; a < NR; ++a)
-                            if (a == pa) {
-#pragma unroll
-                                for (int c = 0; c < 16; ++c) prow_sh[c] = pr[a][c];
-                            }
-                    }
-                    wave_sync();
-                    T prow[16];
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) prow[c] = prow_sh[c];
-                    const T rinv = T(1) / prow[0];
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) prow[c] *= rinv;
-#pragma unroll
-                    for (int a = 0; a < NR; ++a) {
-                        const bool isp = (a == pa) && (lane == pl);
-                        const T cr = isp ? T(0) : pr[a][0];
-#pragma unroll
-                        for (int c = 1; c < 16; ++c) pr[a][c] -= cr * prow[c];
-                        pr[a][0] = -cr * rinv;
-                        used[a] = used[a] || isp;
-                    }
-                    if (lane == pl) {
-#pragma unroll
-                        for (int a = 0; a < NR; ++a)
-                            if (a == pa) {
-#pragma unroll
-                                for (int c = 1; c < 16; ++c) pr[a][c] = prow[c];
-                                pr[a][0] = rinv;
-                            }
-                    }
-                    if (lane == 0) { rowtag[p] = k0 + j + 1; pivrow[k0 + j] = p; }
-                    wave_sync();   // prow_sh is rewritten by the next step
-                }
-#pragma unroll
-                for (int a = 0; a < NR; ++a) {
-                    const T first = pr[a][0];
-#pragma unroll
-                    for (int c = 0; c < 15; ++c) pr[a][c] = pr[a][c + 1];
-                    pr[a][15] = first;
-                }
-            }
-            // C - E_P back to LDS (the ones of E_P sit at (p_j, j))
-#pragma unroll
-            for (int a = 0; a < NR; ++a) {
-                const int r = lane + 64 * a;
-                if (r < NP) {
-                    const int tg = rowtag[r] - 1 - k0;     // panel step of this row, if any
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) G[r * GP + c] = pr[a][c] - ((tg == c) ? T(1) : T(0));
-                }
-            }
+            const bool bad = panel_gj<T, NR>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh);
             if (bad && lane == 0) bad_sh = 1;
         }
         __syncthreads();
@@ -555,6 +569,106 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
     }
 }
 
+// Dinv_i = inv(D_i) for any n <= 64*NR, same blocked Gauss-Jordan as
+// k_fac_binv with the working matrix in global memory (W, one slot per
+// workgroup, L2-resident) instead of registers.  The rank-16 update runs on
+// the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
+// matrix column, so its 16 lanes touch 16 consecutive elements of the
+// column-major block.  LDS (dynamic): the panel (NP x 17), the pivot rows
+// (16 x (NP + 1)), rowtag, pivrow.
+template <typename T, int NR>
+__global__ void __launch_bounds__(256) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
+                                                   const int32_t *__restrict__ rows, int n,
+                                                   int32_t *__restrict__ err) {
+    constexpr int GP = 17;
+    extern __shared__ unsigned char smem_raw[];
+    const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
+    T *G = reinterpret_cast<T *>(smem_raw);                 // NP * GP
+    T *XP = G + (size_t)NP * GP;                            // 16 * XPP
+    T *prow_sh = XP + (size_t)16 * XPP;                     // 16
+    int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);   // NP
+    int32_t *pivrow = rowtag + NP;                          // NP
+    __shared__ int32_t bad_sh;
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int blk = rows[blockIdx.x];
+    T *M = D + (size_t)blk * ld;
+    T *S = Wk + (size_t)blockIdx.x * ldw;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nn = n * n;
+    for (int e = tid; e < nn; e += 256) S[e] = M[e];
+    for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
+    for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
+    if (tid == 0) bad_sh = 0;
+    __syncthreads();
+    for (int kb = 0; kb < NT; ++kb) {
+        const int k0 = 16 * kb;
+        const int w = min(16, n - k0);
+        // 1. panel to LDS (zero beyond n and beyond the panel width)
+        for (int e = tid; e < NP * 16; e += 256) {
+            const int c = e / NP, r = e - c * NP;
+            G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
+        }
+        __syncthreads();
+        // 2. panel Gauss-Jordan (wave 0)
+        if (wv == 0) {
+            const bool bad = panel_gj<T, NR>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh);
+            if (bad && lane == 0) bad_sh = 1;
+        }
+        __syncthreads();
+        // 3. pivot rows X(P, :) of the other columns
+        for (int e = tid; e < w * n; e += 256) {
+            const int c = e / w, j = e - c * w;
+            if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
+        }
+        __syncthreads();
+        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles); the panel takes C
+        for (int t = wv; t < NT * NT; t += 4) {
+            const int ti = t % NT, tj = t / NT;     // rows 16ti.., columns 16tj..
+            const int r = 16 * ti + (lane & 15);
+            if (tj != kb) {
+                acc_t acc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 16 * tj + Mfma<T>::row(lane, q);
+                    acc[q] = (r < n && c < n) ? S[(size_t)c * n + r] : T(0);
+                }
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
+                    const T b = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
+                    acc = Mfma<T>::run(a, b, acc);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 16 * tj + Mfma<T>::row(lane, q);
+                    if (r < n && c < n) S[(size_t)c * n + r] = acc[q];
+                }
+            } else if (r < n) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int cc = Mfma<T>::row(lane, q);
+                    const int c = k0 + cc;
+                    if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && bad_sh) atomicMin(err, blk);
+    // inv(rowstep(r), pivrow(c)) = S(r, c)
+    for (int e = tid; e < nn; e += 256) {
+        const int c = e / n, r = e - c * n;
+        M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
+    }
+}
+
+template <typename T, int NR>
+size_t ginv_lds(int n) {
+    const int NP = 16 * ((n + 15) / 16);
+    return sizeof(T) * ((size_t)NP * 17 + (size_t)16 * (NP + 1) + 16) + sizeof(int32_t) * 2 * (size_t)NP;
+}
+
 __device__ __forceinline__ cdg_bf16 to_bf16(float f) {
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
@@ -582,11 +696,41 @@ __global__ void k_f32_to_bf16(const float *__restrict__ src, cdg_bf16 *__restric
         dst[k] = to_bf16(src[k]);
 }
 
-template <typename T>
-int launch_inv(cdg_ctx *ctx, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n, int32_t *err) {
-    dim3 grid(nrows_lvl), block(256);
+template <typename T, int NR>
+int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n,
+                int32_t *err) {
+    const size_t lds = ginv_lds<T, NR>(n);
+    static bool attr_set = false;
+    if (!attr_set) {
+        CDG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_ginv<T, NR>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = lds <= 160 * 1024;
+    }
+    hipLaunchKernelGGL((k_fac_ginv<T, NR>), dim3(nrows_lvl), dim3(256), lds, ctx->stream, D, ld,
+                       static_cast<T *>(M->inv_w), M->ld, rows, n, err);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+// Block-inverse kernel: 0 register-resident Gauss-Jordan (n <= 192), 1 blocked
+// in registers (n <= 144), 2 blocked in global memory (any n <= 320); larger
+// blocks always take 2.  CDG_INV overrides the default.
+int inv_mode() {
     static const int mode = [] { const char *e = std::getenv("CDG_INV"); return e ? std::atoi(e) : 0; }();
-    if (mode == 1 && n <= 144) {
+    return mode;
+}
+
+template <typename T>
+int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n, int32_t *err) {
+    if ((n > 192 || inv_mode() == 2) && !M->inv_w) return cdg_fail(ctx, CDG_ERR_STATE, "inverse scratch missing");
+    if (n > 192 || inv_mode() == 2) {
+        if (n <= 64) return launch_ginv<T, 1>(ctx, M, D, ld, rows, nrows_lvl, n, err);
+        if (n <= 192) return launch_ginv<T, 3>(ctx, M, D, ld, rows, nrows_lvl, n, err);
+        if (n <= 256) return launch_ginv<T, 4>(ctx, M, D, ld, rows, nrows_lvl, n, err);
+        return launch_ginv<T, 5>(ctx, M, D, ld, rows, nrows_lvl, n, err);
+    }
+    dim3 grid(nrows_lvl), block(256);
+    if (inv_mode() == 1 && n <= 144) {
         if (n <= 48) hipLaunchKernelGGL((k_fac_binv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
         else if (n <= 96) hipLaunchKernelGGL((k_fac_binv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
         else hipLaunchKernelGGL((k_fac_binv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
@@ -618,7 +762,7 @@ int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D) {
             hipLaunchKernelGGL((k_fac_p2<T>), dim3(tiles * tiles, r1 - r0), dim3(GT), 0, ctx->stream, A->blocks, A->ld,
                                L, D, M->ld, M->d_lev_rows + r0, M->d_lo_ptr, M->d_lo_tr, M->d_diag, n, tiles);
             CDG_HIP_TRY(hipGetLastError());
-            int e = launch_inv<T>(ctx, D, M->ld, M->d_lev_rows + r0, r1 - r0, n, M->d_err);
+            int e = launch_inv<T>(ctx, M, D, M->ld, M->d_lev_rows + r0, r1 - r0, n, M->d_err);
             if (e) return e;
         }
     }
@@ -629,10 +773,15 @@ int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D) {
 
 int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row) {
     const int n = M->n;
-    if (M->precision == CDG_FP64 && n > 192)
-        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "FP64 ILU factor for block size %d not built", n);
-    if (n > 192)
-        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU factor for block size %d not built", n);
+    if (n > 320) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU factor for block size %d not built (n <= 320)", n);
+    if ((n > 192 || inv_mode() == 2) && !M->inv_w) {   // working slots of the global-memory inverse
+        int maxrows = 1;
+        for (int l = 0; l < M->nlev; ++l) maxrows = std::max(maxrows, M->lev_row_ptr[l + 1] - M->lev_row_ptr[l]);
+        const size_t bytes = (M->precision == CDG_FP64 ? 8 : 4) * (size_t)maxrows * M->ld;
+        int e = dev_alloc(ctx, &M->inv_w, bytes);
+        if (e) return e;
+        M->owned.push_back({M->inv_w, bytes});
+    }
     const int32_t big = 0x7fffffff;
     CDG_HIP_TRY(hipMemcpyAsync(M->d_err, &big, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     int e = CDG_OK;

@@ -1,0 +1,173 @@
+"""Request loop for parameter-sweep solves (``cdg_server_*`` in
+include/chidg_hip.h): the Python side submits synthetic systems and collects
+results; the server's worker threads, admission against the device-memory
+budget and per-slot HIP streams are native.
+
+``poisson_load`` drives a server with Poisson arrivals of requests drawn from
+a size distribution and reports throughput and latency percentiles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .linsolve import _PRECISION, fgmres
+
+
+@dataclass
+class Request:
+    """A synthetic system: hex box nx*ny*nz, degree ``order`` (P0..P3), periodic
+    faces by bit (1: xi, 2: eta, 4: zeta); blocks 2I + G*s_diag on the diagonal
+    and G*s_off elsewhere (seed_A), right-hand side G (seed_b)."""
+    nx: int
+    ny: int
+    nz: int
+    order: int
+    seed_A: int
+    seed_b: int = 1
+    periodic: int = 7
+    precision: str = "fp32"
+    s_diag: float | None = None
+    s_off: float | None = None
+
+    @property
+    def n(self) -> int:
+        return (self.order + 1) ** 3 * 5
+
+    @property
+    def nrows(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def c(self) -> _lib.Request:
+        sd = self.s_diag if self.s_diag is not None else 1.0 / np.sqrt(self.n)
+        so = self.s_off if self.s_off is not None else 0.1 / np.sqrt(self.n)
+        return _lib.Request(self.nx, self.ny, self.nz, self.order, self.periodic, _PRECISION[self.precision],
+                            self.seed_A, self.seed_b, sd, so)
+
+
+@dataclass
+class Result:
+    status: int
+    niter: int
+    residual_norm: float
+    t_queue: float
+    t_run: float
+    t_latency: float
+    t_factor: float
+    slot: int
+    x: np.ndarray | None = None
+
+
+class Server:
+    """``nslots`` workers on GPU ``device``; ``hbm_budget`` bytes of device memory
+    for running requests (None: 80% of the free memory); FGMRES parameters as
+    in ``linsolve.fgmres``."""
+
+    def __init__(self, nslots: int = 4, device: int = 0, hbm_budget: int | None = None, **fgmres_kw):
+        self.lib = _lib.load()
+        ls = fgmres(**fgmres_kw)
+        self.params = ls.params()
+        h = C.c_void_p()
+        code = self.lib.cdg_server_create(device, nslots, int(hbm_budget or 0), C.byref(self.params), C.byref(h))
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_create", "")
+        self.h = h
+        self._out = {}
+
+    def footprint(self, req: Request) -> int:
+        b = C.c_int64()
+        r = req.c()
+        code = self.lib.cdg_server_footprint(C.byref(r), C.byref(self.params), C.byref(b))
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_footprint", "")
+        return b.value
+
+    def submit(self, req: Request, want_x: bool = False) -> int:
+        x = np.empty(req.nrows * req.n) if want_x else None
+        t = C.c_uint64()
+        r = req.c()
+        xp = x.ctypes.data_as(C.c_void_p) if x is not None else None
+        code = self.lib.cdg_server_submit(self.h, C.byref(r), xp, C.byref(t))
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_submit", "")
+        self._out[t.value] = x
+        return t.value
+
+    def wait(self, ticket: int) -> Result:
+        r = _lib.Result()
+        code = self.lib.cdg_server_wait(self.h, ticket, C.byref(r))
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_wait", "")
+        x = self._out.pop(ticket, None)
+        res = Result(r.status, r.niter, r.residual_norm, r.t_queue, r.t_run, r.t_latency, r.t_factor, r.slot, x)
+        if res.status != _lib.CDG_OK:
+            msg = (self.lib.cdg_server_last_error(self.h) or b"").decode()
+            raise _lib.CdgError(res.status, "request", msg)
+        return res
+
+    def info(self) -> dict:
+        q, a, pk = C.c_int32(), C.c_int32(), C.c_int32()
+        use, bud = C.c_int64(), C.c_int64()
+        self.lib.cdg_server_info(self.h, C.byref(q), C.byref(a), C.byref(pk), C.byref(use), C.byref(bud))
+        return dict(queued=q.value, active=a.value, peak_active=pk.value, in_use=use.value, budget=bud.value)
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.lib.cdg_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sweep_requests(count: int, rng: np.random.Generator, meshes=(4, 8), orders=(1, 2, 3),
+                   precision: str = "fp32", mach=(0.2, 0.6)) -> list[Request]:
+    """Parameter-sweep requests: cube meshes N^3 with N uniform in [meshes[0],
+    meshes[1]], degree uniform over ``orders``; the Mach number M ~ U(mach)
+    scales the off-diagonal coupling, s_off = (M / 0.4) * 0.1 / sqrt(n)."""
+    out = []
+    for k in range(count):
+        N = int(rng.integers(meshes[0], meshes[1] + 1))
+        P = int(rng.choice(orders))
+        M = float(rng.uniform(*mach))
+        n = (P + 1) ** 3 * 5
+        out.append(Request(N, N, N, P, seed_A=int(rng.integers(1, 2**62)), seed_b=int(rng.integers(1, 2**62)),
+                           precision=precision, s_off=(M / 0.4) * 0.1 / np.sqrt(n)))
+    return out
+
+
+def poisson_load(server: Server, reqs: list[Request], rate: float, rng: np.random.Generator) -> dict:
+    """Submit ``reqs`` with exponential inter-arrival times of mean 1/rate
+    (seconds), wait for all; latency = submit -> done (queueing included)."""
+    gaps = rng.exponential(1.0 / rate, size=len(reqs))
+    tickets = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for r, g in zip(reqs, gaps):
+        t_next += g
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(server.submit(r))
+    res = [server.wait(t) for t in tickets]
+    wall = time.perf_counter() - t0
+    lat = np.array([r.t_latency for r in res])
+    return dict(rate=rate, requests=len(reqs), wall_s=wall, solves_per_s=len(reqs) / wall,
+                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                mean_queue_ms=1e3 * float(np.mean([r.t_queue for r in res])),
+                mean_run_ms=1e3 * float(np.mean([r.t_run for r in res])),
+                max_residual=float(max(r.residual_norm for r in res)),
+                peak_active=server.info()["peak_active"])

@@ -165,6 +165,47 @@ int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, const double *
 /* Reference defaults of linear_solver_t (type_linear_solver.f90:21-40). */
 void cdg_fgmres_default_params(cdg_fgmres_params *params);
 
+/* ---- request loop for parameter-sweep solves ----------------------------------
+ * No reference counterpart (ChiDG solves one system per MPI rank); for callers
+ * that solve many independent systems on one GPU.  A server runs `nslots`
+ * workers, each with its own context and HIP stream.  Requests queue in
+ * arrival order; a free worker admits the head of the queue as soon as its
+ * device-memory footprint (cdg_server_footprint) fits `hbm_budget` bytes next
+ * to the running requests (hbm_budget <= 0: 80% of the free device memory),
+ * so work of concurrent requests overlaps on the device.  A request names a
+ * synthetic system: uniform hex box nx*ny*nz in ChiDG element order, degree
+ * `order` (block n = (order+1)^3 * 5), periodic faces by bit (1: xi, 2: eta,
+ * 4: zeta), blocks from cdg_mat_fill_synthetic(seed_A, s_diag, s_off), right-
+ * hand side cdg_vec_fill_synthetic(seed_b, 1), ILU(0) in `precision`; the
+ * server's FGMRES parameters.  Results do not depend on what runs beside a
+ * request.  x_out (may be NULL) receives the solution and must stay valid
+ * until cdg_server_wait returns for that ticket. */
+typedef struct cdg_server cdg_server;
+typedef struct {
+    int32_t nx, ny, nz, order, periodic, precision;
+    uint64_t seed_A, seed_b;
+    double s_diag, s_off;
+} cdg_request;
+typedef struct {
+    int32_t status;            /* CDG_OK or the failing call's code */
+    int32_t niter;
+    double residual_norm;      /* ||b - A x|| / ||b|| */
+    double t_queue;            /* seconds from submit to admission */
+    double t_run;              /* seconds from admission to completion */
+    double t_latency;          /* seconds from submit to completion */
+    double t_factor;           /* ILU(0) update inside t_run */
+    int32_t slot;              /* worker that ran the request */
+} cdg_result;
+int cdg_server_create(int32_t device, int32_t nslots, int64_t hbm_budget, const cdg_fgmres_params *params,
+                      cdg_server **out);
+int cdg_server_submit(cdg_server *s, const cdg_request *req, double *x_out, uint64_t *ticket);
+int cdg_server_wait(cdg_server *s, uint64_t ticket, cdg_result *res);
+int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_params *params, int64_t *bytes);
+int cdg_server_info(cdg_server *s, int32_t *queued, int32_t *active, int32_t *peak_active, int64_t *in_use,
+                    int64_t *budget);
+const char *cdg_server_last_error(cdg_server *s);
+int cdg_server_destroy(cdg_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,13 +31,15 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header(tmp_path):
-    """ctypes mirrors of the two ABI structs have the C compiler's size and offsets."""
+    """ctypes mirrors of the ABI structs have the C compiler's size and offsets."""
     import shutil
     import subprocess
     if not shutil.which("gcc"):
         pytest.skip("gcc not available")
     checks = []
-    for cname, py in (("cdg_fgmres_params", _lib.FgmresParams), ("cdg_fgmres_stats", _lib.FgmresStats)):
+    structs = (("cdg_fgmres_params", _lib.FgmresParams), ("cdg_fgmres_stats", _lib.FgmresStats),
+               ("cdg_request", _lib.Request), ("cdg_result", _lib.Result))
+    for cname, py in structs:
         checks.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             checks.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
@@ -48,7 +50,7 @@ def test_struct_layouts_match_header(tmp_path):
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                          check=True).stdout.splitlines())
-    for cname, py in (("cdg_fgmres_params", _lib.FgmresParams), ("cdg_fgmres_stats", _lib.FgmresStats)):
+    for cname, py in structs:
         assert int(got[cname]) == C.sizeof(py)
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)

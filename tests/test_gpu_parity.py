@@ -274,3 +274,29 @@ def test_full_size_fgmres_fp64(big):
     assert ls.residual_norm / float(torch.linalg.norm(b)) == pytest.approx(true_res, rel=1e-6)
     z1 = M.apply(A, b)
     assert torch.equal(z1, M.apply(A, b))
+
+
+# ------------------------------------------- P3 (320 x 320 blocks, config 3)
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16", 5e-2)])
+def test_p3_factor_apply_fgmres(ctx, precision, tol):
+    """n = 320 (P3, 5 fields): factor through the global-memory block inverse,
+    apply against the FP64 oracle, FGMRES(30) to rtol 1e-8 in FP64."""
+    p, blocks = synthetic_system(3, 2, 2, 3, seed=31, periodic=(True, False, False))
+    A = upload(ctx, p, blocks)
+    M = precon_ILU0(precision).init(A)
+    M.update(A)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    if precision == "fp64":
+        _, D = M.factors(A)
+        assert rel(D, D_ref) < 1e-12
+    v = po.fill_vector(p.vec_len, 9)
+    assert rel(M.apply(A, v), po.ilu0_apply(p, blocks, L_ref, D_ref, v)) < tol
+    b = po.fill_vector(p.vec_len, 10)
+    x = np.zeros(p.vec_len)
+    ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
+    ls.solve(A, x, b, M)
+    x_ref, n_ref, _, _ = po.fgmres(p, blocks, b, nkrylov=30, tol=1e-30, rtol=1e-8)
+    assert ls.residual_norm / np.linalg.norm(b) < 1e-8
+    assert rel(x, x_ref) < 1e-7
+    if precision == "fp64":
+        assert abs(ls.niter - n_ref) <= 1
