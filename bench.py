@@ -131,6 +131,82 @@ def gpu_run(args):
     return p, out
 
 
+def p3_run(args):
+    """16^3 periodic P3 (320 x 320 blocks, A = 23.5 GB FP64): one FGMRES(30)
+    solve to rtol 1e-8 per ILU(0) precision; the FP64-ILU solution is the
+    reference the FP32 and BF16-stored forms are measured against."""
+    import torch
+    from chidg_amd import pattern as pt
+    from chidg_amd.linsolve import Context, chidg_matrix, fgmres, precon_ILU0
+    n = pt.block_size(3, NFIELDS)
+    p = pt.hex_box(N_ELEM, N_ELEM, N_ELEM, n)
+    ctx = Context(0)
+    A = chidg_matrix(p, ctx)
+    A.fill_synthetic(SEED_A, 1.0 / np.sqrt(n), 0.1 / np.sqrt(n))
+    b = A.synthetic_vector(SEED_B, 1.0, like=torch.empty(0, dtype=torch.float64, device="cuda:0"))
+    out, xs = {}, {}
+    for prec in ("fp64", "fp32", "bf16"):
+        M = precon_ILU0(prec).init(A)
+        ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False)
+        x = torch.zeros_like(b)
+        ls.solve(A, x, b, M)             # warm-up (allocations)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ls.solve(A, x, b, M)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        xs[prec] = x
+        out[prec] = dict(solves_per_s=1.0 / dt, ms_per_solve=1e3 * dt, niter=ls.niter,
+                         residual_rel=ls.residual_norm / float(torch.linalg.norm(b)),
+                         t_factor_ms=1e3 * ls.stats.t_factor)
+        log(f"P3 {prec}: {1e3 * dt:.1f} ms/solve, niter {ls.niter}, factor {1e3 * ls.stats.t_factor:.1f} ms")
+        M.tear_down()
+    for prec in ("fp32", "bf16"):
+        out[prec]["x_rel_vs_fp64_ilu"] = float(torch.linalg.norm(xs[prec] - xs["fp64"]) / torch.linalg.norm(xs["fp64"]))
+    A.release()
+    return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in out.items()}
+
+
+def batch_run(args):
+    """64 independent systems, meshes N^3 with N in 4..12, P1..P3 and 1..8
+    right-hand sides drawn uniformly (seeded), FP32 ILU(0), all submitted at
+    once to the request loop (8 slots)."""
+    from chidg_amd.serve import Server, batch, sweep_requests
+    rng = np.random.default_rng(20261202)
+    reqs = sweep_requests(64, rng, meshes=(4, 12), orders=(1, 2, 3), nrhs=(1, 8))
+    with Server(nslots=8, nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False) as srv:
+        out = batch(srv, reqs)
+    log(f"batch: {out['systems']} systems / {out['rhs']} rhs in {out['wall_s']:.2f} s "
+        f"({out['rhs_per_s']:.1f} rhs/s), peak {out['peak_active']} concurrent")
+    return {k: sig(float(v)) if isinstance(v, float) else v for k, v in out.items()}
+
+
+def serve_run(args):
+    """Poisson arrivals of sweep solves (meshes 4^3..8^3, P1..P3, Mach-scaled
+    coupling), FP32 ILU(0), 8 slots; p50/p99 latency from submit to done."""
+    from chidg_amd.serve import Server, poisson_load, sweep_requests
+    out = []
+    with Server(nslots=8, nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False) as srv:
+        warm = sweep_requests(4, np.random.default_rng(1), meshes=(4, 6))
+        poisson_load(srv, warm, rate=1000.0, rng=np.random.default_rng(2))
+        for rate in (10.0, 100.0, 500.0):
+            rng = np.random.default_rng(int(rate))
+            reqs = sweep_requests(args.serve_requests, rng, meshes=(4, 8))
+            r = poisson_load(srv, reqs, rate=rate, rng=rng)
+            log(f"serve rate {rate:.0f}/s: {r['solves_per_s']:.1f} solves/s, p50 {r['p50_ms']:.1f} ms, "
+                f"p99 {r['p99_ms']:.1f} ms, peak {r['peak_active']} concurrent")
+            out.append({k: sig(float(v)) if isinstance(v, float) else v for k, v in r.items()})
+    return out
+
+
+def _guarded(fn, args, what):
+    try:
+        return fn(args)
+    except Exception as e:   # an extra section never costs the headline line
+        log(f"{what} failed: {e!r}")
+        return {"error": repr(e)}
+
+
 def _mv(A, x):
     from chidg_amd.linsolve import chidg_mv
     return chidg_mv(A, x)
@@ -171,6 +247,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--schedule", choices=["levels", "dataflow"], default=None,
                     help="ILU(0) apply schedule (default: the library's)")
+    ap.add_argument("--extras", default="p3,batch,serve",
+                    help="comma list of extra sections: p3 (16^3 P3 mixed precision), batch (64 uneven "
+                         "systems), serve (Poisson request loop); empty for none")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests per arrival rate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -189,6 +269,14 @@ def main():
         del src, dst
     p, g = gpu_run(args)
     f32, f64 = g["fp32"], g["fp64"]
+    extras = [e for e in args.extras.split(",") if e]
+    sections = {}
+    if "p3" in extras:
+        sections["p3_16c_mixed_precision"] = _guarded(p3_run, args, "p3")
+    if "batch" in extras:
+        sections["batch_64_uneven"] = _guarded(batch_run, args, "batch")
+    if "serve" in extras:
+        sections["serving"] = _guarded(serve_run, args, "serve")
     cpu = None
     if not args.no_cpu:
         cpu, x_cpu, n_cpu = cpu_baseline(p, f64["niter"])
@@ -213,6 +301,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    line.update(sections)
     print(json.dumps(line), flush=True)
 
 

@@ -35,7 +35,7 @@ class Request(C.Structure):
     """cdg_request: one synthetic parameter-sweep system."""
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("order", C.c_int32),
                 ("periodic", C.c_int32), ("precision", C.c_int32), ("seed_A", C.c_uint64), ("seed_b", C.c_uint64),
-                ("s_diag", C.c_double), ("s_off", C.c_double)]
+                ("s_diag", C.c_double), ("s_off", C.c_double), ("nrhs", C.c_int32)]
 
 
 class Result(C.Structure):

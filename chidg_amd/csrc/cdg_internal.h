@@ -84,7 +84,8 @@ struct cdg_prec {
     void *U = nullptr;                 // nup slots (FP32/BF16 copies of A's upper blocks)
     void *Dinv = nullptr;              // nrows slots
     float *L32 = nullptr, *D32 = nullptr;   // FP32 work for the BF16 variant
-    void *inv_w = nullptr;             // working slots of the global-memory block inverse (n > 192)
+    void *inv_w = nullptr;             // working slots of the global-memory block inverse
+    int32_t *d_redo = nullptr;         // per row: block inverse must be redone with partial pivoting
     // device index arrays
     int32_t *d_lo_ptr = nullptr, *d_lo_blk = nullptr, *d_lo_col = nullptr, *d_lo_tr = nullptr;
     int32_t *d_up_ptr = nullptr, *d_up_idx = nullptr, *d_up_col = nullptr, *d_up_blk = nullptr;

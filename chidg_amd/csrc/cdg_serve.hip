@@ -137,18 +137,27 @@ int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result 
     if (!e) e = dev_alloc(ctx, (void **)&x, sizeof(double) * len);
     if (!e) e = launch_vec_synthetic(ctx, len, r.seed_b, 1.0, b);
     if (!e) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, r.precision, &M);
-    cdg_fgmres_stats st{};
-    if (!e) e = cdg_fgmres_solve(ctx, A, M, b, x, &params, &st);
-    double bn = 0.0;
-    if (!e) e = ctx_norm2(ctx, len, b, &bn);
-    if (!e && job.x_out) {
-        if (hipMemcpyAsync(job.x_out, x, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
-            e = cdg_fail(ctx, CDG_ERR_HIP, "copying the solution out failed");
+    // right-hand sides k = 0.. share the matrix and the factor (updated once)
+    const int nrhs = std::max(1, r.nrhs);
+    cdg_fgmres_params prm = params;
+    res.niter = 0;
+    res.residual_norm = 0.0;
+    for (int k = 0; k < nrhs && !e; ++k) {
+        if (k > 0) e = launch_vec_synthetic(ctx, len, r.seed_b + (uint64_t)k, 1.0, b);
+        prm.update_preconditioner = k == 0 ? 1 : 0;
+        cdg_fgmres_stats st{};
+        if (!e) e = cdg_fgmres_solve(ctx, A, M, b, x, &prm, &st);
+        double bn = 0.0;
+        if (!e) e = ctx_norm2(ctx, len, b, &bn);
+        if (!e && k == 0 && job.x_out) {
+            if (hipMemcpyAsync(job.x_out, x, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                e = cdg_fail(ctx, CDG_ERR_HIP, "copying the solution out failed");
+        }
+        res.niter += st.niter;
+        res.residual_norm = std::max(res.residual_norm, bn > 0 ? st.residual_norm / bn : st.residual_norm);
+        res.t_factor += st.t_factor;
     }
-    res.niter = st.niter;
-    res.residual_norm = bn > 0 ? st.residual_norm / bn : st.residual_norm;
-    res.t_factor = st.t_factor;
     if (M) cdg_prec_destroy(ctx, M);
     if (A) cdg_mat_destroy(ctx, A);
     dev_free(ctx, b, 0);

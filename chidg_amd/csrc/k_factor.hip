@@ -355,10 +355,18 @@ __device__ __forceinline__ float readlane(float x, int l) {
 // The panel's columns rotate left by one per step, so the column being
 // eliminated always sits at register position 0 and the loop body is not
 // unrolled; after 16 rotations the columns are back in place.
-template <typename T, int NR>
+//
+// PIVOT = false takes row k as the pivot of column k (the rows pivoted so far
+// being 0..k-1) and only checks that partial pivoting would have chosen the
+// same row: no unused row with a strictly larger |a(r,k)| and a nonzero
+// finite pivot.  The check is off the dependency chain of the elimination;
+// when it holds at every step the arithmetic is exactly that of the pivoting
+// path.  *miss reports a step where it did not hold.
+template <typename T, int NR, bool PIVOT>
 __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, int w, int32_t *rowtag,
-                                         int32_t *pivrow, T *prow_sh) {
+                                         int32_t *pivrow, T *prow_sh, bool *miss) {
     const int lane = threadIdx.x & 63;
+    bool viol = false;
     T pr[NR][16];
     bool used[NR];
 #pragma unroll
@@ -372,21 +380,24 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
 #pragma unroll 1
     for (int j = 0; j < 16; ++j) {
         if (j < w) {
-            T best = T(-1);
-            int bi = 0x7fffffff, fu = 0x7fffffff;
+            int p = k0 + j;
+            if (PIVOT) {
+                T best = T(-1);
+                int bi = 0x7fffffff, fu = 0x7fffffff;
 #pragma unroll
-            for (int a = 0; a < NR; ++a) {
-                const int r = lane + 64 * a;
-                if (!used[a]) {
-                    const T av = fabs(pr[a][0]);
-                    if (av > best) { best = av; bi = r; }
-                    fu = min(fu, r);
+                for (int a = 0; a < NR; ++a) {
+                    const int r = lane + 64 * a;
+                    if (!used[a]) {
+                        const T av = fabs(pr[a][0]);
+                        if (av > best) { best = av; bi = r; }
+                        fu = min(fu, r);
+                    }
                 }
+                wave_argmax<T>(best, bi, fu);
+                const T bbest = readlane(best, 63);
+                p = __builtin_amdgcn_readlane(bi, 63);
+                if (!(bbest > T(0)) || p >= n) { bad = true; p = __builtin_amdgcn_readlane(fu, 63); }
             }
-            wave_argmax<T>(best, bi, fu);
-            const T bbest = readlane(best, 63);
-            int p = __builtin_amdgcn_readlane(bi, 63);
-            if (!(bbest > T(0)) || p >= n) { bad = true; p = __builtin_amdgcn_readlane(fu, 63); }
             const int pa = p >> 6, pl = p & 63;
             // the pivot row through LDS
             if (lane == pl) {
@@ -401,6 +412,13 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
             T prow[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) prow[c] = prow_sh[c];
+            if (!PIVOT) {     // would partial pivoting have chosen row p?
+                const T ap = fabs(prow[0]);
+                if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
+#pragma unroll
+                for (int a = 0; a < NR; ++a)
+                    if (!used[a] && (lane + 64 * a) != p && fabs(pr[a][0]) > ap) viol = true;
+            }
             const T rinv = T(1) / prow[0];
 #pragma unroll
             for (int c = 0; c < 16; ++c) prow[c] *= rinv;
@@ -424,6 +442,7 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
             }
             if (lane == 0) { rowtag[p] = k0 + j + 1; pivrow[k0 + j] = p; }
             wave_sync();   // prow_sh is rewritten by the next step; rowtag is read below
+            if (!PIVOT && __builtin_amdgcn_readfirstlane(__ballot(viol) != 0)) break;
         }
 #pragma unroll
         for (int a = 0; a < NR; ++a) {
@@ -443,7 +462,121 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
             for (int c = 0; c < 16; ++c) G[r * gp + c] = pr[a][c] - ((tg == c) ? T(1) : T(0));
         }
     }
+    if (!PIVOT) *miss = __ballot(viol) != 0;
     return bad;
+}
+
+// Blocked Gauss-Jordan inverse of the n x n block M (n <= 64*NR) by one
+// workgroup, working in the global slot S (L2-resident).  The rank-16 update
+// runs on the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
+// matrix column, so its 16 lanes touch 16 consecutive elements of the
+// column-major block.  LDS: the panel G (NP x 17), the pivot rows XP
+// (16 x (NP + 1)), prow_sh, rowtag, pivrow (NP = 16 * ceil(n/16)).  PIVOT =
+// false returns false as soon as a step needs another pivot than the
+// diagonal (M untouched); otherwise M receives the inverse and *bad reports
+// a zero or non-finite pivot column.
+template <typename T, int NR, bool PIVOT>
+__device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh, int32_t *rowtag,
+                          int32_t *pivrow, int32_t *flag_sh, bool *bad) {
+    constexpr int GP = 17;
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nn = n * n;
+    for (int e = tid; e < nn; e += 256) S[e] = M[e];
+    for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
+    for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
+    if (tid == 0) *flag_sh = 0;
+    __syncthreads();
+    for (int kb = 0; kb < NT; ++kb) {
+        const int k0 = 16 * kb;
+        const int w = min(16, n - k0);
+        // 1. panel to LDS (zero beyond n and beyond the panel width)
+        for (int e = tid; e < NP * 16; e += 256) {
+            const int c = e / NP, r = e - c * NP;
+            G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
+        }
+        __syncthreads();
+        // 2. panel Gauss-Jordan (wave 0); flag: zero pivot (PIVOT) or another pivot needed (!PIVOT)
+        if (wv == 0) {
+            bool miss = false;
+            const bool b = panel_gj<T, NR, PIVOT>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
+            if (lane == 0 && (PIVOT ? b : miss)) *flag_sh = 1;
+        }
+        __syncthreads();
+        if (!PIVOT && __builtin_amdgcn_readfirstlane(*flag_sh)) return false;
+        // 3. pivot rows X(P, :) of the other columns
+        for (int e = tid; e < w * n; e += 256) {
+            const int c = e / w, j = e - c * w;
+            if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
+        }
+        __syncthreads();
+        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles); the panel takes C
+        for (int t = wv; t < NT * NT; t += 4) {
+            const int ti = t % NT, tj = t / NT;     // rows 16ti.., columns 16tj..
+            const int r = 16 * ti + (lane & 15);
+            if (tj != kb) {
+                acc_t acc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 16 * tj + Mfma<T>::row(lane, q);
+                    acc[q] = (r < n && c < n) ? S[(size_t)c * n + r] : T(0);
+                }
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
+                    const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
+                    acc = Mfma<T>::run(a, bb, acc);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 16 * tj + Mfma<T>::row(lane, q);
+                    if (r < n && c < n) S[(size_t)c * n + r] = acc[q];
+                }
+            } else if (r < n) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int cc = Mfma<T>::row(lane, q);
+                    const int c = k0 + cc;
+                    if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (PIVOT) *bad = __builtin_amdgcn_readfirstlane(*flag_sh) != 0;
+    // inv(rowstep(r), pivrow(c)) = S(r, c)
+    for (int e = tid; e < nn; e += 256) {
+        const int c = e / n, r = e - c * n;
+        M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
+    }
+    return true;
+}
+
+// Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body with diagonal pivots,
+// repeated with partial pivoting when the check fails.  LDS is dynamic.
+template <typename T, int NR>
+__global__ void __launch_bounds__(256) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
+                                                   const int32_t *__restrict__ rows, int n,
+                                                   int32_t *__restrict__ err) {
+    extern __shared__ unsigned char smem_raw[];
+    const int NP = 16 * ((n + 15) / 16), XPP = NP + 1;
+    T *G = reinterpret_cast<T *>(smem_raw);                 // NP * 17
+    T *XP = G + (size_t)NP * 17;                            // 16 * XPP
+    T *prow_sh = XP + (size_t)16 * XPP;                     // 16
+    int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);   // NP
+    int32_t *pivrow = rowtag + NP;                          // NP
+    __shared__ int32_t flag_sh;
+    const int blk = rows[blockIdx.x];
+    T *M = D + (size_t)blk * ld;
+    T *S = Wk + (size_t)blockIdx.x * ldw;
+    bool bad = false;
+    if (!ginv_body<T, NR, false>(M, S, n, G, XP, prow_sh, rowtag, pivrow, &flag_sh, &bad)) {
+        __syncthreads();
+        ginv_body<T, NR, true>(M, S, n, G, XP, prow_sh, rowtag, pivrow, &flag_sh, &bad);
+    }
+    if (threadIdx.x == 0 && bad) atomicMin(err, blk);
 }
 
 // Dinv_i = inv(D_i) for blocks of n <= 16*NT, one workgroup of 4 waves per
@@ -462,7 +595,8 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
 // permutation.
 template <typename T, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err) {
+k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err,
+           int32_t *__restrict__ redo) {
     constexpr int NP = 16 * NT;                  // padded size
     constexpr int TPW = (NT * NT + 3) / 4;       // tiles per wave
     constexpr int NR = (NP + 63) / 64;           // panel rows per lane of wave 0
@@ -472,7 +606,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
     __shared__ T XP[16 * XPP];                   // pivot rows of the other columns
     __shared__ int32_t rowtag[NP];               // 1 + step at which the row was pivot (0: not yet)
     __shared__ int32_t pivrow[NP];
-    __shared__ int32_t bad_sh;
+    __shared__ int32_t bad_sh, miss_sh;
     __shared__ T prow_sh[16];
     typedef typename Mfma<T>::acc_t acc_t;
     const int blk = rows[blockIdx.x];
@@ -480,6 +614,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     acc_t acc[TPW];
+    // diagonal pivots, checked; M is not written before the end
 #pragma unroll
     for (int s = 0; s < TPW; ++s) {
         const int t = wv + 4 * s;
@@ -490,9 +625,10 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
             acc[s][r] = (t < NT * NT && row < n && col < n) ? M[(size_t)col * n + row] : T(0);
         }
     }
+    __syncthreads();
     for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
     for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
-    if (tid == 0) bad_sh = 0;
+    if (tid == 0) { bad_sh = 0; miss_sh = 0; }
     for (int kb = 0; kb < NT; ++kb) {
         const int k0 = 16 * kb;
         const int w = min(16, n - k0);
@@ -509,10 +645,12 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
         __syncthreads();
         // 2. panel Gauss-Jordan (wave 0)
         if (wv == 0) {
-            const bool bad = panel_gj<T, NR>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh);
-            if (bad && lane == 0) bad_sh = 1;
+            bool miss = false;
+            (void)panel_gj<T, NR, false>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
+            if (miss && lane == 0) miss_sh = 1;
         }
         __syncthreads();
+        if (__builtin_amdgcn_readfirstlane(miss_sh)) break;
         // 3. pivot rows of the other columns
 #pragma unroll
         for (int s = 0; s < TPW; ++s) {
@@ -554,7 +692,12 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
         }
         __syncthreads();
     }
-    if (tid == 0 && bad_sh) atomicMin(err, blk);
+    if (__builtin_amdgcn_readfirstlane(miss_sh)) {
+        // another pivot than the diagonal was needed: M is untouched and the
+        // block is left to k_fac_pivot (partial pivoting)
+        if (tid == 0) redo[blk] = 1;
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < TPW; ++s) {
         const int t = wv + 4 * s;
@@ -569,97 +712,28 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
     }
 }
 
-// Dinv_i = inv(D_i) for any n <= 64*NR, same blocked Gauss-Jordan as
-// k_fac_binv with the working matrix in global memory (W, one slot per
-// workgroup, L2-resident) instead of registers.  The rank-16 update runs on
-// the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
-// matrix column, so its 16 lanes touch 16 consecutive elements of the
-// column-major block.  LDS (dynamic): the panel (NP x 17), the pivot rows
-// (16 x (NP + 1)), rowtag, pivrow.
+// Partial-pivoting inverse of the blocks k_fac_binv flagged in redo[]; the
+// other workgroups leave at once.
 template <typename T, int NR>
-__global__ void __launch_bounds__(256) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
-                                                   const int32_t *__restrict__ rows, int n,
-                                                   int32_t *__restrict__ err) {
-    constexpr int GP = 17;
-    extern __shared__ unsigned char smem_raw[];
-    const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
-    T *G = reinterpret_cast<T *>(smem_raw);                 // NP * GP
-    T *XP = G + (size_t)NP * GP;                            // 16 * XPP
-    T *prow_sh = XP + (size_t)16 * XPP;                     // 16
-    int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);   // NP
-    int32_t *pivrow = rowtag + NP;                          // NP
-    __shared__ int32_t bad_sh;
-    typedef typename Mfma<T>::acc_t acc_t;
+__global__ void __launch_bounds__(256) k_fac_pivot(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
+                                                    const int32_t *__restrict__ rows, int n,
+                                                    int32_t *__restrict__ err, int32_t *__restrict__ redo) {
     const int blk = rows[blockIdx.x];
-    T *M = D + (size_t)blk * ld;
-    T *S = Wk + (size_t)blockIdx.x * ldw;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int nn = n * n;
-    for (int e = tid; e < nn; e += 256) S[e] = M[e];
-    for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
-    for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
-    if (tid == 0) bad_sh = 0;
-    __syncthreads();
-    for (int kb = 0; kb < NT; ++kb) {
-        const int k0 = 16 * kb;
-        const int w = min(16, n - k0);
-        // 1. panel to LDS (zero beyond n and beyond the panel width)
-        for (int e = tid; e < NP * 16; e += 256) {
-            const int c = e / NP, r = e - c * NP;
-            G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
-        }
-        __syncthreads();
-        // 2. panel Gauss-Jordan (wave 0)
-        if (wv == 0) {
-            const bool bad = panel_gj<T, NR>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh);
-            if (bad && lane == 0) bad_sh = 1;
-        }
-        __syncthreads();
-        // 3. pivot rows X(P, :) of the other columns
-        for (int e = tid; e < w * n; e += 256) {
-            const int c = e / w, j = e - c * w;
-            if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
-        }
-        __syncthreads();
-        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles); the panel takes C
-        for (int t = wv; t < NT * NT; t += 4) {
-            const int ti = t % NT, tj = t / NT;     // rows 16ti.., columns 16tj..
-            const int r = 16 * ti + (lane & 15);
-            if (tj != kb) {
-                acc_t acc;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int c = 16 * tj + Mfma<T>::row(lane, q);
-                    acc[q] = (r < n && c < n) ? S[(size_t)c * n + r] : T(0);
-                }
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
-                    const T b = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
-                    acc = Mfma<T>::run(a, b, acc);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int c = 16 * tj + Mfma<T>::row(lane, q);
-                    if (r < n && c < n) S[(size_t)c * n + r] = acc[q];
-                }
-            } else if (r < n) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int cc = Mfma<T>::row(lane, q);
-                    const int c = k0 + cc;
-                    if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (tid == 0 && bad_sh) atomicMin(err, blk);
-    // inv(rowstep(r), pivrow(c)) = S(r, c)
-    for (int e = tid; e < nn; e += 256) {
-        const int c = e / n, r = e - c * n;
-        M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
+    if (__builtin_amdgcn_readfirstlane(redo[blk]) == 0) return;
+    extern __shared__ unsigned char smem_raw[];
+    const int NP = 16 * ((n + 15) / 16), XPP = NP + 1;
+    T *G = reinterpret_cast<T *>(smem_raw);
+    T *XP = G + (size_t)NP * 17;
+    T *prow_sh = XP + (size_t)16 * XPP;
+    int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);
+    int32_t *pivrow = rowtag + NP;
+    __shared__ int32_t flag_sh;
+    bool bad = false;
+    ginv_body<T, NR, true>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh, rowtag, pivrow,
+                           &flag_sh, &bad);
+    if (threadIdx.x == 0) {
+        redo[blk] = 0;
+        if (bad) atomicMin(err, blk);
     }
 }
 
@@ -722,7 +796,7 @@ int inv_mode() {
 
 template <typename T>
 int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n, int32_t *err) {
-    if ((n > 192 || inv_mode() == 2) && !M->inv_w) return cdg_fail(ctx, CDG_ERR_STATE, "inverse scratch missing");
+    if (!M->inv_w) return cdg_fail(ctx, CDG_ERR_STATE, "inverse scratch missing");
     if (n > 192 || inv_mode() == 2) {
         if (n <= 64) return launch_ginv<T, 1>(ctx, M, D, ld, rows, nrows_lvl, n, err);
         if (n <= 192) return launch_ginv<T, 3>(ctx, M, D, ld, rows, nrows_lvl, n, err);
@@ -731,9 +805,16 @@ int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows,
     }
     dim3 grid(nrows_lvl), block(256);
     if (inv_mode() == 1 && n <= 144) {
-        if (n <= 48) hipLaunchKernelGGL((k_fac_binv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
-        else if (n <= 96) hipLaunchKernelGGL((k_fac_binv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
-        else hipLaunchKernelGGL((k_fac_binv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err);
+        if (n <= 48) hipLaunchKernelGGL((k_fac_binv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err, M->d_redo);
+        else if (n <= 96) hipLaunchKernelGGL((k_fac_binv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err, M->d_redo);
+        else hipLaunchKernelGGL((k_fac_binv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err, M->d_redo);
+        CDG_HIP_TRY(hipGetLastError());
+        // blocks that needed another pivot than the diagonal
+        const size_t lds = ginv_lds<T, 3>(n);
+        if (n <= 64) hipLaunchKernelGGL((k_fac_pivot<T, 1>), grid, block, lds, ctx->stream, D, ld,
+                                        static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
+        else hipLaunchKernelGGL((k_fac_pivot<T, 3>), grid, block, lds, ctx->stream, D, ld,
+                                static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
         CDG_HIP_TRY(hipGetLastError());
         return CDG_OK;
     }
@@ -774,7 +855,13 @@ int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D) {
 int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row) {
     const int n = M->n;
     if (n > 320) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU factor for block size %d not built (n <= 320)", n);
-    if ((n > 192 || inv_mode() == 2) && !M->inv_w) {   // working slots of the global-memory inverse
+    if (!M->d_redo) {
+        int e = dev_alloc(ctx, (void **)&M->d_redo, sizeof(int32_t) * std::max(1, M->nrows));
+        if (e) return e;
+        M->owned.push_back({M->d_redo, sizeof(int32_t) * std::max(1, M->nrows)});
+        CDG_HIP_TRY(hipMemsetAsync(M->d_redo, 0, sizeof(int32_t) * std::max(1, M->nrows), ctx->stream));
+    }
+    if (!M->inv_w) {   // global working slots of the block inverse, one per row of a level
         int maxrows = 1;
         for (int l = 0; l < M->nlev; ++l) maxrows = std::max(maxrows, M->lev_row_ptr[l + 1] - M->lev_row_ptr[l]);
         const size_t bytes = (M->precision == CDG_FP64 ? 8 : 4) * (size_t)maxrows * M->ld;

@@ -33,6 +33,7 @@ class Request:
     precision: str = "fp32"
     s_diag: float | None = None
     s_off: float | None = None
+    nrhs: int = 1
 
     @property
     def n(self) -> int:
@@ -46,7 +47,7 @@ class Request:
         sd = self.s_diag if self.s_diag is not None else 1.0 / np.sqrt(self.n)
         so = self.s_off if self.s_off is not None else 0.1 / np.sqrt(self.n)
         return _lib.Request(self.nx, self.ny, self.nz, self.order, self.periodic, _PRECISION[self.precision],
-                            self.seed_A, self.seed_b, sd, so)
+                            self.seed_A, self.seed_b, sd, so, self.nrhs)
 
 
 @dataclass
@@ -134,10 +135,11 @@ class Server:
 
 
 def sweep_requests(count: int, rng: np.random.Generator, meshes=(4, 8), orders=(1, 2, 3),
-                   precision: str = "fp32", mach=(0.2, 0.6)) -> list[Request]:
+                   precision: str = "fp32", mach=(0.2, 0.6), nrhs=(1, 1)) -> list[Request]:
     """Parameter-sweep requests: cube meshes N^3 with N uniform in [meshes[0],
-    meshes[1]], degree uniform over ``orders``; the Mach number M ~ U(mach)
-    scales the off-diagonal coupling, s_off = (M / 0.4) * 0.1 / sqrt(n)."""
+    meshes[1]], degree uniform over ``orders``, right-hand sides uniform in
+    [nrhs[0], nrhs[1]]; the Mach number M ~ U(mach) scales the off-diagonal
+    coupling, s_off = (M / 0.4) * 0.1 / sqrt(n)."""
     out = []
     for k in range(count):
         N = int(rng.integers(meshes[0], meshes[1] + 1))
@@ -145,8 +147,22 @@ def sweep_requests(count: int, rng: np.random.Generator, meshes=(4, 8), orders=(
         M = float(rng.uniform(*mach))
         n = (P + 1) ** 3 * 5
         out.append(Request(N, N, N, P, seed_A=int(rng.integers(1, 2**62)), seed_b=int(rng.integers(1, 2**62)),
-                           precision=precision, s_off=(M / 0.4) * 0.1 / np.sqrt(n)))
+                           precision=precision, s_off=(M / 0.4) * 0.1 / np.sqrt(n),
+                           nrhs=int(rng.integers(nrhs[0], nrhs[1] + 1))))
     return out
+
+
+def batch(server: Server, reqs: list[Request]) -> dict:
+    """Submit every request at once and wait for all (a batch of independent
+    systems); throughput in right-hand sides solved per second."""
+    t0 = time.perf_counter()
+    tickets = [server.submit(r) for r in reqs]
+    res = [server.wait(t) for t in tickets]
+    wall = time.perf_counter() - t0
+    nrhs = sum(max(1, r.nrhs) for r in reqs)
+    return dict(systems=len(reqs), rhs=nrhs, wall_s=wall, rhs_per_s=nrhs / wall, systems_per_s=len(reqs) / wall,
+                iterations=int(sum(r.niter for r in res)), max_residual=float(max(r.residual_norm for r in res)),
+                peak_active=server.info()["peak_active"])
 
 
 def poisson_load(server: Server, reqs: list[Request], rate: float, rng: np.random.Generator) -> dict:

@@ -178,18 +178,19 @@ void cdg_fgmres_default_params(cdg_fgmres_params *params);
  * 4: zeta), blocks from cdg_mat_fill_synthetic(seed_A, s_diag, s_off), right-
  * hand side cdg_vec_fill_synthetic(seed_b, 1), ILU(0) in `precision`; the
  * server's FGMRES parameters.  Results do not depend on what runs beside a
- * request.  x_out (may be NULL) receives the solution and must stay valid
- * until cdg_server_wait returns for that ticket. */
+ * request.  x_out (may be NULL) receives the solution of the first right-hand
+ * side and must stay valid until cdg_server_wait returns for that ticket. */
 typedef struct cdg_server cdg_server;
 typedef struct {
     int32_t nx, ny, nz, order, periodic, precision;
     uint64_t seed_A, seed_b;
     double s_diag, s_off;
+    int32_t nrhs;              /* right-hand sides seed_b, seed_b+1, ... (0 or 1: one); one factor */
 } cdg_request;
 typedef struct {
     int32_t status;            /* CDG_OK or the failing call's code */
-    int32_t niter;
-    double residual_norm;      /* ||b - A x|| / ||b|| */
+    int32_t niter;             /* FGMRES iterations, summed over the right-hand sides */
+    double residual_norm;      /* max over right-hand sides of ||b - A x|| / ||b|| */
     double t_queue;            /* seconds from submit to admission */
     double t_run;              /* seconds from admission to completion */
     double t_latency;          /* seconds from submit to completion */

@@ -55,3 +55,16 @@ def test_poisson_burst_completes():
     assert out["requests"] == 16
     assert out["max_residual"] < 1e-8
     assert out["p99_ms"] >= out["p50_ms"] > 0
+
+
+def test_multi_rhs_shares_factor():
+    """nrhs right-hand sides on one factor: the first equals a one-RHS request
+    bitwise, every one converges."""
+    one = Request(6, 6, 6, 2, seed_A=301, seed_b=11)
+    three = Request(6, 6, 6, 2, seed_A=301, seed_b=11, nrhs=3)
+    with Server(nslots=2, **KW) as s:
+        a = s.wait(s.submit(one, want_x=True))
+        b = s.wait(s.submit(three, want_x=True))
+    assert np.array_equal(a.x, b.x)
+    assert b.niter >= 3 * 1 and b.niter >= a.niter
+    assert b.residual_norm < 1e-8
