@@ -442,7 +442,6 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
             }
             if (lane == 0) { rowtag[p] = k0 + j + 1; pivrow[k0 + j] = p; }
             wave_sync();   // prow_sh is rewritten by the next step; rowtag is read below
-            if (!PIVOT && __builtin_amdgcn_readfirstlane(__ballot(viol) != 0)) break;
         }
 #pragma unroll
         for (int a = 0; a < NR; ++a) {
@@ -475,7 +474,7 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
 // false returns false as soon as a step needs another pivot than the
 // diagonal (M untouched); otherwise M receives the inverse and *bad reports
 // a zero or non-finite pivot column.
-template <typename T, int NR, bool PIVOT>
+template <typename T, int NR, bool PIVOT, int NTH>
 __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh, int32_t *rowtag,
                           int32_t *pivrow, int32_t *flag_sh, bool *bad) {
     constexpr int GP = 17;
@@ -484,16 +483,16 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nn = n * n;
-    for (int e = tid; e < nn; e += 256) S[e] = M[e];
-    for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
-    for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
+    for (int e = tid; e < nn; e += NTH) S[e] = M[e];
+    for (int i = tid; i < NP; i += NTH) { rowtag[i] = 0; pivrow[i] = 0; }
+    for (int i = tid; i < 16 * XPP; i += NTH) XP[i] = T(0);
     if (tid == 0) *flag_sh = 0;
     __syncthreads();
     for (int kb = 0; kb < NT; ++kb) {
         const int k0 = 16 * kb;
         const int w = min(16, n - k0);
         // 1. panel to LDS (zero beyond n and beyond the panel width)
-        for (int e = tid; e < NP * 16; e += 256) {
+        for (int e = tid; e < NP * 16; e += NTH) {
             const int c = e / NP, r = e - c * NP;
             G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
         }
@@ -507,39 +506,53 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
         __syncthreads();
         if (!PIVOT && __builtin_amdgcn_readfirstlane(*flag_sh)) return false;
         // 3. pivot rows X(P, :) of the other columns
-        for (int e = tid; e < w * n; e += 256) {
+        for (int e = tid; e < w * n; e += NTH) {
             const int c = e / w, j = e - c * w;
             if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
         }
         __syncthreads();
-        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles); the panel takes C
-        for (int t = wv; t < NT * NT; t += 4) {
-            const int ti = t % NT, tj = t / NT;     // rows 16ti.., columns 16tj..
-            const int r = 16 * ti + (lane & 15);
-            if (tj != kb) {
-                acc_t acc;
+        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles); the panel takes C.
+        //    Each wave loads UB tiles before multiplying, so UB * NTH/64
+        //    tile loads are in flight per block.
+        constexpr int NWV = NTH / 64, UB = 4;
+        for (int t0 = wv; t0 < NT * NT; t0 += NWV * UB) {
+            acc_t acc[UB];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int t = t0 + NWV * u;
+                const int ti = t % NT, tj = t / NT;
+                const int r = 16 * ti + (lane & 15);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int c = 16 * tj + Mfma<T>::row(lane, q);
-                    acc[q] = (r < n && c < n) ? S[(size_t)c * n + r] : T(0);
+                    acc[u][q] = (t < NT * NT && tj != kb && r < n && c < n) ? S[(size_t)c * n + r] : T(0);
                 }
+            }
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
-                    const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
-                    acc = Mfma<T>::run(a, bb, acc);
-                }
+            for (int u = 0; u < UB; ++u) {
+                const int t = t0 + NWV * u;
+                if (t >= NT * NT) continue;
+                const int ti = t % NT, tj = t / NT;     // rows 16ti.., columns 16tj..
+                const int r = 16 * ti + (lane & 15);
+                if (tj != kb) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int c = 16 * tj + Mfma<T>::row(lane, q);
-                    if (r < n && c < n) S[(size_t)c * n + r] = acc[q];
-                }
-            } else if (r < n) {
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
+                        const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
+                        acc[u] = Mfma<T>::run(a, bb, acc[u]);
+                    }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int cc = Mfma<T>::row(lane, q);
-                    const int c = k0 + cc;
-                    if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = 16 * tj + Mfma<T>::row(lane, q);
+                        if (r < n && c < n) S[(size_t)c * n + r] = acc[u][q];
+                    }
+                } else if (r < n) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int cc = Mfma<T>::row(lane, q);
+                        const int c = k0 + cc;
+                        if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
+                    }
                 }
             }
         }
@@ -547,19 +560,21 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
     }
     if (PIVOT) *bad = __builtin_amdgcn_readfirstlane(*flag_sh) != 0;
     // inv(rowstep(r), pivrow(c)) = S(r, c)
-    for (int e = tid; e < nn; e += 256) {
+    for (int e = tid; e < nn; e += NTH) {
         const int c = e / n, r = e - c * n;
         M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
     }
     return true;
 }
 
-// Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body with diagonal pivots,
-// repeated with partial pivoting when the check fails.  LDS is dynamic.
+// Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body with diagonal pivots; a
+// block whose check fails is flagged for k_fac_pivot.  LDS is dynamic.
+constexpr int kInvThreads = 512;   // global-memory inverse: 8 waves per block
+
 template <typename T, int NR>
-__global__ void __launch_bounds__(256) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
+__global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
                                                    const int32_t *__restrict__ rows, int n,
-                                                   int32_t *__restrict__ err) {
+                                                   int32_t *__restrict__ redo) {
     extern __shared__ unsigned char smem_raw[];
     const int NP = 16 * ((n + 15) / 16), XPP = NP + 1;
     T *G = reinterpret_cast<T *>(smem_raw);                 // NP * 17
@@ -572,11 +587,9 @@ __global__ void __launch_bounds__(256) k_fac_ginv(T *__restrict__ D, int64_t ld,
     T *M = D + (size_t)blk * ld;
     T *S = Wk + (size_t)blockIdx.x * ldw;
     bool bad = false;
-    if (!ginv_body<T, NR, false>(M, S, n, G, XP, prow_sh, rowtag, pivrow, &flag_sh, &bad)) {
-        __syncthreads();
-        ginv_body<T, NR, true>(M, S, n, G, XP, prow_sh, rowtag, pivrow, &flag_sh, &bad);
+    if (!ginv_body<T, NR, false, kInvThreads>(M, S, n, G, XP, prow_sh, rowtag, pivrow, &flag_sh, &bad)) {
+        if (threadIdx.x == 0) redo[blk] = 1;    // left to k_fac_pivot; M untouched
     }
-    if (threadIdx.x == 0 && bad) atomicMin(err, blk);
 }
 
 // Dinv_i = inv(D_i) for blocks of n <= 16*NT, one workgroup of 4 waves per
@@ -715,7 +728,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
 // Partial-pivoting inverse of the blocks k_fac_binv flagged in redo[]; the
 // other workgroups leave at once.
 template <typename T, int NR>
-__global__ void __launch_bounds__(256) k_fac_pivot(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
+__global__ void __launch_bounds__(kInvThreads) k_fac_pivot(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
                                                     const int32_t *__restrict__ rows, int n,
                                                     int32_t *__restrict__ err, int32_t *__restrict__ redo) {
     const int blk = rows[blockIdx.x];
@@ -729,8 +742,8 @@ __global__ void __launch_bounds__(256) k_fac_pivot(T *__restrict__ D, int64_t ld
     int32_t *pivrow = rowtag + NP;
     __shared__ int32_t flag_sh;
     bool bad = false;
-    ginv_body<T, NR, true>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh, rowtag, pivrow,
-                           &flag_sh, &bad);
+    ginv_body<T, NR, true, kInvThreads>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh, rowtag,
+                                        pivrow, &flag_sh, &bad);
     if (threadIdx.x == 0) {
         redo[blk] = 0;
         if (bad) atomicMin(err, blk);
@@ -780,40 +793,57 @@ int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr_set = lds <= 160 * 1024;
     }
-    hipLaunchKernelGGL((k_fac_ginv<T, NR>), dim3(nrows_lvl), dim3(256), lds, ctx->stream, D, ld,
-                       static_cast<T *>(M->inv_w), M->ld, rows, n, err);
+    hipLaunchKernelGGL((k_fac_ginv<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,
+                       static_cast<T *>(M->inv_w), M->ld, rows, n, M->d_redo);
+    CDG_HIP_TRY(hipGetLastError());
+    static bool attr_p = false;
+    if (!attr_p) {
+        CDG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_pivot<T, NR>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_p = lds <= 160 * 1024;
+    }
+    hipLaunchKernelGGL((k_fac_pivot<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,
+                       static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
 
 // Block-inverse kernel: 0 register-resident Gauss-Jordan (n <= 192), 1 blocked
-// in registers (n <= 144), 2 blocked in global memory (any n <= 320); larger
-// blocks always take 2.  CDG_INV overrides the default.
-int inv_mode() {
-    static const int mode = [] { const char *e = std::getenv("CDG_INV"); return e ? std::atoi(e) : 0; }();
+// in registers with checked diagonal pivots (n <= 144), 2 blocked in global
+// memory (any n <= 320; larger blocks always take it).  CDG_INV overrides
+// the default, which is the fastest measured at 16^3: 1 for FP32 and for
+// n <= 48, else 0 (the FP64 register tiles of 1 spill at n = 135).
+int inv_mode_env() {
+    static const int mode = [] { const char *e = std::getenv("CDG_INV"); return e ? std::atoi(e) : -1; }();
     return mode;
+}
+template <typename T> int inv_mode(int n) {
+    const int m = inv_mode_env();
+    if (m >= 0) return m;
+    return (sizeof(T) == 4 && n <= 144) || n <= 48 ? 1 : 0;
 }
 
 template <typename T>
 int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n, int32_t *err) {
     if (!M->inv_w) return cdg_fail(ctx, CDG_ERR_STATE, "inverse scratch missing");
-    if (n > 192 || inv_mode() == 2) {
+    const int mode = inv_mode<T>(n);
+    if (n > 192 || mode == 2) {
         if (n <= 64) return launch_ginv<T, 1>(ctx, M, D, ld, rows, nrows_lvl, n, err);
         if (n <= 192) return launch_ginv<T, 3>(ctx, M, D, ld, rows, nrows_lvl, n, err);
         if (n <= 256) return launch_ginv<T, 4>(ctx, M, D, ld, rows, nrows_lvl, n, err);
         return launch_ginv<T, 5>(ctx, M, D, ld, rows, nrows_lvl, n, err);
     }
     dim3 grid(nrows_lvl), block(256);
-    if (inv_mode() == 1 && n <= 144) {
+    if (mode == 1 && n <= 144) {
         if (n <= 48) hipLaunchKernelGGL((k_fac_binv<T, 3>), grid, block, 0, ctx->stream, D, ld, rows, n, err, M->d_redo);
         else if (n <= 96) hipLaunchKernelGGL((k_fac_binv<T, 6>), grid, block, 0, ctx->stream, D, ld, rows, n, err, M->d_redo);
         else hipLaunchKernelGGL((k_fac_binv<T, 9>), grid, block, 0, ctx->stream, D, ld, rows, n, err, M->d_redo);
         CDG_HIP_TRY(hipGetLastError());
         // blocks that needed another pivot than the diagonal
         const size_t lds = ginv_lds<T, 3>(n);
-        if (n <= 64) hipLaunchKernelGGL((k_fac_pivot<T, 1>), grid, block, lds, ctx->stream, D, ld,
+        if (n <= 64) hipLaunchKernelGGL((k_fac_pivot<T, 1>), grid, dim3(kInvThreads), lds, ctx->stream, D, ld,
                                         static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
-        else hipLaunchKernelGGL((k_fac_pivot<T, 3>), grid, block, lds, ctx->stream, D, ld,
+        else hipLaunchKernelGGL((k_fac_pivot<T, 3>), grid, dim3(kInvThreads), lds, ctx->stream, D, ld,
                                 static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
         CDG_HIP_TRY(hipGetLastError());
         return CDG_OK;
