@@ -465,6 +465,68 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
     return bad;
 }
 
+// The checked-diagonal panel of panel_gj<T, NR, false>, spread over the NW
+// waves of the workgroup: row r sits in wave r % NW, lane r / NW (NP <=
+// 64 * NW), so every lane holds one row.  The pivot row of step j (row k0+j)
+// is published by its owner through a double-buffered LDS row, one barrier
+// per step (a wave can only overwrite buffer j % 2 after every wave has
+// passed step j + 1's barrier, i.e. has read it).  Same arithmetic, same
+// check, same outputs (G = C - E_P, rowtag, pivrow); returns whether some
+// step needed another pivot.  Called by all 64 * NW threads.
+template <typename T, int NW>
+__device__ __forceinline__ bool panel_gjw(T *G, int gp, int NP, int n, int k0, int w, int32_t *rowtag,
+                                          int32_t *pivrow, T (*prow_sh)[16], int32_t *miss_sh) {
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63;
+    const int row = lane * NW + wv;                        // this lane's panel row
+    T pr[16];
+    const bool live = row < NP;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) pr[c] = live ? G[row * gp + c] : T(0);
+    bool used = row >= n || (live && rowtag[row] != 0);
+    bool viol = false;
+#pragma unroll 1
+    for (int j = 0; j < 16; ++j) {
+        if (j < w) {
+            const int p = k0 + j;
+            const int buf = j & 1;
+            if (row == p) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) prow_sh[buf][c] = pr[c];
+            }
+            __syncthreads();
+            T prow[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) prow[c] = prow_sh[buf][c];
+            const T ap = fabs(prow[0]);
+            if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
+            if (!used && row != p && fabs(pr[0]) > ap) viol = true;
+            const T rinv = T(1) / prow[0];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) prow[c] *= rinv;
+            const bool isp = row == p;
+            const T cr = isp ? T(0) : pr[0];
+#pragma unroll
+            for (int c = 1; c < 16; ++c) pr[c] = isp ? prow[c] : pr[c] - cr * prow[c];
+            pr[0] = isp ? rinv : -cr * rinv;
+            used = used || isp;
+        }
+        const T first = pr[0];
+#pragma unroll
+        for (int c = 0; c < 15; ++c) pr[c] = pr[c + 1];
+        pr[15] = first;
+    }
+    if (tid < 16 && tid < w) { rowtag[k0 + tid] = k0 + tid + 1; pivrow[k0 + tid] = k0 + tid; }
+    if (live) {
+        const int tg = (row >= k0 && row < k0 + w) ? row - k0 : -1;   // this panel's pivot rows are k0..k0+w-1
+#pragma unroll
+        for (int c = 0; c < 16; ++c) G[row * gp + c] = pr[c] - ((tg == c) ? T(1) : T(0));
+    }
+    if (viol) *miss_sh = 1;
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*miss_sh) != 0;
+}
+
 // Blocked Gauss-Jordan inverse of the n x n block M (n <= 64*NR) by one
 // workgroup, working in the global slot S (L2-resident).  The rank-16 update
 // runs on the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
@@ -475,8 +537,8 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
 // diagonal (M untouched); otherwise M receives the inverse and *bad reports
 // a zero or non-finite pivot column.
 template <typename T, int NR, bool PIVOT, int NTH>
-__device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh, int32_t *rowtag,
-                          int32_t *pivrow, int32_t *flag_sh, bool *bad) {
+__device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh, T (*prow2_sh)[16],
+                          int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh, bool *bad) {
     constexpr int GP = 17;
     typedef typename Mfma<T>::acc_t acc_t;
     const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
@@ -497,14 +559,19 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
             G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
         }
         __syncthreads();
-        // 2. panel Gauss-Jordan (wave 0); flag: zero pivot (PIVOT) or another pivot needed (!PIVOT)
-        if (wv == 0) {
-            bool miss = false;
-            const bool b = panel_gj<T, NR, PIVOT>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
-            if (lane == 0 && (PIVOT ? b : miss)) *flag_sh = 1;
+        // 2. panel Gauss-Jordan; flag: zero pivot (PIVOT) or another pivot needed (!PIVOT).
+        //    The checked path spreads the panel over all waves, one row per lane.
+        if (!PIVOT && NP <= 64 * (NTH / 64)) {
+            if (panel_gjw<T, NTH / 64>(G, GP, NP, n, k0, w, rowtag, pivrow, prow2_sh, flag_sh)) return false;
+        } else {
+            if (wv == 0) {
+                bool miss = false;
+                const bool b = panel_gj<T, NR, PIVOT>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
+                if (lane == 0 && (PIVOT ? b : miss)) *flag_sh = 1;
+            }
+            __syncthreads();
+            if (!PIVOT && __builtin_amdgcn_readfirstlane(*flag_sh)) return false;
         }
-        __syncthreads();
-        if (!PIVOT && __builtin_amdgcn_readfirstlane(*flag_sh)) return false;
         // 3. pivot rows X(P, :) of the other columns
         for (int e = tid; e < w * n; e += NTH) {
             const int c = e / w, j = e - c * w;
@@ -587,7 +654,8 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int
     T *M = D + (size_t)blk * ld;
     T *S = Wk + (size_t)blockIdx.x * ldw;
     bool bad = false;
-    if (!ginv_body<T, NR, false, kInvThreads>(M, S, n, G, XP, prow_sh, rowtag, pivrow, &flag_sh, &bad)) {
+    __shared__ T prow2_sh[2][16];
+    if (!ginv_body<T, NR, false, kInvThreads>(M, S, n, G, XP, prow_sh, prow2_sh, rowtag, pivrow, &flag_sh, &bad)) {
         if (threadIdx.x == 0) redo[blk] = 1;    // left to k_fac_pivot; M untouched
     }
 }
@@ -620,7 +688,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
     __shared__ int32_t rowtag[NP];               // 1 + step at which the row was pivot (0: not yet)
     __shared__ int32_t pivrow[NP];
     __shared__ int32_t bad_sh, miss_sh;
-    __shared__ T prow_sh[16];
+    __shared__ T prow2_sh[2][16];
     typedef typename Mfma<T>::acc_t acc_t;
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
@@ -656,14 +724,8 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
             }
         }
         __syncthreads();
-        // 2. panel Gauss-Jordan (wave 0)
-        if (wv == 0) {
-            bool miss = false;
-            (void)panel_gj<T, NR, false>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
-            if (miss && lane == 0) miss_sh = 1;
-        }
-        __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(miss_sh)) break;
+        // 2. panel Gauss-Jordan (all four waves, checked diagonal pivots)
+        if (panel_gjw<T, 4>(G, GP, NP, n, k0, w, rowtag, pivrow, prow2_sh, &miss_sh)) break;
         // 3. pivot rows of the other columns
 #pragma unroll
         for (int s = 0; s < TPW; ++s) {
@@ -742,8 +804,8 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_pivot(T *__restrict__ D, in
     int32_t *pivrow = rowtag + NP;
     __shared__ int32_t flag_sh;
     bool bad = false;
-    ginv_body<T, NR, true, kInvThreads>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh, rowtag,
-                                        pivrow, &flag_sh, &bad);
+    ginv_body<T, NR, true, kInvThreads>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh,
+                                        nullptr, rowtag, pivrow, &flag_sh, &bad);
     if (threadIdx.x == 0) {
         redo[blk] = 0;
         if (bad) atomicMin(err, blk);
@@ -811,8 +873,7 @@ int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows
 // Block-inverse kernel: 0 register-resident Gauss-Jordan (n <= 192), 1 blocked
 // in registers with checked diagonal pivots (n <= 144), 2 blocked in global
 // memory (any n <= 320; larger blocks always take it).  CDG_INV overrides
-// the default, which is the fastest measured at 16^3: 1 for FP32 and for
-// n <= 48, else 0 (the FP64 register tiles of 1 spill at n = 135).
+// the default, the fastest measured at 16^3: 1 up to n = 144, 0 above.
 int inv_mode_env() {
     static const int mode = [] { const char *e = std::getenv("CDG_INV"); return e ? std::atoi(e) : -1; }();
     return mode;
@@ -820,7 +881,7 @@ int inv_mode_env() {
 template <typename T> int inv_mode(int n) {
     const int m = inv_mode_env();
     if (m >= 0) return m;
-    return (sizeof(T) == 4 && n <= 144) || n <= 48 ? 1 : 0;
+    return n <= 144 ? 1 : 0;
 }
 
 template <typename T>
