@@ -118,6 +118,7 @@ struct cdg_prec {
     unsigned long long *d_ticket = nullptr;
     int32_t *d_dferr = nullptr;        // raised by a sweep that gave up waiting
     int32_t df_epoch = 0;              // calls since the counters were zeroed
+    int32_t df_spin = 1 << 21;         // polls before a dependency wait gives up
     uint64_t stamp = 0;
     int32_t factored = 0;
     std::vector<DevMem> owned;

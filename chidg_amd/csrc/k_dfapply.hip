@@ -348,6 +348,10 @@ int df_configure(cdg_ctx *ctx, cdg_prec *M) {
     CDG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sh.T, M->df_lds));
     if (per_cu < 1) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "sweep kernel does not fit a CU");
     M->df_grid = per_cu * ctx->num_cus;
+    // polls before a wait gives up (each poll sleeps ~64 cycles); CDG_DF_SPIN
+    // lowers it, which is how the error path is exercised
+    M->df_spin = 1 << 21;
+    if (const char *e = std::getenv("CDG_DF_SPIN")) M->df_spin = std::max(1, std::atoi(e));
     return CDG_OK;
 }
 
@@ -389,7 +393,7 @@ int df_launch(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, doub
     a.plen = (int64_t)M->nrows * M->df_S * M->n;
     a.nrows = M->nrows; a.n = M->n; a.S = M->df_S; a.w = M->df_w; a.rounds = M->df_rounds; a.dc = M->df_dc;
     a.maxdep = M->maxdep;
-    a.spin_limit = 1 << 21;
+    a.spin_limit = M->df_spin;
     a.target = (M->df_epoch + 1) * M->df_S;
     hipLaunchKernelGGL((k_df_apply<S, Acc, DfRounds<S>::R, kDPF>), dim3(grid), dim3(M->df_T), M->df_lds, ctx->stream, a);
     CDG_HIP_TRY(hipGetLastError());

@@ -44,7 +44,16 @@ constexpr int KC = 16;       // K chunk staged in LDS
 constexpr int GW = 3;        // waves per workgroup
 constexpr int GT = 64 * GW;  // threads per workgroup
 
-constexpr int TMP = TM + 1;  // LDS row pitch: the padding spreads column-wise writes over the banks
+// LDS tiles [KC][Pitch<T>].  FP64: pitch TM + 1 spreads the column-wise B
+// writes over the banks (measured conflict-free).  FP32: pitch TM keeps the
+// operand reads (4 rows x 16 columns per wave) conflict-free, and the B tile's
+// column index is XOR-swizzled by its row pair, (kb >> 1) & 7, so the
+// column-wise writes (16 rows of one column) land in 16 distinct banks too.
+template <typename T> struct Pitch { static constexpr int P = TM + 1; };
+template <> struct Pitch<float> { static constexpr int P = TM; };
+template <typename T> __device__ __forceinline__ int bswz(int kb, int cc) {
+    return sizeof(T) == 4 ? (cc ^ ((kb >> 1) & 7)) : cc;
+}
 constexpr int LPT = KC * TM / GT;   // elements per thread per operand tile and chunk (4)
 
 // Loads chunk kc of the A strip (rows r0.., K = kc..kc+KC) and the B strip
@@ -66,7 +75,8 @@ __device__ __forceinline__ void tile_fetch(const SA *__restrict__ A, const SB *_
 }
 
 template <typename T>
-__device__ __forceinline__ void tile_store(T (*As)[TMP], T (*Bs)[TMP], const T (&ra)[LPT], const T (&rb)[LPT]) {
+__device__ __forceinline__ void tile_store(T (*As)[Pitch<T>::P], T (*Bs)[Pitch<T>::P], const T (&ra)[LPT],
+                                           const T (&rb)[LPT]) {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {
@@ -74,7 +84,7 @@ __device__ __forceinline__ void tile_store(T (*As)[TMP], T (*Bs)[TMP], const T (
         const int kk = idx / TM, rr = idx - kk * TM;
         As[kk][rr] = ra[i];
         const int cc = idx / KC, kb = idx - cc * KC;
-        Bs[kb][cc] = rb[i];
+        Bs[kb][bswz<T>(kb, cc)] = rb[i];
     }
 }
 
@@ -83,7 +93,8 @@ __device__ __forceinline__ void tile_store(T (*As)[TMP], T (*Bs)[TMP], const T (
 // the MFMAs, so one barrier per chunk and the global latency is hidden.
 template <typename T, typename SA, typename SB>
 __device__ __forceinline__ void tile_mma(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0,
-                                         int c0, T (*As)[KC][TMP], T (*Bs)[KC][TMP], typename Mfma<T>::acc_t *acc) {
+                                         int c0, T (*As)[KC][Pitch<T>::P], T (*Bs)[KC][Pitch<T>::P],
+                                         typename Mfma<T>::acc_t *acc) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     T ra[LPT], rb[LPT];
     tile_fetch<T, SA, SB>(A, B, n, r0, c0, 0, ra, rb);
@@ -99,7 +110,8 @@ __device__ __forceinline__ void tile_mma(const SA *__restrict__ A, const SB *__r
             const T a = As[cur][4 * s + (lane >> 4)][16 * wv + (lane & 15)];
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
-                const T b = Bs[cur][4 * s + (lane >> 4)][16 * t + (lane & 15)];
+                const int kb = 4 * s + (lane >> 4);
+                const T b = Bs[cur][kb][bswz<T>(kb, 16 * t + (lane & 15))];
                 acc[t] = Mfma<T>::run(a, b, acc[t]);
             }
         }
@@ -115,7 +127,7 @@ __global__ void __launch_bounds__(GT)
 k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ jobs, const int32_t *__restrict__ lo_blk, const int32_t *__restrict__ lo_col,
          int n, int tiles) {
-    __shared__ T As[2][KC][TMP], Bs[2][KC][TMP];
+    __shared__ T As[2][KC][Pitch<T>::P], Bs[2][KC][Pitch<T>::P];
     const int q = jobs[blockIdx.y];
     const int tm = blockIdx.x % tiles, tn = blockIdx.x / tiles;
     typename Mfma<T>::acc_t acc[3];
@@ -140,7 +152,7 @@ __global__ void __launch_bounds__(GT)
 k_fac_p2(const double *__restrict__ Ab, int64_t lda, const T *__restrict__ L, T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ rows, const int32_t *__restrict__ lo_ptr, const int32_t *__restrict__ lo_tr,
          const int32_t *__restrict__ diag, int n, int tiles) {
-    __shared__ T As[2][KC][TMP], Bs[2][KC][TMP];
+    __shared__ T As[2][KC][Pitch<T>::P], Bs[2][KC][Pitch<T>::P];
     const int i = rows[blockIdx.y];
     const int tm = blockIdx.x % tiles, tn = blockIdx.x / tiles;
     typename Mfma<T>::acc_t acc[3];

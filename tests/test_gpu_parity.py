@@ -300,3 +300,21 @@ def test_p3_factor_apply_fgmres(ctx, precision, tol):
     assert rel(x, x_ref) < 1e-7
     if precision == "fp64":
         assert abs(ls.niter - n_ref) <= 1
+
+
+def test_tiny_and_empty_systems(ctx):
+    """One block row (n = 1 and n = 40) and an empty matrix."""
+    for n in (1, 40):
+        p = pt.hex_box(1, 1, 1, n, periodic=(False, False, False))
+        blocks = po.fill_synthetic(p, 3, 1 / np.sqrt(n), 0.1 / np.sqrt(n))
+        A = upload(ctx, p, blocks)
+        b = po.fill_vector(p.vec_len, 2)
+        x = np.zeros(p.vec_len)
+        ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-12, inner_fgmres=False)
+        ls.solve(A, x, b, precon_ILU0())
+        assert ls.niter == 1        # ILU(0) of one block is its exact inverse
+        assert rel(x, np.linalg.solve(blocks.reshape(n, n).T, b)) < 1e-12
+    p = pt.from_rows(5, [])
+    A = chidg_matrix(p, ctx)
+    A.store(np.zeros(0))
+    assert chidg_mv(A, np.zeros(0)).shape == (0,)

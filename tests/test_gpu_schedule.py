@@ -104,3 +104,24 @@ def test_dataflow_full_size(ctx):
             assert ls.residual_norm / float(torch.linalg.norm(v)) < 1e-8
         Md.tear_down()
     A.release()
+
+
+def test_dataflow_gives_up_cleanly(ctx, monkeypatch):
+    """A wait that exceeds its spin limit (forced from the host: CDG_DF_SPIN=1)
+    ends the sweep normally: the call reports an error, the counters are
+    reset, and a fresh preconditioner on the same context works."""
+    p, blocks = synthetic_system(4, 4, 4, 1, seed=17)
+    A = upload(ctx, p, blocks)
+    v = po.fill_vector(p.vec_len, 3)
+    monkeypatch.setenv("CDG_DF_SPIN", "1")
+    M = precon_ILU0(schedule="dataflow").init(A)
+    M.update(A)
+    with pytest.raises(Exception):
+        for _ in range(20):          # a run may finish before any wait starts; many will not
+            M.apply(A, v)
+    M.tear_down()
+    monkeypatch.delenv("CDG_DF_SPIN")
+    M2 = precon_ILU0(schedule="dataflow").init(A)
+    M2.update(A)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    assert rel(M2.apply(A, v), po.ilu0_apply(p, blocks, L_ref, D_ref, v)) < 1e-12
