@@ -539,6 +539,119 @@ __device__ __forceinline__ bool panel_gjw(T *G, int gp, int NP, int n, int k0, i
     return __builtin_amdgcn_readfirstlane(*miss_sh) != 0;
 }
 
+// Lane gather (ds_bpermute: the value of lane addr / 4) and broadcast of quad
+// lane Q to its quad (DPP quad_perm [Q,Q,Q,Q]).
+__device__ __forceinline__ float bperm(float x, int addr) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double bperm(double x, int addr) {
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, int2{__builtin_amdgcn_ds_bpermute(addr, v.x), __builtin_amdgcn_ds_bpermute(addr, v.y)});
+}
+template <int Q, typename T> __device__ __forceinline__ T quad_bcast(T x) { return dpp<Q * 0x55>(x); }
+
+// The checked-diagonal panel in two stages, for one workgroup of 64 * NW
+// threads.
+//   Stage A (wave 0): the 16 x 16 pivot block (panel rows k0..k0+15), lane
+//   4i + q holding row i, columns 4q..4q+3.  Step j takes the pivot a_jj with
+//   v_readlane, the pivot row's entries with ds_bpermute and each row's
+//   multiplier a_ij from its quad with DPP; the step's scaled pivot row s
+//   (s_j = 1/a_jj) and the raw pivot go to the history H[j] (zeros for the
+//   steps j >= w the panel does not have, which leave the rows unchanged).
+//   Stage B (every lane, row lane * NW + wave outside the pivot block)
+//   replays the 16 steps from H with no barrier between them.
+// Both stages check at each step that partial pivoting would have chosen the
+// diagonal: no unused row (row > k0 + j, row < n) with a strictly larger
+// |a(r,j)|, and a nonzero finite pivot.  The arithmetic is that of panel_gjw
+// step for step.  On exit G holds C - E_P (E_P: ones at (k0 + c, c), c < w);
+// returns whether some step needed another pivot.
+constexpr int kHP = 20;   // history pitch (16 scaled pivot-row entries, raw pivot; 16-byte rows)
+template <typename T, int NW>
+__device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, int w, T (*H)[kHP], int32_t *miss_sh) {
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int i = tid >> 2, q = tid & 3;
+        T a[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[e] = G[(k0 + i) * gp + 4 * q + e];
+        bool viol = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int jq = j >> 2, je = j & 3;
+            if (j < w) {
+                const T piv = readlane(a[je], 4 * j + jq);
+                const T rinv = T(1) / piv;
+                const T ap = fabs(piv);
+                if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
+                T prow[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) prow[e] = bperm(a[e], (4 * j + q) * 4);
+                T cr;
+                switch (jq) {
+                case 0: cr = quad_bcast<0>(a[je]); break;
+                case 1: cr = quad_bcast<1>(a[je]); break;
+                case 2: cr = quad_bcast<2>(a[je]); break;
+                default: cr = quad_bcast<3>(a[je]); break;
+                }
+                if (i > j && k0 + i < n && fabs(cr) > ap) viol = true;
+                const bool isp = i == j, pc = q == jq;
+                T sv[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) sv[e] = (pc && e == je) ? rinv : prow[e] * rinv;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const T upd = isp ? sv[e] : fma(-cr, sv[e], a[e]);
+                    a[e] = (e == je && pc) ? (isp ? rinv : -cr * rinv) : upd;
+                }
+                if (i == 0) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) H[j][4 * q + e] = sv[e];
+                    if (q == 0) H[j][16] = piv;
+                }
+            } else if (i == 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) H[j][4 * q + e] = T(0);
+                if (q == 0) H[j][16] = T(0);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = 4 * q + e;
+            G[(k0 + i) * gp + c] = a[e] - ((i == c && c < w) ? T(1) : T(0));
+        }
+        if (viol) *miss_sh = 1;
+    }
+    __syncthreads();
+    const int wv = tid >> 6, lane = tid & 63;
+    const int row = lane * NW + wv;
+    const bool live = row < NP && (row < k0 || row >= k0 + 16);
+    const bool unused = row >= k0 + 16 && row < n;
+    T pr[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) pr[c] = live ? G[row * gp + c] : T(0);
+    bool viol = false;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        T sv[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) sv[c] = H[j][c];
+        const T ap = fabs(H[j][16]);
+        const T cr = pr[j];
+        viol = viol | ((j < w) & unused & (fabs(cr) > ap));
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            if (c != j) pr[c] = fma(-cr, sv[c], pr[c]);
+        pr[j] = -cr * sv[j];
+    }
+    if (live) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) G[row * gp + c] = pr[c];
+    }
+    if (viol) *miss_sh = 1;
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*miss_sh) != 0;
+}
+
 // Blocked Gauss-Jordan inverse of the n x n block M (n <= 64*NR) by one
 // workgroup, working in the global slot S (L2-resident).  The rank-16 update
 // runs on the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
@@ -673,107 +786,169 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int
 }
 
 // Dinv_i = inv(D_i) for blocks of n <= 16*NT, one workgroup of 4 waves per
-// block.  The block lives in MFMA accumulator tiles (16 x 16 tile t =
-// (ti, tj) held by wave t % 4).  Columns are processed in panels of 16:
-//   1. the panel's tiles are written to LDS and wave 0 runs Gauss-Jordan with
-//      partial pivoting on the n x 16 panel in registers (pivot: largest
-//      |a(r,k)| over unused rows, lower row on ties, found with DPP); rows are
-//      never swapped.  The panel then holds C = [W on the pivot rows P,
-//      -A(r,P) W elsewhere], W the inverse of the panel's pivot block;
-//   2. every other column is updated at once, X += (C - E_P) X(P,:), one
-//      rank-16 product on MFMA (E_P: ones at (p_j, j));
-//   3. the panel's tiles take C.
-// This is the unblocked in-place Gauss-Jordan (same pivots) reorganized; the
-// final scatter inv(rowstep(r), pivrow(m)) = S(r, m) undoes the implicit row
-// permutation.
+// block, pivoting on the diagonal.  The block lives in MFMA accumulator tiles.
+// Wave v holds the whole tile columns tj = v + 4c (c < NT / 4), so that those
+// tiles' operands sit at compile-time offsets from a per-lane base; the tiles
+// of the last NT % 4 columns are dealt round-robin (tile e to wave e % 4), so
+// that no wave holds more than ceil(NT^2 / 4) tiles.  Columns are processed in
+// panels of 16:
+//   1. the panel's tiles go to LDS (G), and so do the other tiles of the
+//      panel's pivot rows k0..k0+15 (XP);
+//   2. panel_ab runs the checked-diagonal Gauss-Jordan on the n x 16 panel,
+//      which then holds C = [W on the pivot rows P, -A(r,P) W elsewhere], W
+//      the inverse of the panel's pivot block;
+//   3. every other column is updated at once, X += (C - E_P) X(P,:), one
+//      rank-16 product on MFMA (E_P: ones at (k0 + j, j)), and the panel's
+//      tiles take C, formed on MFMA as (C - E_P + E_P) I from a zero
+//      accumulator (an accumulator written by VALU makes the compiler hold
+//      every tile in arch VGPRs and copy them all in each panel).
+// This is the unblocked in-place Gauss-Jordan reorganized, with the pivots
+// partial pivoting chooses whenever the check holds; a block where it fails
+// is left untouched and flagged in redo[] for k_fac_pivot.
 template <typename T, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err,
            int32_t *__restrict__ redo) {
     constexpr int NP = 16 * NT;                  // padded size
-    constexpr int TPW = (NT * NT + 3) / 4;       // tiles per wave
-    constexpr int NR = (NP + 63) / 64;           // panel rows per lane of wave 0
+    constexpr int NFC = NT / 4;                  // whole tile columns per wave
+    constexpr int NFA = NFC > 0 ? NFC : 1;
+    constexpr int NXT = (NT - 4 * NFC) * NT;     // tiles of the remaining columns
+    constexpr int NX = (NXT + 3) / 4;            // of which one wave holds at most NX
+    constexpr int NXA = NX > 0 ? NX : 1;
     constexpr int GP = 17;                       // LDS pitch of the panel (bank spread)
     constexpr int XPP = NP + 1;                  // LDS pitch of the pivot rows
     __shared__ T G[NP * GP];                     // panel in, C - E_P out
     __shared__ T XP[16 * XPP];                   // pivot rows of the other columns
-    __shared__ int32_t rowtag[NP];               // 1 + step at which the row was pivot (0: not yet)
-    __shared__ int32_t pivrow[NP];
-    __shared__ int32_t bad_sh, miss_sh;
-    __shared__ T prow2_sh[2][16];
+    __shared__ T H[16][kHP];                     // panel step history
+    __shared__ int32_t miss_sh;
     typedef typename Mfma<T>::acc_t acc_t;
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    acc_t acc[TPW];
-    // diagonal pivots, checked; M is not written before the end
+    const int tid = threadIdx.x, lane0 = tid & 63;
+    const int wv0 = __builtin_amdgcn_readfirstlane(tid >> 6);
+    acc_t acc[NFA][NT];     // tile (ti, wave + 4c)
+    acc_t accx[NXA];        // remaining tile e = wave + 4x: (e % NT, 4 NFC + e / NT)
+    // a 32-bit byte offset from the block's (uniform) base keeps one VGPR per address
+    auto ldM = [&](int row, int col, bool ok) -> T {
+        return (ok && row < n && col < n) ? *reinterpret_cast<const T *>(
+                   reinterpret_cast<const char *>(M) + (uint32_t)((col * n + row) * (int)sizeof(T))) : T(0);
+    };
 #pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        const int t = wv + 4 * s;
-        const int ti = t / NT, tj = t - ti * NT;
+    for (int c = 0; c < NFC; ++c)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + (lane & 15);
-            acc[s][r] = (t < NT * NT && row < n && col < n) ? M[(size_t)col * n + row] : T(0);
-        }
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                acc[c][ti][r] = ldM(16 * ti + Mfma<T>::row(lane0, r), 16 * (wv0 + 4 * c) + (lane0 & 15), true);
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+        const int e = wv0 + 4 * x, eti = e % NT, etj = 4 * NFC + e / NT;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accx[x][r] = ldM(16 * eti + Mfma<T>::row(lane0, r), 16 * etj + (lane0 & 15), e < NXT);
     }
+    for (int e = tid; e < 16 * XPP; e += 256) XP[e] = T(0);
+    if (tid == 0) miss_sh = 0;
+    // B operand of the identity: I[4kk + lane/16][lane%16]
+    T ident[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) ident[kk] = (lane0 & 15) == 4 * kk + (lane0 >> 4) ? T(1) : T(0);
     __syncthreads();
-    for (int i = tid; i < NP; i += 256) { rowtag[i] = 0; pivrow[i] = 0; }
-    for (int i = tid; i < 16 * XPP; i += 256) XP[i] = T(0);
-    if (tid == 0) { bad_sh = 0; miss_sh = 0; }
     for (int kb = 0; kb < NT; ++kb) {
         const int k0 = 16 * kb;
         const int w = min(16, n - k0);
-        // 1. panel tiles to LDS
+        // wave and lane indices made opaque per panel: the LDS addresses below
+        // are recomputed in each panel rather than hoisted out of the loop
+        // (kept live across it they exceed the register file)
+        int wv, lane;
+        asm volatile("s_mov_b32 %0, %1" : "=s"(wv) : "s"(wv0));
+        asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(lane0));
+        auto storeG = [&](const acc_t &v, int ti) {
 #pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int t = wv + 4 * s;
-            const int ti = t / NT, tj = t - ti * NT;
-            if (t < NT * NT && tj == kb) {
+            for (int r = 0; r < 4; ++r) G[(16 * ti + Mfma<T>::row(lane, r)) * GP + (lane & 15)] = v[r];
+        };
+        auto storeXP = [&](const acc_t &v, int tj) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) G[(16 * ti + Mfma<T>::row(lane, r)) * GP + (lane & 15)] = acc[s][r];
+            for (int r = 0; r < 4; ++r) XP[Mfma<T>::row(lane, r) * XPP + 16 * tj + (lane & 15)] = v[r];
+        };
+        // 1. the panel's tile column to G, the pivot rows' other tiles to XP
+#pragma unroll
+        for (int c = 0; c < NFC; ++c) {
+            const int tj = wv + 4 * c;
+            if (tj == kb) {
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti) storeG(acc[c][ti], ti);
+            } else {
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti)
+                    if (ti == kb) storeXP(acc[c][ti], tj);
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+            const int e = wv + 4 * x, eti = e % NT, etj = 4 * NFC + e / NT;
+            if (e < NXT) {
+                if (etj == kb) storeG(accx[x], eti);
+                else if (eti == kb) storeXP(accx[x], etj);
             }
         }
         __syncthreads();
-        // 2. panel Gauss-Jordan (all four waves, checked diagonal pivots)
-        if (panel_gjw<T, 4>(G, GP, NP, n, k0, w, rowtag, pivrow, prow2_sh, &miss_sh)) break;
-        // 3. pivot rows of the other columns
+        // the pivot rows (B operands of step 3) are final now
+        T b[NFA][4], bx[NXA][4];
 #pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int t = wv + 4 * s;
-            const int ti = t / NT, tj = t - ti * NT;
-            if (t < NT * NT && tj != kb) {
+        for (int c = 0; c < NFC; ++c)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * ti + Mfma<T>::row(lane, r);
-                    const int j = rowtag[row] - 1 - k0;
-                    if (j >= 0 && j < 16) XP[j * XPP + 16 * tj + (lane & 15)] = acc[s][r];
+            for (int kk = 0; kk < 4; ++kk) b[c][kk] = XP[(4 * kk + (lane >> 4)) * XPP + 16 * (wv + 4 * c) + (lane & 15)];
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+            const int e = min(wv + 4 * x, NXT - 1), etj = 4 * NFC + e / NT;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) bx[x][kk] = XP[(4 * kk + (lane >> 4)) * XPP + 16 * etj + (lane & 15)];
+        }
+        // 2. panel Gauss-Jordan (checked diagonal pivots)
+        if (panel_ab<T, 4>(G, GP, NP, n, k0, w, H, &miss_sh)) break;
+        // 3. X += (C - E_P) X(P,:) on MFMA; the panel's tiles take C
+        auto ldA = [&](int ti, int kk) -> T { return G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)]; };
+        auto panel_tile = [&](const T (&av)[4], int ti) -> acc_t {
+            acc_t r = {};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int cc = 4 * kk + (lane >> 4);
+                const T ae = av[kk] + ((ti == kb && (lane & 15) == cc && cc < w) ? T(1) : T(0));
+                r = Mfma<T>::run(ae, ident[kk], r);
+            }
+            return r;
+        };
+        if (NFC > 0) {
+            T a[NT][4];
+#pragma unroll
+            for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) a[ti][kk] = ldA(ti, kk);
+#pragma unroll
+            for (int c = 0; c < NFC; ++c) {
+                if (wv + 4 * c == kb) {
+#pragma unroll
+                    for (int ti = 0; ti < NT; ++ti) acc[c][ti] = panel_tile(a[ti], ti);
+                } else {
+#pragma unroll
+                    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) acc[c][ti] = Mfma<T>::run(a[ti][kk], b[c][kk], acc[c][ti]);
                 }
             }
         }
-        __syncthreads();
-        // 4. X += (C - E_P) X(P,:) on MFMA; the panel tiles take C
 #pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int t = wv + 4 * s;
-            const int ti = t / NT, tj = t - ti * NT;
-            if (t < NT * NT) {
-                if (tj != kb) {
+        for (int x = 0; x < NX; ++x) {
+            const int e = wv + 4 * x, eti = e % NT, etj = 4 * NFC + e / NT;
+            if (e < NXT) {
+                T ax[4];
 #pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                        const T a = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
-                        const T b = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
-                        acc[s] = Mfma<T>::run(a, b, acc[s]);
-                    }
+                for (int kk = 0; kk < 4; ++kk) ax[kk] = ldA(eti, kk);
+                if (etj == kb) {
+                    accx[x] = panel_tile(ax, eti);
                 } else {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * ti + Mfma<T>::row(lane, r);
-                        const int c = lane & 15;
-                        const int tg = rowtag[row] - 1 - k0;
-                        acc[s][r] = G[row * GP + c] + ((tg == c) ? T(1) : T(0));
-                    }
+                    for (int kk = 0; kk < 4; ++kk) accx[x] = Mfma<T>::run(ax[kk], bx[x][kk], accx[x]);
                 }
             }
         }
@@ -785,16 +960,27 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
         if (tid == 0) redo[blk] = 1;
         return;
     }
+    // the lane index again, opaque to the compiler: the store offsets are then
+    // recomputed here instead of being kept live (spilled) since the loads
+    int lane2;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lane2) : "v"(lane0));
+    auto stM = [&](int row, int col, T v) {
+        if (row < n && col < n)
+            *reinterpret_cast<T *>(reinterpret_cast<char *>(M) + (uint32_t)((col * n + row) * (int)sizeof(T))) = v;
+    };
 #pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        const int t = wv + 4 * s;
-        const int ti = t / NT, tj = t - ti * NT;
-        if (t < NT * NT) {
+    for (int c = 0; c < NFC; ++c)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * ti + Mfma<T>::row(lane, r), col = 16 * tj + (lane & 15);
-                if (row < n && col < n) M[(size_t)pivrow[col] * n + (rowtag[row] - 1)] = acc[s][r];
-            }
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                stM(16 * ti + Mfma<T>::row(lane2, r), 16 * (wv0 + 4 * c) + (lane2 & 15), acc[c][ti][r]);
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+        const int e = wv0 + 4 * x, eti = e % NT, etj = 4 * NFC + e / NT;
+        if (e < NXT) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) stM(16 * eti + Mfma<T>::row(lane2, r), 16 * etj + (lane2 & 15), accx[x][r]);
         }
     }
 }
