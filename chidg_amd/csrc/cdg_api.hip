@@ -534,6 +534,13 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     if (!e) e = upload_array(ctx, &M->d_lo_blk, A->lo_blk, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_lo_col, A->lo_col, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_lo_tr, A->lo_tr, &M->owned);
+    if (!e && precision != CDG_FP64) {
+        // A_ji (j < i) is an upper block of row j, so every lo_tr[q] has a U slot
+        std::vector<int32_t> slot_of(std::max(A->nnzb, 1), -1), lo_tru(A->lo_tr.size());
+        for (int u = 0; u < M->nup; ++u) slot_of[A->up_blk[u]] = u;
+        for (size_t q = 0; q < lo_tru.size(); ++q) lo_tru[q] = slot_of[A->lo_tr[q]];
+        e = upload_array(ctx, &M->d_lo_tru, lo_tru, &M->owned);
+    }
     if (!e) e = upload_array(ctx, &M->d_up_ptr, A->up_ptr, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_up_blk, A->up_blk, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_up_idx, up_idx, &M->owned);

@@ -88,6 +88,7 @@ struct cdg_prec {
     int32_t *d_redo = nullptr;         // per row: block inverse must be redone with partial pivoting
     // device index arrays
     int32_t *d_lo_ptr = nullptr, *d_lo_blk = nullptr, *d_lo_col = nullptr, *d_lo_tr = nullptr;
+    int32_t *d_lo_tru = nullptr;       // U slot of lo_tr[q] (FP32/BF16: the upper blocks' copies)
     int32_t *d_up_ptr = nullptr, *d_up_idx = nullptr, *d_up_col = nullptr, *d_up_blk = nullptr;
     int32_t *d_diag = nullptr;
     int32_t *d_err = nullptr;          // first singular row (factor)

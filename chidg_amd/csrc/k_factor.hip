@@ -44,133 +44,201 @@ constexpr int KC = 16;       // K chunk staged in LDS
 constexpr int GW = 3;        // waves per workgroup
 constexpr int GT = 64 * GW;  // threads per workgroup
 
-// LDS tiles [KC][Pitch<T>].  FP64: pitch TM + 1 spreads the column-wise B
-// writes over the banks (measured conflict-free).  FP32: pitch TM keeps the
-// operand reads (4 rows x 16 columns per wave) conflict-free, and the B tile's
-// column index is XOR-swizzled by its row pair, (kb >> 1) & 7, so the
-// column-wise writes (16 rows of one column) land in 16 distinct banks too.
+// LDS pitch of the A tiles [KC][Pitch<T>] (rows written and read 16 to 64
+// consecutive at a time): FP64 TM + 1, FP32 TM.
 template <typename T> struct Pitch { static constexpr int P = TM + 1; };
 template <> struct Pitch<float> { static constexpr int P = TM; };
-template <typename T> __device__ __forceinline__ int bswz(int kb, int cc) {
-    return sizeof(T) == 4 ? (cc ^ ((kb >> 1) & 7)) : cc;
-}
-constexpr int LPT = KC * TM / GT;   // elements per thread per operand tile and chunk (4)
 
-// Loads chunk kc of the A strip (rows r0.., K = kc..kc+KC) and the B strip
-// (K = kc.., cols c0..) into registers, zero outside n.
-template <typename T, typename SA, typename SB>
-__device__ __forceinline__ void tile_fetch(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0, int c0,
-                                           int kc, T (&ra)[LPT], T (&rb)[LPT]) {
+// Batched block GEMM.  A workgroup (3 waves) computes a 48 x 48 tile of one
+// output block, acc = sum_p A_p B_p over the concatenated K of its operand
+// pairs (A_p, B_p column-major n x n); wave w holds rows 16w..16w+15 (three
+// 16 x 16 MFMA tiles).  K advances in chunks of KC through two LDS buffers;
+// the global fetch of chunk c + 2 goes out before chunk c's products, so a
+// fetch has two chunks of products to arrive.
+constexpr int LA = KC * TM / GT;    // A elements per thread and chunk (4)
+constexpr int LB = KC * TM / GT;    // B elements per thread and chunk (4)
+constexpr int PB = TM + 1;          // B tile pitch (column-wise writes spread over the banks)
+template <typename T> using ATile = T[KC][Pitch<T>::P];
+template <typename T> using BTile = T[KC][PB];
+
+// a 32-bit element offset from a uniform base (one VGPR per address)
+template <typename S> __device__ __forceinline__ S ld_off(const S *base, int off) {
+    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + (uint32_t)(off * (int)sizeof(S)));
+}
+
+// The fetch loads raw source values (SA, SB) from in-bounds (clamped)
+// addresses with no conditions, so nothing waits for them here; conversion to
+// T and the zeros outside n happen when the chunk is stored to LDS.  (A
+// conversion at fetch time makes every fetch wait for its own loads.)
+template <typename SA, typename SB>
+__device__ __forceinline__ void gemm_fetch(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0, int c0,
+                                           int kc, SA (&ra)[LA], SB (&rb)[LB]) {
     const int tid = threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < LPT; ++i) {
+    for (int i = 0; i < LA; ++i) {
         const int idx = tid + GT * i;
         const int kk = idx / TM, rr = idx - kk * TM;
-        const int gr = r0 + rr, gk = kc + kk;
-        ra[i] = (gr < n && gk < n) ? static_cast<T>(A[(size_t)gk * n + gr]) : T(0);
+        ra[i] = ld_off(A, min(kc + kk, n - 1) * n + min(r0 + rr, n - 1));
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+        const int idx = tid + GT * i;
         const int cc = idx / KC, kb = idx - cc * KC;
-        const int gc = c0 + cc, gk2 = kc + kb;
-        rb[i] = (gc < n && gk2 < n) ? static_cast<T>(B[(size_t)gc * n + gk2]) : T(0);
+        rb[i] = ld_off(B, min(c0 + cc, n - 1) * n + min(kc + kb, n - 1));
+    }
+}
+
+template <typename T, typename SA, typename SB>
+__device__ __forceinline__ void gemm_store(ATile<T> &As, BTile<T> &Bs, const SA (&ra)[LA], const SB (&rb)[LB], int n,
+                                           int r0, int c0, int kc) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+        const int idx = tid + GT * i;
+        const int kk = idx / TM, rr = idx - kk * TM;
+        As[kk][rr] = (r0 + rr < n && kc + kk < n) ? static_cast<T>(ra[i]) : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+        const int idx = tid + GT * i;
+        const int cc = idx / KC, kb = idx - cc * KC;
+        Bs[kb][cc] = (c0 + cc < n && kc + kb < n) ? static_cast<T>(rb[i]) : T(0);
     }
 }
 
 template <typename T>
-__device__ __forceinline__ void tile_store(T (*As)[Pitch<T>::P], T (*Bs)[Pitch<T>::P], const T (&ra)[LPT],
-                                           const T (&rb)[LPT]) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ void gemm_chunk(const ATile<T> &As, const BTile<T> &Bs, typename Mfma<T>::acc_t (&acc)[3]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < LPT; ++i) {
-        const int idx = tid + GT * i;
-        const int kk = idx / TM, rr = idx - kk * TM;
-        As[kk][rr] = ra[i];
-        const int cc = idx / KC, kb = idx - cc * KC;
-        Bs[kb][bswz<T>(kb, cc)] = rb[i];
+    for (int s = 0; s < KC / 4; ++s) {
+        const int k = 4 * s + (lane >> 4);
+        const T a = As[k][16 * wv + (lane & 15)];
+        T b[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b[j] = Bs[k][16 * j + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[j] = Mfma<T>::run(a, b[j], acc[j]);
     }
 }
 
-// acc += A[r0.., :] * B[:, c0..] over the full K = n; A, B column-major n x n.
-// Two LDS buffers: chunk c+1 is fetched into registers while chunk c feeds
-// the MFMAs, so one barrier per chunk and the global latency is hidden.
-template <typename T, typename SA, typename SB>
-__device__ __forceinline__ void tile_mma(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0,
-                                         int c0, T (*As)[KC][Pitch<T>::P], T (*Bs)[KC][Pitch<T>::P],
-                                         typename Mfma<T>::acc_t *acc) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    T ra[LPT], rb[LPT];
-    tile_fetch<T, SA, SB>(A, B, n, r0, c0, 0, ra, rb);
-    __syncthreads();   // the previous tile_mma call may still read buffer 0
-    tile_store<T>(As[0], Bs[0], ra, rb);
+// acc = sum over the npairs operand pairs, pair(p, A, B) naming pair p.
+template <typename T, typename SA, typename SB, typename Pair>
+__device__ __forceinline__ void gemm_pairs(Pair pair, int npairs, int n, int r0, int c0, ATile<T> (&As)[2],
+                                           BTile<T> (&Bs)[2], typename Mfma<T>::acc_t (&acc)[3]) {
+    // Branch-free pipeline over an even number of chunks: fetches past the
+    // last chunk reload it (in bounds) and their stores write zeros, so the
+    // compiler's wait counts see loads in issue order (a conditional fetch
+    // makes it wait for every outstanding load at each store).
+    const int cpb = (n + KC - 1) / KC, nch = npairs * cpb, nch2 = (nch + 1) & ~1;
+    SA ra0[LA], ra1[LA];
+    SB rb0[LB], rb1[LB];
+    auto fetch = [&](int c, SA (&ra)[LA], SB (&rb)[LB]) {
+        c = min(c, nch - 1);
+        const int p = c / cpb;
+        const SA *A;
+        const SB *B;
+        pair(p, A, B);
+        gemm_fetch<SA, SB>(A, B, n, r0, c0, (c - p * cpb) * KC, ra, rb);
+    };
+    auto store = [&](int c, int buf, const SA (&ra)[LA], const SB (&rb)[LB]) {
+        // chunk c >= nch: kc = n stores zeros
+        const int kc = c < nch ? (c - (c / cpb) * cpb) * KC : n;
+        gemm_store<T, SA, SB>(As[buf], Bs[buf], ra, rb, n, r0, c0, kc);
+    };
+    fetch(0, ra0, rb0);
+    fetch(1, ra1, rb1);
+    store(0, 0, ra0, rb0);
     __syncthreads();
-    int cur = 0;
-    for (int kc = 0; kc < n; kc += KC) {
-        const bool more = kc + KC < n;
-        if (more) tile_fetch<T, SA, SB>(A, B, n, r0, c0, kc + KC, ra, rb);
-#pragma unroll
-        for (int s = 0; s < KC / 4; ++s) {
-            const T a = As[cur][4 * s + (lane >> 4)][16 * wv + (lane & 15)];
-#pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                const int kb = 4 * s + (lane >> 4);
-                const T b = Bs[cur][kb][bswz<T>(kb, 16 * t + (lane & 15))];
-                acc[t] = Mfma<T>::run(a, b, acc[t]);
-            }
-        }
-        if (more) tile_store<T>(As[cur ^ 1], Bs[cur ^ 1], ra, rb);
+    for (int c = 0; c < nch2; c += 2) {
+        fetch(c + 2, ra0, rb0);
+        gemm_chunk<T>(As[0], Bs[0], acc);
+        store(c + 1, 1, ra1, rb1);
         __syncthreads();
-        cur ^= 1;
+        fetch(c + 3, ra1, rb1);
+        gemm_chunk<T>(As[1], Bs[1], acc);
+        store(c + 2, 0, ra0, rb0);
+        __syncthreads();
     }
+}
+
+// XCD-aware job/tile numbering.  Workgroups go to the 8 XCDs (each with its
+// own L2) round-robin by linear id; id L runs tile (L / 8) % tiles2 of job
+// 8 * ((L / 8) / tiles2) + L % 8, so all tiles of a job share one XCD's L2
+// (they read the same operand strips).  Returns false for the ids past the
+// last job.
+__device__ __forceinline__ bool xcd_job(int njobs, int tiles2, int &job, int &tile) {
+    const int L = blockIdx.x, k = L >> 3;
+    job = 8 * (k / tiles2) + (L & 7);
+    tile = k - (k / tiles2) * tiles2;
+    return job < njobs;
+}
+inline int xcd_grid(int njobs, int tiles2) { return 8 * tiles2 * ((njobs + 7) / 8); }
+
+template <typename T, typename F>
+__device__ __forceinline__ void gemm_out(int tm, int tn, int n, typename Mfma<T>::acc_t (&acc)[3], F put) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = tm * TM + 16 * wv + Mfma<T>::row(lane, r);
+            const int col = tn * TM + 16 * j + (lane & 15);
+            if (row < n && col < n) put(row, col, acc[j][r]);
+        }
 }
 
 // phase 1: L_q = A_{lo_blk[q]} * Dinv_{lo_col[q]}
 template <typename T>
 __global__ void __launch_bounds__(GT)
 k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
-         const int32_t *__restrict__ jobs, const int32_t *__restrict__ lo_blk, const int32_t *__restrict__ lo_col,
-         int n, int tiles) {
-    __shared__ T As[2][KC][Pitch<T>::P], Bs[2][KC][Pitch<T>::P];
-    const int q = jobs[blockIdx.y];
-    const int tm = blockIdx.x % tiles, tn = blockIdx.x / tiles;
+         const int32_t *__restrict__ jobs, int njobs, const int32_t *__restrict__ lo_blk,
+         const int32_t *__restrict__ lo_col, int n, int tiles) {
+    __shared__ ATile<T> As[2];
+    __shared__ BTile<T> Bs[2];
+    int job, tile;
+    if (!xcd_job(njobs, tiles * tiles, job, tile)) return;
+    const int q = jobs[job];
+    const int tm = tile % tiles, tn = tile / tiles;
     typename Mfma<T>::acc_t acc[3];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) acc[t] = typename Mfma<T>::acc_t{0, 0, 0, 0};
-    tile_mma<T, double, T>(Ab + (size_t)lo_blk[q] * lda, D + (size_t)lo_col[q] * ld, n, tm * TM, tn * TM, As, Bs, acc);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    const double *A = Ab + (size_t)lo_blk[q] * lda;
+    const T *B = D + (size_t)lo_col[q] * ld;
+    gemm_pairs<T, double, T>([&](int, const double *&a, const T *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As,
+                             Bs, acc);
     T *out = L + (size_t)q * ld;
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = tm * TM + 16 * wv + Mfma<T>::row(lane, r);
-            const int col = tn * TM + 16 * t + (lane & 15);
-            if (row < n && col < n) out[(size_t)col * n + row] = acc[t][r];
-        }
+    gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) { out[(size_t)col * n + row] = v; });
 }
 
-// phase 2: D_i = A_{diag[i]} - sum_q L_q * A_{lo_tr[q]}
-template <typename T>
+// phase 2: D_i = A_{diag[i]} - sum_q L_q * U_q, U_q = A_{lo_tr[q]} read from
+// Ub (stride ldu): A's own blocks (FP64), or their FP32 copies.
+template <typename T, typename SB>
 __global__ void __launch_bounds__(GT)
-k_fac_p2(const double *__restrict__ Ab, int64_t lda, const T *__restrict__ L, T *__restrict__ D, int64_t ld,
-         const int32_t *__restrict__ rows, const int32_t *__restrict__ lo_ptr, const int32_t *__restrict__ lo_tr,
-         const int32_t *__restrict__ diag, int n, int tiles) {
-    __shared__ T As[2][KC][Pitch<T>::P], Bs[2][KC][Pitch<T>::P];
-    const int i = rows[blockIdx.y];
-    const int tm = blockIdx.x % tiles, tn = blockIdx.x / tiles;
+k_fac_p2(const double *__restrict__ Ab, int64_t lda, const SB *__restrict__ Ub, int64_t ldu, const T *__restrict__ L,
+         T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int njobs, const int32_t *__restrict__ lo_ptr,
+         const int32_t *__restrict__ lo_tr, const int32_t *__restrict__ diag, int n, int tiles) {
+    __shared__ ATile<T> As[2];
+    __shared__ BTile<T> Bs[2];
+    int job, tile;
+    if (!xcd_job(njobs, tiles * tiles, job, tile)) return;
+    const int i = rows[job];
+    const int tm = tile % tiles, tn = tile / tiles;
     typename Mfma<T>::acc_t acc[3];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) acc[t] = typename Mfma<T>::acc_t{0, 0, 0, 0};
-    for (int q = lo_ptr[i]; q < lo_ptr[i + 1]; ++q)
-        tile_mma<T, T, double>(L + (size_t)q * ld, Ab + (size_t)lo_tr[q] * lda, n, tm * TM, tn * TM, As, Bs, acc);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    const int q0 = lo_ptr[i], nq = lo_ptr[i + 1] - q0;
+    if (nq > 0)
+        gemm_pairs<T, T, SB>(
+            [&](int p, const T *&a, const SB *&b) {
+                a = L + (size_t)(q0 + p) * ld;
+                b = Ub + (size_t)lo_tr[q0 + p] * ldu;
+            },
+            nq, n, tm * TM, tn * TM, As, Bs, acc);
     const double *a = Ab + (size_t)diag[i] * lda;
     T *out = D + (size_t)i * ld;
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = tm * TM + 16 * wv + Mfma<T>::row(lane, r);
-            const int col = tn * TM + 16 * t + (lane & 15);
-            if (row < n && col < n) out[(size_t)col * n + row] = static_cast<T>(a[(size_t)col * n + row]) - acc[t][r];
-        }
+    gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) {
+        out[(size_t)col * n + row] = static_cast<T>(a[(size_t)col * n + row]) - v;
+    });
 }
 
 // DPP moves inside a 16-lane row: xor 1, xor 2 (quad permutes), rotate 4, 8.
@@ -1117,20 +1185,27 @@ int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows,
 }
 
 template <typename T>
-int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D) {
+int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D, const float *Ub32) {
     const int n = M->n;
     const int tiles = (n + TM - 1) / TM;
     for (int l = 0; l < M->nlev; ++l) {
         const int q0 = M->lev_lo_ptr[l], q1 = M->lev_lo_ptr[l + 1];
         const int r0 = M->lev_row_ptr[l], r1 = M->lev_row_ptr[l + 1];
         if (q1 > q0) {
-            hipLaunchKernelGGL((k_fac_p1<T>), dim3(tiles * tiles, q1 - q0), dim3(GT), 0, ctx->stream, A->blocks, A->ld,
-                               L, D, M->ld, M->d_lev_lo + q0, M->d_lo_blk, M->d_lo_col, n, tiles);
+            hipLaunchKernelGGL((k_fac_p1<T>), dim3(xcd_grid(q1 - q0, tiles * tiles)), dim3(GT), 0, ctx->stream,
+                               A->blocks, A->ld, L, D, M->ld, M->d_lev_lo + q0, q1 - q0, M->d_lo_blk, M->d_lo_col, n,
+                               tiles);
             CDG_HIP_TRY(hipGetLastError());
         }
         if (r1 > r0) {
-            hipLaunchKernelGGL((k_fac_p2<T>), dim3(tiles * tiles, r1 - r0), dim3(GT), 0, ctx->stream, A->blocks, A->ld,
-                               L, D, M->ld, M->d_lev_rows + r0, M->d_lo_ptr, M->d_lo_tr, M->d_diag, n, tiles);
+            if (Ub32)   // FP32 factor: the upper blocks' FP32 copies (the same values the FP64 reads convert to)
+                hipLaunchKernelGGL((k_fac_p2<T, float>), dim3(xcd_grid(r1 - r0, tiles * tiles)), dim3(GT), 0,
+                                   ctx->stream, A->blocks, A->ld, Ub32, M->ld, L, D, M->ld, M->d_lev_rows + r0, r1 - r0,
+                                   M->d_lo_ptr, M->d_lo_tru, M->d_diag, n, tiles);
+            else
+                hipLaunchKernelGGL((k_fac_p2<T, double>), dim3(xcd_grid(r1 - r0, tiles * tiles)), dim3(GT), 0,
+                                   ctx->stream, A->blocks, A->ld, A->blocks, A->ld, L, D, M->ld, M->d_lev_rows + r0,
+                                   r1 - r0, M->d_lo_ptr, M->d_lo_tr, M->d_diag, n, tiles);
             CDG_HIP_TRY(hipGetLastError());
             int e = launch_inv<T>(ctx, M, D, M->ld, M->d_lev_rows + r0, r1 - r0, n, M->d_err);
             if (e) return e;
@@ -1160,16 +1235,9 @@ int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_
     }
     const int32_t big = 0x7fffffff;
     CDG_HIP_TRY(hipMemcpyAsync(M->d_err, &big, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    int e = CDG_OK;
-    if (M->precision == CDG_FP64) {
-        e = factor_levels<double>(ctx, M, A, static_cast<double *>(M->L), static_cast<double *>(M->Dinv));
-    } else if (M->precision == CDG_FP32) {
-        e = factor_levels<float>(ctx, M, A, static_cast<float *>(M->L), static_cast<float *>(M->Dinv));
-    } else {
-        e = factor_levels<float>(ctx, M, A, M->L32, M->D32);
-    }
-    if (e) return e;
     const int nn = n * n;
+    // the FP32/BF16 copies of A's upper blocks (read by the backward sweep; the
+    // FP32 factor's phase 2 reads its U_q from them too)
     if (M->precision != CDG_FP64 && M->nup > 0) {
         dim3 grid((nn + 255) / 256 < 64 ? (nn + 255) / 256 : 64, std::min(M->nup, 65535));
         if (M->precision == CDG_FP32)
@@ -1180,6 +1248,16 @@ int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_
                                static_cast<cdg_bf16 *>(M->U), M->ld, M->d_up_blk, M->nup, nn);
         CDG_HIP_TRY(hipGetLastError());
     }
+    int e = CDG_OK;
+    if (M->precision == CDG_FP64) {
+        e = factor_levels<double>(ctx, M, A, static_cast<double *>(M->L), static_cast<double *>(M->Dinv), nullptr);
+    } else if (M->precision == CDG_FP32) {
+        e = factor_levels<float>(ctx, M, A, static_cast<float *>(M->L), static_cast<float *>(M->Dinv),
+                                 static_cast<const float *>(M->U));
+    } else {
+        e = factor_levels<float>(ctx, M, A, M->L32, M->D32, nullptr);
+    }
+    if (e) return e;
     if (M->precision == CDG_BF16) {
         const int64_t cl = (int64_t)M->nlo * M->ld, cd = (int64_t)M->nrows * M->ld;
         if (cl) hipLaunchKernelGGL(k_f32_to_bf16, dim3(2048), dim3(256), 0, ctx->stream, M->L32,
