@@ -618,8 +618,120 @@ __device__ __forceinline__ double bperm(double x, int addr) {
 }
 template <int Q, typename T> __device__ __forceinline__ T quad_bcast(T x) { return dpp<Q * 0x55>(x); }
 
-// The checked-diagonal panel in two stages, for one workgroup of 64 * NW
-// threads.
+// The checked-diagonal panel in two concurrent stages, for one workgroup of
+// 64 * NW threads (NW >= 2).
+//   Stage A (wave 0): the 16 x 16 pivot block (panel rows k0..k0+15), lane
+//   4i + q holding row i, columns 4q..4q+3.  Step j takes the pivot a_jj with
+//   v_readlane, the pivot row's entries with ds_bpermute and each row's
+//   multiplier a_ij from its quad with DPP; the step's scaled pivot row s
+//   (s_j = 1/a_jj) and the raw pivot go to the history H[j] (zeros for the
+//   steps j >= w the panel does not have, which leave the rows unchanged), and
+//   the step counter *step_sh is raised to base + j + 1 (release).
+//   Stage B (waves 1..NW-1, one row per lane, row = lane * (NW-1) + wave - 1,
+//   outside the pivot block) replays step j as soon as the counter shows it
+//   (acquire), so the two stages overlap.  Wave 0 never waits, so every wait
+//   ends.
+// Both stages check at each step that partial pivoting would have chosen the
+// diagonal: no unused row (row > k0 + j, row < n) with a strictly larger
+// |a(r,j)|, and a nonzero finite pivot.  The arithmetic is that of panel_gjw
+// step for step.  On exit G holds C - E_P (E_P: ones at (k0 + c, c), c < w);
+// returns whether some step needed another pivot.  base = 16 * (panel index):
+// the counter only grows during the kernel.
+constexpr int kHP = 20;   // history pitch (16 scaled pivot-row entries, raw pivot; 16-byte rows)
+template <typename T, int NW>
+__device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, int w, T (*H)[kHP], int32_t *miss_sh,
+                                         int32_t *step_sh) {
+    const int tid = threadIdx.x;
+    const int base = k0;   // 16 steps per panel
+    if (tid < 64) {
+        const int i = tid >> 2, q = tid & 3;
+        T a[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[e] = G[(k0 + i) * gp + 4 * q + e];
+        bool viol = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int jq = j >> 2, je = j & 3;
+            if (j < w) {
+                const T piv = readlane(a[je], 4 * j + jq);
+                const T rinv = T(1) / piv;
+                const T ap = fabs(piv);
+                if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
+                T prow[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) prow[e] = bperm(a[e], (4 * j + q) * 4);
+                T cr;
+                switch (jq) {
+                case 0: cr = quad_bcast<0>(a[je]); break;
+                case 1: cr = quad_bcast<1>(a[je]); break;
+                case 2: cr = quad_bcast<2>(a[je]); break;
+                default: cr = quad_bcast<3>(a[je]); break;
+                }
+                if (i > j && k0 + i < n && fabs(cr) > ap) viol = true;
+                const bool isp = i == j, pc = q == jq;
+                T sv[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) sv[e] = (pc && e == je) ? rinv : prow[e] * rinv;
+                if (i == 0) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) H[j][4 * q + e] = sv[e];
+                    if (q == 0) H[j][16] = piv;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const T upd = isp ? sv[e] : fma(-cr, sv[e], a[e]);
+                    a[e] = (e == je && pc) ? (isp ? rinv : -cr * rinv) : upd;
+                }
+            } else if (i == 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) H[j][4 * q + e] = T(0);
+                if (q == 0) H[j][16] = T(0);
+            }
+            if (tid == 0) __hip_atomic_store(step_sh, base + j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = 4 * q + e;
+            G[(k0 + i) * gp + c] = a[e] - ((i == c && c < w) ? T(1) : T(0));
+        }
+        if (viol) *miss_sh = 1;
+    } else {
+        const int wv = tid >> 6, lane = tid & 63;
+        const int row = lane * (NW - 1) + wv - 1;
+        const bool live = row < NP && (row < k0 || row >= k0 + 16);
+        const bool unused = row >= k0 + 16 && row < n;
+        T pr[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) pr[c] = live ? G[row * gp + c] : T(0);
+        bool viol = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            while (__hip_atomic_load(step_sh, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <= base + j)
+                __builtin_amdgcn_s_sleep(1);
+            T sv[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) sv[c] = H[j][c];
+            const T ap = fabs(H[j][16]);
+            const T cr = pr[j];
+            viol = viol | ((j < w) & unused & (fabs(cr) > ap));
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c != j) pr[c] = fma(-cr, sv[c], pr[c]);
+            pr[j] = -cr * sv[j];
+        }
+        if (live) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) G[row * gp + c] = pr[c];
+        }
+        if (viol) *miss_sh = 1;
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*miss_sh) != 0;
+}
+
+// The checked-diagonal panel in two stages run one after the other, for one
+// workgroup of 64 * NW threads (the FP64 form: panel_ab's concurrent stages
+// need more registers than FP64 leaves).
 //   Stage A (wave 0): the 16 x 16 pivot block (panel rows k0..k0+15), lane
 //   4i + q holding row i, columns 4q..4q+3.  Step j takes the pivot a_jj with
 //   v_readlane, the pivot row's entries with ds_bpermute and each row's
@@ -633,9 +745,8 @@ template <int Q, typename T> __device__ __forceinline__ T quad_bcast(T x) { retu
 // |a(r,j)|, and a nonzero finite pivot.  The arithmetic is that of panel_gjw
 // step for step.  On exit G holds C - E_P (E_P: ones at (k0 + c, c), c < w);
 // returns whether some step needed another pivot.
-constexpr int kHP = 20;   // history pitch (16 scaled pivot-row entries, raw pivot; 16-byte rows)
 template <typename T, int NW>
-__device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, int w, T (*H)[kHP], int32_t *miss_sh) {
+__device__ __forceinline__ bool panel_ab_seq(T *G, int gp, int NP, int n, int k0, int w, T (*H)[kHP], int32_t *miss_sh) {
     const int tid = threadIdx.x;
     if (tid < 64) {
         const int i = tid >> 2, q = tid & 3;
@@ -862,7 +973,8 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int
 // panels of 16:
 //   1. the panel's tiles go to LDS (G), and so do the other tiles of the
 //      panel's pivot rows k0..k0+15 (XP);
-//   2. panel_ab runs the checked-diagonal Gauss-Jordan on the n x 16 panel,
+//   2. panel_ab (FP32) or panel_ab_seq (FP64) runs the checked-diagonal
+//      Gauss-Jordan on the n x 16 panel,
 //      which then holds C = [W on the pivot rows P, -A(r,P) W elsewhere], W
 //      the inverse of the panel's pivot block;
 //   3. every other column is updated at once, X += (C - E_P) X(P,:), one
@@ -888,7 +1000,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
     __shared__ T G[NP * GP];                     // panel in, C - E_P out
     __shared__ T XP[16 * XPP];                   // pivot rows of the other columns
     __shared__ T H[16][kHP];                     // panel step history
-    __shared__ int32_t miss_sh;
+    __shared__ int32_t miss_sh, step_sh;
     typedef typename Mfma<T>::acc_t acc_t;
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
@@ -915,7 +1027,7 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
         for (int r = 0; r < 4; ++r) accx[x][r] = ldM(16 * eti + Mfma<T>::row(lane0, r), 16 * etj + (lane0 & 15), e < NXT);
     }
     for (int e = tid; e < 16 * XPP; e += 256) XP[e] = T(0);
-    if (tid == 0) miss_sh = 0;
+    if (tid == 0) { miss_sh = 0; step_sh = 0; }
     // B operand of the identity: I[4kk + lane/16][lane%16]
     T ident[4];
 #pragma unroll
@@ -973,7 +1085,9 @@ k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int 
             for (int kk = 0; kk < 4; ++kk) bx[x][kk] = XP[(4 * kk + (lane >> 4)) * XPP + 16 * etj + (lane & 15)];
         }
         // 2. panel Gauss-Jordan (checked diagonal pivots)
-        if (panel_ab<T, 4>(G, GP, NP, n, k0, w, H, &miss_sh)) break;
+        const bool miss = sizeof(T) == 4 ? panel_ab<T, 4>(G, GP, NP, n, k0, w, H, &miss_sh, &step_sh)
+                                         : panel_ab_seq<T, 4>(G, GP, NP, n, k0, w, H, &miss_sh);
+        if (miss) break;
         // 3. X += (C - E_P) X(P,:) on MFMA; the panel's tiles take C
         auto ldA = [&](int ti, int kk) -> T { return G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)]; };
         auto panel_tile = [&](const T (&av)[4], int ti) -> acc_t {
