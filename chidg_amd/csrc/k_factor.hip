@@ -618,6 +618,15 @@ __device__ __forceinline__ double bperm(double x, int addr) {
 }
 template <int Q, typename T> __device__ __forceinline__ T quad_bcast(T x) { return dpp<Q * 0x55>(x); }
 
+// 1/x for the FP32 pivot chain: v_rcp_f32 and one Newton step (within an ulp
+// of the quotient; 3 dependent operations instead of the 11 of the IEEE
+// division, which sits on the chain of every pivot step).  FP64 divides.
+__device__ __forceinline__ float recip(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // The checked-diagonal panel in two concurrent stages, for one workgroup of
 // 64 * NW threads (NW >= 2).
 //   Stage A (wave 0): the 16 x 16 pivot block (panel rows k0..k0+15), lane
@@ -654,7 +663,7 @@ __device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, in
             const int jq = j >> 2, je = j & 3;
             if (j < w) {
                 const T piv = readlane(a[je], 4 * j + jq);
-                const T rinv = T(1) / piv;
+                const T rinv = recip(piv);
                 const T ap = fabs(piv);
                 if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
                 T prow[4];
