@@ -44,11 +44,6 @@ constexpr int KC = 16;       // K chunk staged in LDS
 constexpr int GW = 3;        // waves per workgroup
 constexpr int GT = 64 * GW;  // threads per workgroup
 
-// LDS pitch of the A tiles [KC][Pitch<T>] (rows written and read 16 to 64
-// consecutive at a time): FP64 TM + 1, FP32 TM.
-template <typename T> struct Pitch { static constexpr int P = TM + 1; };
-template <> struct Pitch<float> { static constexpr int P = TM; };
-
 // Batched block GEMM.  A workgroup (3 waves) computes a 48 x 48 tile of one
 // output block, acc = sum_p A_p B_p over the concatenated K of its operand
 // pairs (A_p, B_p column-major n x n); wave w holds rows 16w..16w+15 (three
@@ -57,9 +52,14 @@ template <> struct Pitch<float> { static constexpr int P = TM; };
 // fetch has two chunks of products to arrive.
 constexpr int LA = KC * TM / GT;    // A elements per thread and chunk (4)
 constexpr int LB = KC * TM / GT;    // B elements per thread and chunk (4)
-constexpr int PB = TM + 1;          // B tile pitch (column-wise writes spread over the banks)
-template <typename T> using ATile = T[KC][Pitch<T>::P];
-template <typename T> using BTile = T[KC][PB];
+// LDS tiles are k-contiguous: As[r][k] = A(r0 + r, kc + k), Bs[c][k] =
+// B(kc + k, c0 + c), pitch KP (16-byte rows; 20 floats spread the transposing
+// A stores over the banks).  An MFMA sums over k, so lane l takes the
+// contiguous k range (KC/4)(l/16) .. +KC/4 for both operands: one 16-byte LDS
+// read per operand and step pair instead of one read per product.
+template <typename T> struct KPitch { static constexpr int P = KC + 16 / (int)sizeof(T); };
+template <typename T> using ATile = T[TM][KPitch<T>::P];
+template <typename T> using BTile = T[TM][KPitch<T>::P];
 
 // a 32-bit element offset from a uniform base (one VGPR per address)
 template <typename S> __device__ __forceinline__ S ld_off(const S *base, int off) {
@@ -96,29 +96,30 @@ __device__ __forceinline__ void gemm_store(ATile<T> &As, BTile<T> &Bs, const SA 
     for (int i = 0; i < LA; ++i) {
         const int idx = tid + GT * i;
         const int kk = idx / TM, rr = idx - kk * TM;
-        As[kk][rr] = (r0 + rr < n && kc + kk < n) ? static_cast<T>(ra[i]) : T(0);
+        As[rr][kk] = (r0 + rr < n && kc + kk < n) ? static_cast<T>(ra[i]) : T(0);
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
         const int idx = tid + GT * i;
         const int cc = idx / KC, kb = idx - cc * KC;
-        Bs[kb][cc] = (c0 + cc < n && kc + kb < n) ? static_cast<T>(rb[i]) : T(0);
+        Bs[cc][kb] = (c0 + cc < n && kc + kb < n) ? static_cast<T>(rb[i]) : T(0);
     }
 }
 
 template <typename T>
 __device__ __forceinline__ void gemm_chunk(const ATile<T> &As, const BTile<T> &Bs, typename Mfma<T>::acc_t (&acc)[3]) {
+    constexpr int KS = KC / 4;                   // k values per lane and chunk
+    typedef T vec_t __attribute__((ext_vector_type(KS)));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k0 = KS * (lane >> 4);
+    const vec_t a = *reinterpret_cast<const vec_t *>(&As[16 * wv + (lane & 15)][k0]);
+    vec_t b[3];
 #pragma unroll
-    for (int s = 0; s < KC / 4; ++s) {
-        const int k = 4 * s + (lane >> 4);
-        const T a = As[k][16 * wv + (lane & 15)];
-        T b[3];
+    for (int j = 0; j < 3; ++j) b[j] = *reinterpret_cast<const vec_t *>(&Bs[16 * j + (lane & 15)][k0]);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) b[j] = Bs[k][16 * j + (lane & 15)];
+    for (int s = 0; s < KS; ++s)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) acc[j] = Mfma<T>::run(a, b[j], acc[j]);
-    }
+        for (int j = 0; j < 3; ++j) acc[j] = Mfma<T>::run(a[s], b[j][s], acc[j]);
 }
 
 // acc = sum over the npairs operand pairs, pair(p, A, B) naming pair p.
@@ -193,8 +194,8 @@ __global__ void __launch_bounds__(GT)
 k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ jobs, int njobs, const int32_t *__restrict__ lo_blk,
          const int32_t *__restrict__ lo_col, int n, int tiles) {
-    __shared__ ATile<T> As[2];
-    __shared__ BTile<T> Bs[2];
+    __shared__ __attribute__((aligned(16))) ATile<T> As[2];
+    __shared__ __attribute__((aligned(16))) BTile<T> Bs[2];
     int job, tile;
     if (!xcd_job(njobs, tiles * tiles, job, tile)) return;
     const int q = jobs[job];
@@ -217,8 +218,8 @@ __global__ void __launch_bounds__(GT)
 k_fac_p2(const double *__restrict__ Ab, int64_t lda, const SB *__restrict__ Ub, int64_t ldu, const T *__restrict__ L,
          T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int njobs, const int32_t *__restrict__ lo_ptr,
          const int32_t *__restrict__ lo_tr, const int32_t *__restrict__ diag, int n, int tiles) {
-    __shared__ ATile<T> As[2];
-    __shared__ BTile<T> Bs[2];
+    __shared__ __attribute__((aligned(16))) ATile<T> As[2];
+    __shared__ __attribute__((aligned(16))) BTile<T> Bs[2];
     int job, tile;
     if (!xcd_job(njobs, tiles * tiles, job, tile)) return;
     const int i = rows[job];
