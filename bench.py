@@ -113,6 +113,24 @@ def gpu_run(args):
         y = torch.empty_like(b)
         t_apply = time_async(ctx, lambda: M.apply_async(A, v, z), args.kreps)
         t_mv = time_async(ctx, lambda: chidg_mv_async(A, v, y), args.kreps)
+        folded = {}
+        if (args.schedule or "levels") == "levels":
+            # the same factor under the folded schedule (W = U Dinv formed at the
+            # first apply): fold cost and apply time, for factors serving many applies
+            M.set_schedule("folded")
+            s = torch.cuda.ExternalStream(ctx.stream_ptr())
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(s)
+            M.apply_async(A, v, z)
+            e1.record(s)
+            e1.synchronize()
+            t_first = e0.elapsed_time(e1) / 1e3
+            t_af = time_async(ctx, lambda: M.apply_async(A, v, z), args.kreps)
+            folded = dict(apply_folded_ms=1e3 * t_af,
+                          apply_folded_gbs=ilu_apply_bytes(p, 4 if prec == "fp32" else 8) / t_af / 1e9,
+                          fold_ms=1e3 * (t_first - t_af))
+            M.set_schedule("levels")
         true_res = float(torch.linalg.norm(b - _mv(A, x)) / bnorm)
         out[prec] = dict(solves_per_s=args.steps / wall, ms_per_solve=1e3 * wall / args.steps,
                          p50_ms=1e3 * statistics.median(lat), p99_ms=1e3 * float(np.percentile(lat, 99)),
@@ -120,11 +138,13 @@ def gpu_run(args):
                          t_factor_ms=1e3 * st.t_factor, t_precon_ms=1e3 * st.t_precon, t_mv_ms=1e3 * st.t_mv,
                          t_dot_ms=1e3 * st.t_dot, t_norm_ms=1e3 * st.t_norm,
                          apply_ms=1e3 * t_apply, apply_gbs=ilu_apply_bytes(p, 4 if prec == "fp32" else 8) / t_apply / 1e9,
-                         mv_ms=1e3 * t_mv, mv_gbs=mv_bytes(p) / t_mv / 1e9, wall_s=wall, x=x)
+                         mv_ms=1e3 * t_mv, mv_gbs=mv_bytes(p) / t_mv / 1e9, wall_s=wall, x=x, **folded)
         log(f"{prec}: {out[prec]['solves_per_s']:.2f} solves/s, niter {st.niter}, "
             f"res {out[prec]['residual_rel']:.2e}, factor {1e3 * st.t_factor:.2f} ms, "
             f"apply {1e3 * t_apply:.3f} ms ({out[prec]['apply_gbs']:.0f} GB/s), "
-            f"mv {1e3 * t_mv:.3f} ms ({out[prec]['mv_gbs']:.0f} GB/s)")
+            f"mv {1e3 * t_mv:.3f} ms ({out[prec]['mv_gbs']:.0f} GB/s)"
+            + (f", folded apply {folded['apply_folded_ms']:.3f} ms ({folded['apply_folded_gbs']:.0f} GB/s, "
+               f"fold {folded['fold_ms']:.2f} ms)" if folded else ""))
         M.tear_down()
     x64, x32 = out["fp64"].pop("x"), out["fp32"].pop("x")
     out["fp32"]["x_rel_vs_fp64"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
@@ -245,7 +265,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kreps", type=int, default=20, help="kernel timing repetitions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--schedule", choices=["levels", "dataflow"], default=None,
+    ap.add_argument("--schedule", choices=["levels", "dataflow", "folded"], default=None,
                     help="ILU(0) apply schedule (default: the library's)")
     ap.add_argument("--extras", default="p3,batch,serve",
                     help="comma list of extra sections: p3 (16^3 P3 mixed precision), batch (64 uneven "

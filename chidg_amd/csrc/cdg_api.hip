@@ -564,6 +564,11 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
         M->lev_lo_ptr[l + 1] = (int32_t)lev_lo.size();
     }
     if (!e) e = upload_array(ctx, &M->d_lev_lo, lev_lo, &M->owned);
+    {   // the W jobs: every upper slot
+        std::vector<int32_t> all(std::max(M->nup, 1));
+        for (int u = 0; u < M->nup; ++u) all[u] = u;
+        if (!e) e = upload_array(ctx, &M->d_up_all, all, &M->owned);
+    }
     if (!e) e = upload_array(ctx, &M->d_lev_rows, forder, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_blev_rows, border, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_err, std::vector<int32_t>(1, 0), &M->owned);
@@ -605,12 +610,14 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     if (const char *m = std::getenv("CDG_APPLY_MODE")) {
         if (!std::strcmp(m, "dataflow")) M->apply_mode = CDG_APPLY_DATAFLOW;
         else if (!std::strcmp(m, "levels")) M->apply_mode = CDG_APPLY_LEVELS;
+        else if (!std::strcmp(m, "folded") && precision != CDG_BF16) M->apply_mode = CDG_APPLY_FOLDED;
     }
     M->h_lev_rows = forder;
     M->h_blev_rows = border;
     const int64_t part_doubles = 3 * M->part_len;
     if (!e) e = alloc((void **)&M->d_part, sizeof(double) * (size_t)part_doubles);
     if (!e && M->apply_mode == CDG_APPLY_DATAFLOW) e = df_setup(ctx, M);
+    if (!e && M->apply_mode == CDG_APPLY_FOLDED) e = alloc(&M->W, es * (size_t)M->nup * M->ld);
     if (!e) {
         hipError_t he = hipStreamSynchronize(ctx->stream);
         if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
@@ -622,7 +629,7 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
 
 extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule) {
     if (!ctx || !M) return CDG_ERR_ARG;
-    if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW)
+    if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW && schedule != CDG_SCHEDULE_FOLDED)
         return cdg_fail(ctx, CDG_ERR_ARG, "unknown apply schedule %d", schedule);
     if (M->type != CDG_PREC_ILU0) return CDG_OK;
     if (schedule == CDG_SCHEDULE_DATAFLOW) {
@@ -630,7 +637,18 @@ extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule
         if (e) return e;
         CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW : CDG_APPLY_LEVELS;
+    if (schedule == CDG_SCHEDULE_FOLDED) {
+        if (M->precision == CDG_BF16)
+            return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "folded schedule: FP64 and FP32 factors only");
+        if (!M->W) {
+            const size_t bytes = std::max<size_t>(16, (M->precision == CDG_FP64 ? 8 : 4) * (size_t)M->nup * M->ld);
+            int e = dev_alloc(ctx, &M->W, bytes);
+            if (e) return e;
+            M->owned.push_back({M->W, bytes});
+        }
+    }
+    M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW
+                    : schedule == CDG_SCHEDULE_FOLDED ? CDG_APPLY_FOLDED : CDG_APPLY_LEVELS;
     return CDG_OK;
 }
 
@@ -653,6 +671,7 @@ extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int3
     if (A->nrows != M->nrows || A->n != M->n) return cdg_fail(ctx, CDG_ERR_ARG, "matrix does not match preconditioner");
     if (M->type == CDG_PREC_IDENTITY) { M->stamp = A->stamp; M->factored = 1; return CDG_OK; }
     M->factored = 0;
+    M->w_ready = 0;
     int e = launch_ilu_factor(ctx, M, A, bad_row);
     if (e) return e;
     M->stamp = A->stamp;
@@ -665,6 +684,11 @@ static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const dou
     if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len, v, z);
     if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
     if (M->apply_mode == CDG_APPLY_DATAFLOW) return launch_ilu_apply_df(ctx, M, A, v, z);
+    if (M->apply_mode == CDG_APPLY_FOLDED && !M->w_ready) {   // schedule chosen after the factor
+        int e = launch_fold_all(ctx, M, A);
+        if (e) return e;
+        M->w_ready = 1;
+    }
     return launch_ilu_apply(ctx, M, A, v, z);
 }
 

@@ -82,6 +82,12 @@ struct cdg_prec {
     // factor storage (element type by precision)
     void *L = nullptr;                 // nlo slots
     void *U = nullptr;                 // nup slots (FP32/BF16 copies of A's upper blocks)
+    // nup slots, FP64/FP32: W_u = A_{up_blk[u]} Dinv_{up_col[u]}, the upper
+    // blocks with the Dinv of their column folded in (the level schedule's
+    // backward sweep is then one stage per level); null for BF16
+    void *W = nullptr;
+    int32_t w_ready = 0;               // W holds the current factor's products
+    int32_t *d_up_all = nullptr;       // 0..nup-1 (the W jobs)
     void *Dinv = nullptr;              // nrows slots
     float *L32 = nullptr, *D32 = nullptr;   // FP32 work for the BF16 variant
     void *inv_w = nullptr;             // working slots of the global-memory block inverse
@@ -109,8 +115,9 @@ struct cdg_prec {
     // `slice_w` columns; shape[precision] is the fixed-row walk for that element
     int32_t slices = 1, slice_w = 1, unroll = 4;
     StreamShape shape[3];
-    // apply schedule: CDG_APPLY_LEVELS (one launch per level and stage) or
-    // CDG_APPLY_DATAFLOW (one persistent launch, k_dfapply.hip)
+    // apply schedule: CDG_APPLY_LEVELS (one launch per level and stage),
+    // CDG_APPLY_DATAFLOW (one persistent launch, k_dfapply.hip) or
+    // CDG_APPLY_FOLDED (levels, backward over W = U Dinv)
     int32_t apply_mode = 0;
     int32_t df_T = 0, df_dc = 0, df_rounds = 0, df_w = 0, df_S = 0, df_grid = 0, df_ntasks = 0;
     size_t df_lds = 0;
@@ -131,7 +138,9 @@ int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *
 int launch_ilu_apply_df(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 int df_configure(cdg_ctx *ctx, cdg_prec *M);
 int df_reset(cdg_ctx *ctx, cdg_prec *M);
-enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1 };
+enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2 };
+// W_u = A_{up_blk[u]} Dinv_{up_col[u]} for all upper blocks (after a factor)
+int launch_fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A);
 int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row);
 int launch_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_diag, double s_off);
 int launch_vec_synthetic(cdg_ctx *ctx, int64_t len, uint64_t seed, double scale, double *v);

@@ -109,6 +109,7 @@ extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_par
     int64_t f = 8 * nnzb * ld;                                   // A
     f += es * (nlo + nrows) * ld;                                // L, Dinv
     if (req->precision != CDG_FP64) f += es * nlo * ld;         // U copy
+    if (req->precision != CDG_BF16 && req->nrhs > 1) f += es * nlo * ld;   // W = U Dinv (folded schedule)
     if (req->precision == CDG_BF16) f += 4 * (nlo + nrows) * ld; // FP32 factor work
     if (n > 192) f += 8 * nrows * ld;                            // inverse working slots (bound)
     f += 3 * 8 * nrows * 8 * n;                                  // apply partial sums (<= 8 slices)
@@ -137,6 +138,9 @@ int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result 
     if (!e) e = dev_alloc(ctx, (void **)&x, sizeof(double) * len);
     if (!e) e = launch_vec_synthetic(ctx, len, r.seed_b, 1.0, b);
     if (!e) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, r.precision, &M);
+    // several right-hand sides share the factor: fold Dinv into the upper
+    // blocks (one more GEMM per upper block, a shorter backward sweep)
+    if (!e && r.nrhs > 1 && r.precision != CDG_BF16) e = cdg_prec_set_schedule(ctx, M, CDG_SCHEDULE_FOLDED);
     // right-hand sides k = 0.. share the matrix and the factor (updated once)
     const int nrhs = std::max(1, r.nrhs);
     cdg_fgmres_params prm = params;

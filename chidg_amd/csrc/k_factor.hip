@@ -188,7 +188,8 @@ __device__ __forceinline__ void gemm_out(int tm, int tn, int n, typename Mfma<T>
         }
 }
 
-// phase 1: L_q = A_{lo_blk[q]} * Dinv_{lo_col[q]}
+// phase 1: L_q = A_{lo_blk[q]} * Dinv_{lo_col[q]} (with the upper lists in
+// place of the lower ones: W_u = A_{up_blk[u]} * Dinv_{up_col[u]})
 template <typename T>
 __global__ void __launch_bounds__(GT)
 k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
@@ -1309,8 +1310,20 @@ int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows,
 }
 
 template <typename T>
+int fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const T *D) {
+    const int n = M->n, tiles = (n + TM - 1) / TM;
+    const int u1 = M->nup;
+    if (u1 == 0) return CDG_OK;
+    hipLaunchKernelGGL((k_fac_p1<T>), dim3(xcd_grid(u1, tiles * tiles)), dim3(GT), 0, ctx->stream, A->blocks, A->ld,
+                       static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1, M->d_up_blk, M->d_up_col, n, tiles);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+template <typename T>
 int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D, const float *Ub32) {
     const int n = M->n;
+    const bool fold = M->apply_mode == CDG_APPLY_FOLDED && M->W && M->precision != CDG_BF16;
     const int tiles = (n + TM - 1) / TM;
     for (int l = 0; l < M->nlev; ++l) {
         const int q0 = M->lev_lo_ptr[l], q1 = M->lev_lo_ptr[l + 1];
@@ -1335,8 +1348,14 @@ int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D, const
             if (e) return e;
         }
     }
+    if (fold) {   // W = U Dinv for all upper blocks, one batched launch
+        int e = fold_all<T>(ctx, M, A, D);
+        if (e) return e;
+        M->w_ready = 1;
+    }
     return CDG_OK;
 }
+
 
 }  // namespace
 
@@ -1398,4 +1417,11 @@ int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_
         return cdg_fail(ctx, CDG_ERR_SINGULAR, "singular diagonal block in row %d", err);
     }
     return CDG_OK;
+}
+
+int launch_fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
+    if (!M->W) return cdg_fail(ctx, CDG_ERR_STATE, "folded schedule without W storage");
+    if (M->precision == CDG_FP64) return fold_all<double>(ctx, M, A, static_cast<const double *>(M->Dinv));
+    if (M->precision == CDG_FP32) return fold_all<float>(ctx, M, A, static_cast<const float *>(M->Dinv));
+    return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "folded schedule: FP64 and FP32 factors only");
 }

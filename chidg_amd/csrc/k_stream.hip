@@ -335,20 +335,40 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
         int e = launch_kslice<S, Acc>(ctx, M, f, r1 - r0);
         if (e) return e;
     }
-    for (int l = 0; l < M->nblev; ++l) {
-        const int r0 = M->blev_row_ptr[l], r1 = M->blev_row_ptr[l + 1];
-        KSliceArgs u{};
-        u.B = U; u.ld = ldu; u.rows = M->d_blev_rows + r0;
-        u.ptr = M->d_up_ptr; u.idx = M->d_up_idx; u.col = M->d_up_col;
-        u.x = zb; u.part = QA;
-        int e = launch_kslice<S, Acc>(ctx, M, u, r1 - r0);
-        if (e) return e;
+    if (M->apply_mode == CDG_APPLY_FOLDED && M->w_ready) {
+        // backward with W_ik = U_ik Dinv_k: t_i = y_i - sum_k W_ik t_k, one stage
+        // per level, then z = Dinv t for all rows at once
+        for (int l = 0; l < M->nblev; ++l) {
+            const int r0 = M->blev_row_ptr[l], r1 = M->blev_row_ptr[l + 1];
+            KSliceArgs u{};
+            u.B = M->W; u.ld = M->ld; u.rows = M->d_blev_rows + r0;
+            u.ptr = M->d_up_ptr; u.idx = nullptr; u.col = M->d_up_col;
+            u.x = t; u.part = QA;
+            int e = launch_kslice<S, Acc>(ctx, M, u, r1 - r0);
+            if (e) return e;
+        }
         KSliceArgs d{};
-        d.B = M->Dinv; d.ld = M->ld; d.rows = M->d_blev_rows + r0;
+        d.B = M->Dinv; d.ld = M->ld; d.rows = M->d_blev_rows;
         d.ptr = nullptr; d.idx = nullptr; d.col = nullptr;
         d.x = t; d.part = PB;
-        e = launch_kslice<S, Acc>(ctx, M, d, r1 - r0);
+        int e = launch_kslice<S, Acc>(ctx, M, d, M->nrows);
         if (e) return e;
+    } else {   // U and Dinv stages per level (BF16: no W)
+        for (int l = 0; l < M->nblev; ++l) {
+            const int r0 = M->blev_row_ptr[l], r1 = M->blev_row_ptr[l + 1];
+            KSliceArgs u{};
+            u.B = U; u.ld = ldu; u.rows = M->d_blev_rows + r0;
+            u.ptr = M->d_up_ptr; u.idx = M->d_up_idx; u.col = M->d_up_col;
+            u.x = zb; u.part = QA;
+            int e = launch_kslice<S, Acc>(ctx, M, u, r1 - r0);
+            if (e) return e;
+            KSliceArgs d{};
+            d.B = M->Dinv; d.ld = M->ld; d.rows = M->d_blev_rows + r0;
+            d.ptr = nullptr; d.idx = nullptr; d.col = nullptr;
+            d.x = t; d.part = PB;
+            e = launch_kslice<S, Acc>(ctx, M, d, r1 - r0);
+            if (e) return e;
+        }
     }
     const int64_t len = (int64_t)M->nrows * M->n;
     hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<int64_t>((len + 255) / 256, 4096)), dim3(256), 0,

@@ -201,9 +201,11 @@ class _Preconditioner:
         return self
 
     def set_schedule(self, schedule: str):
-        """ILU(0) apply schedule: 'levels' (one launch per wavefront level) or
-        'dataflow' (one persistent launch)."""
-        code = {"levels": 0, "dataflow": 1}[schedule]
+        """ILU(0) apply schedule: 'levels' (one launch per wavefront level),
+        'dataflow' (one persistent launch) or 'folded' (levels, with Dinv
+        folded into the upper blocks by the factor; for factors that serve
+        many applies; FP64/FP32)."""
+        code = {"levels": 0, "dataflow": 1, "folded": 2}[schedule]
         self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
         self.schedule = schedule
 

@@ -149,9 +149,16 @@ int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
  * apply is one sequential loop over rows, precon_ILU0.f90:286-411).  Both give
  * the same result for every schedule-independent input order:
  *   CDG_SCHEDULE_LEVELS    one launch per wavefront level and stage
- *   CDG_SCHEDULE_DATAFLOW  one persistent launch, row-to-row dependencies  */
+ *   CDG_SCHEDULE_DATAFLOW  one persistent launch, row-to-row dependencies
+ *   CDG_SCHEDULE_FOLDED    level schedule with Dinv folded into the upper
+ *                          blocks: the factor also forms W = U Dinv (one more
+ *                          block GEMM per upper block, nup more block slots),
+ *                          and the backward sweep is one stage per level.
+ *                          Pays off when one factor serves many applies
+ *                          (several right-hand sides); FP64 and FP32 only.  */
 #define CDG_SCHEDULE_LEVELS   0
 #define CDG_SCHEDULE_DATAFLOW 1
+#define CDG_SCHEDULE_FOLDED   2
 int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule);
 /* Factor as FP64 for inspection: L (nnzb*n*n, lower kind-0 slots filled,
  * others zero) and Dinv (nrows*n*n), the LD matrix of precon_ILU0_t. */

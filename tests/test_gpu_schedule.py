@@ -125,3 +125,61 @@ def test_dataflow_gives_up_cleanly(ctx, monkeypatch):
     M2.update(A)
     L_ref, D_ref = po.ilu0_factor(p, blocks)
     assert rel(M2.apply(A, v), po.ilu0_apply(p, blocks, L_ref, D_ref, v)) < 1e-12
+
+
+@pytest.mark.parametrize("dims,degree,precision,tol", [c for c in CASES if c[2] != "bf16"])
+def test_folded_matches_oracle_and_levels(ctx, dims, degree, precision, tol):
+    """Dinv folded into the upper blocks (W = U Dinv formed by the factor):
+    same apply as the oracle and the level schedule up to rounding, and
+    bitwise repeatable."""
+    p, blocks = synthetic_system(*dims, degree, seed=17)
+    A = upload(ctx, p, blocks)
+    Ml = precon_ILU0(precision, schedule="levels").init(A)
+    Mf = precon_ILU0(precision, schedule="folded").init(A)
+    Ml.update(A)
+    Mf.update(A)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    for seed in (3, 4):
+        v = po.fill_vector(p.vec_len, seed)
+        z_ref = po.ilu0_apply(p, blocks, L_ref, D_ref, v)
+        zf = Mf.apply(A, v)
+        assert rel(zf, z_ref) < tol
+        assert rel(zf, Ml.apply(A, v)) < (1e-13 if precision == "fp64" else tol)
+        assert np.array_equal(Mf.apply(A, v), zf)
+
+
+def test_folded_schedule_set_after_factor(ctx):
+    """Choosing the folded schedule after an update forms W at the next apply."""
+    p, blocks = synthetic_system(4, 3, 2, 1, seed=5)
+    A = upload(ctx, p, blocks)
+    M = precon_ILU0("fp64").init(A)
+    M.update(A)
+    v = po.fill_vector(p.vec_len, 9)
+    z_levels = M.apply(A, v)
+    M.set_schedule("folded")
+    assert rel(M.apply(A, v), z_levels) < 1e-13
+    M.update(A)                      # a new factor forms W with it
+    assert rel(M.apply(A, v), z_levels) < 1e-13
+
+
+def test_folded_fgmres_full_size(ctx):
+    """16^3 periodic P2, FP32 ILU, folded schedule: FGMRES(30) reaches rtol 1e-8
+    and matches the level schedule's solution."""
+    import torch
+    n = pt.block_size(2)
+    p = pt.hex_box(16, 16, 16, n)
+    A = chidg_matrix(p, ctx)
+    A.fill_synthetic(20261202, 1 / np.sqrt(n), 0.1 / np.sqrt(n))
+    dev = torch.empty(0, dtype=torch.float64, device="cuda:0")
+    b = A.synthetic_vector(1, like=dev)
+    xs = {}
+    for sched in ("levels", "folded"):
+        x = torch.zeros_like(b)
+        ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
+        M = precon_ILU0("fp32", schedule=sched)
+        ls.solve(A, x, b, M)
+        assert ls.residual_norm / float(torch.linalg.norm(b)) < 1e-8
+        xs[sched] = x
+        M.tear_down()
+    assert float(torch.linalg.norm(xs["folded"] - xs["levels"]) / torch.linalg.norm(xs["levels"])) < 1e-10
+    A.release()

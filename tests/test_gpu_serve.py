@@ -58,13 +58,17 @@ def test_poisson_burst_completes():
 
 
 def test_multi_rhs_shares_factor():
-    """nrhs right-hand sides on one factor: the first equals a one-RHS request
-    bitwise, every one converges."""
+    """nrhs right-hand sides on one factor (folded schedule: Dinv folded into
+    the upper blocks): the first agrees with a one-RHS request up to the
+    re-association of the backward products, every one converges, and the
+    same request gives the same bits again."""
     one = Request(6, 6, 6, 2, seed_A=301, seed_b=11)
     three = Request(6, 6, 6, 2, seed_A=301, seed_b=11, nrhs=3)
     with Server(nslots=2, **KW) as s:
         a = s.wait(s.submit(one, want_x=True))
         b = s.wait(s.submit(three, want_x=True))
-    assert np.array_equal(a.x, b.x)
+        c = s.wait(s.submit(three, want_x=True))
+    assert np.linalg.norm(a.x - b.x) / np.linalg.norm(a.x) < 1e-10
+    assert np.array_equal(b.x, c.x)
     assert b.niter >= 3 * 1 and b.niter >= a.niter
     assert b.residual_norm < 1e-8
