@@ -28,6 +28,7 @@ EXPORTS = [
     "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
     "cdg_server_last_error", "cdg_server_destroy",
     "cdg_fgmres_solve", "cdg_fgmres_default_params",
+    "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian",
 ]
 
 
@@ -118,6 +119,9 @@ def load() -> C.CDLL:
         "cdg_server_info": ([vp, ip, ip, ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "cdg_server_last_error": ([vp], C.c_char_p),
         "cdg_server_destroy": ([vp], C.c_int),
+        "cdg_euler_state": ([C.c_int32] * 4 + [C.c_double, C.c_uint64, C.c_double, dp], C.c_int),
+        "cdg_euler_residual": ([C.c_int32] * 4 + [dp, dp, C.c_int32], C.c_int),
+        "cdg_euler_jacobian": ([C.c_int32] * 4 + [dp, C.c_double, dp, dp, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -605,10 +605,11 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
         cdg_prec_destroy(ctx, M);
         return e ? e : cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply needs %zu bytes of LDS", lds);
     }
-    // apply schedule (CDG_APPLY_MODE=levels|dataflow overrides the default)
-    M->apply_mode = CDG_APPLY_LEVELS;
+    // apply schedule (CDG_APPLY_MODE=levels|dataflow|folded|auto overrides the default)
+    M->apply_mode = CDG_APPLY_AUTO;
     if (const char *m = std::getenv("CDG_APPLY_MODE")) {
         if (!std::strcmp(m, "dataflow")) M->apply_mode = CDG_APPLY_DATAFLOW;
+        else if (!std::strcmp(m, "auto")) M->apply_mode = CDG_APPLY_AUTO;
         else if (!std::strcmp(m, "levels")) M->apply_mode = CDG_APPLY_LEVELS;
         else if (!std::strcmp(m, "folded") && precision != CDG_BF16) M->apply_mode = CDG_APPLY_FOLDED;
     }
@@ -618,6 +619,8 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     if (!e) e = alloc((void **)&M->d_part, sizeof(double) * (size_t)part_doubles);
     if (!e && M->apply_mode == CDG_APPLY_DATAFLOW) e = df_setup(ctx, M);
     if (!e && M->apply_mode == CDG_APPLY_FOLDED) e = alloc(&M->W, es * (size_t)M->nup * M->ld);
+    if (const char *f = std::getenv("CDG_FOLD_AFTER")) M->fold_after = std::max(0, std::atoi(f));
+    if (precision == CDG_BF16) M->fold_after = 0;
     if (!e) {
         hipError_t he = hipStreamSynchronize(ctx->stream);
         if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
@@ -629,7 +632,8 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
 
 extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule) {
     if (!ctx || !M) return CDG_ERR_ARG;
-    if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW && schedule != CDG_SCHEDULE_FOLDED)
+    if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW && schedule != CDG_SCHEDULE_FOLDED &&
+        schedule != CDG_SCHEDULE_AUTO)
         return cdg_fail(ctx, CDG_ERR_ARG, "unknown apply schedule %d", schedule);
     if (M->type != CDG_PREC_ILU0) return CDG_OK;
     if (schedule == CDG_SCHEDULE_DATAFLOW) {
@@ -648,7 +652,8 @@ extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule
         }
     }
     M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW
-                    : schedule == CDG_SCHEDULE_FOLDED ? CDG_APPLY_FOLDED : CDG_APPLY_LEVELS;
+                    : schedule == CDG_SCHEDULE_FOLDED ? CDG_APPLY_FOLDED
+                    : schedule == CDG_SCHEDULE_AUTO ? CDG_APPLY_AUTO : CDG_APPLY_LEVELS;
     return CDG_OK;
 }
 
@@ -672,6 +677,7 @@ extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int3
     if (M->type == CDG_PREC_IDENTITY) { M->stamp = A->stamp; M->factored = 1; return CDG_OK; }
     M->factored = 0;
     M->w_ready = 0;
+    M->napply = 0;
     int e = launch_ilu_factor(ctx, M, A, bad_row);
     if (e) return e;
     M->stamp = A->stamp;
@@ -684,7 +690,16 @@ static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const dou
     if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len, v, z);
     if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
     if (M->apply_mode == CDG_APPLY_DATAFLOW) return launch_ilu_apply_df(ctx, M, A, v, z);
-    if (M->apply_mode == CDG_APPLY_FOLDED && !M->w_ready) {   // schedule chosen after the factor
+    const bool fold_now = !M->w_ready && (M->apply_mode == CDG_APPLY_FOLDED ||   // schedule chosen after the factor
+                                          (M->apply_mode == CDG_APPLY_AUTO && M->fold_after > 0 &&
+                                           ++M->napply > M->fold_after));
+    if (fold_now) {
+        if (!M->W) {
+            const size_t bytes = std::max<size_t>(16, (M->precision == CDG_FP64 ? 8 : 4) * (size_t)M->nup * M->ld);
+            int e = dev_alloc(ctx, &M->W, bytes);
+            if (e) return e;
+            M->owned.push_back({M->W, bytes});
+        }
         int e = launch_fold_all(ctx, M, A);
         if (e) return e;
         M->w_ready = 1;

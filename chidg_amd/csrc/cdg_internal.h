@@ -87,6 +87,9 @@ struct cdg_prec {
     // backward sweep is then one stage per level); null for BF16
     void *W = nullptr;
     int32_t w_ready = 0;               // W holds the current factor's products
+    // CDG_APPLY_AUTO: W is formed once the factor has served fold_after
+    // applies (0: never), so long solves get the shorter backward sweep
+    int32_t fold_after = 12, napply = 0;
     int32_t *d_up_all = nullptr;       // 0..nup-1 (the W jobs)
     void *Dinv = nullptr;              // nrows slots
     float *L32 = nullptr, *D32 = nullptr;   // FP32 work for the BF16 variant
@@ -117,7 +120,8 @@ struct cdg_prec {
     StreamShape shape[3];
     // apply schedule: CDG_APPLY_LEVELS (one launch per level and stage),
     // CDG_APPLY_DATAFLOW (one persistent launch, k_dfapply.hip) or
-    // CDG_APPLY_FOLDED (levels, backward over W = U Dinv)
+    // CDG_APPLY_FOLDED (levels, backward over W = U Dinv) or CDG_APPLY_AUTO
+    // (levels, then folded once the factor has served fold_after applies)
     int32_t apply_mode = 0;
     int32_t df_T = 0, df_dc = 0, df_rounds = 0, df_w = 0, df_S = 0, df_grid = 0, df_ntasks = 0;
     size_t df_lds = 0;
@@ -138,9 +142,13 @@ int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *
 int launch_ilu_apply_df(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 int df_configure(cdg_ctx *ctx, cdg_prec *M);
 int df_reset(cdg_ctx *ctx, cdg_prec *M);
-enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2 };
+enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2, CDG_APPLY_AUTO = 3 };
 // W_u = A_{up_blk[u]} Dinv_{up_col[u]} for all upper blocks (after a factor)
 int launch_fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A);
+// Uniform hex-box pattern in ChiDG row order (cdg_serve.hip); periodic: bit d
+// for direction d; face (optional) 1..6 face id per entry, 7 the diagonal
+void cdg_hex_box(int nx, int ny, int nz, int periodic, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
+                 std::vector<int32_t> &kind, std::vector<int32_t> *face = nullptr);
 int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_row);
 int launch_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_diag, double s_off);
 int launch_vec_synthetic(cdg_ctx *ctx, int64_t len, uint64_t seed, double scale, double *v);

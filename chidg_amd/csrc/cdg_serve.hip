@@ -21,46 +21,58 @@
 #include <thread>
 #include <unordered_map>
 
-namespace {
-
-using clk = std::chrono::steady_clock;
-
-double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
-
 // Uniform hex box in ChiDG row order: element i fastest; per row the interior
 // neighbours in face order XI_MIN, XI_MAX, ETA_MIN, ETA_MAX, ZETA_MIN, ZETA_MAX,
 // then the diagonal, then periodic wrap neighbours as coupling-only blocks
 // (a donor already present is not added twice).  Same order as
-// chidg_amd/pattern.py hex_box (type_domain_matrix.f90:134,253-374).
-void hex_box(int nx, int ny, int nz, int periodic, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
-             std::vector<int32_t> &kind) {
+// chidg_amd/pattern.py hex_box (type_domain_matrix.f90:134,253-374).  face
+// (optional): 1..6 for the face an entry came from (a coupling entry: the
+// first face that found its donor), 7 for the diagonal.
+void cdg_hex_box(int nx, int ny, int nz, int periodic, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
+                 std::vector<int32_t> &kind, std::vector<int32_t> *face) {
     const int dims[3] = {nx, ny, nz};
     const int stride[3] = {1, nx, nx * ny};
     const int nrows = nx * ny * nz;
     row_ptr.assign(nrows + 1, 0);
     col.clear();
     kind.clear();
+    if (face) face->clear();
     for (int e = 0; e < nrows; ++e) {
         const int idx[3] = {e % nx, (e / nx) % ny, e / (nx * ny)};
-        std::vector<int> chi;
+        std::vector<std::pair<int, int>> chi;   // (donor, face)
         for (int d = 0; d < 3; ++d)
             for (int side = -1; side <= 1; side += 2) {
+                const int fid = 1 + 2 * d + (side > 0);
                 const int q = idx[d] + side;
                 if (q >= 0 && q < dims[d]) {
                     col.push_back(e + side * stride[d]);
                     kind.push_back(CDG_KIND_LOCAL);
+                    if (face) face->push_back(fid);
                 } else if (periodic & (1 << d)) {
                     const int wrap = (q + dims[d]) % dims[d];
                     const int donor = e + (wrap - idx[d]) * stride[d];
-                    if (std::find(chi.begin(), chi.end(), donor) == chi.end()) chi.push_back(donor);
+                    bool seen = false;
+                    for (auto &c : chi) seen = seen || c.first == donor;
+                    if (!seen) chi.push_back({donor, fid});
                 }
             }
         col.push_back(e);
         kind.push_back(CDG_KIND_LOCAL);
-        for (int dnr : chi) { col.push_back(dnr); kind.push_back(CDG_KIND_COUPLING); }
+        if (face) face->push_back(7);
+        for (auto &c : chi) {
+            col.push_back(c.first);
+            kind.push_back(CDG_KIND_COUPLING);
+            if (face) face->push_back(c.second);
+        }
         row_ptr[e + 1] = (int32_t)col.size();
     }
 }
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
 int block_n(const cdg_request &r) { return (r.order + 1) * (r.order + 1) * (r.order + 1) * 5; }
 
@@ -99,7 +111,7 @@ extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_par
     if (!req || !params || !bytes) return CDG_ERR_ARG;
     if (req->nx <= 0 || req->ny <= 0 || req->nz <= 0 || req->order < 0 || req->order > 3) return CDG_ERR_ARG;
     std::vector<int32_t> rp, col, kind;
-    hex_box(req->nx, req->ny, req->nz, req->periodic, rp, col, kind);
+    cdg_hex_box(req->nx, req->ny, req->nz, req->periodic, rp, col, kind);
     const int64_t nrows = (int64_t)rp.size() - 1, nnzb = (int64_t)col.size();
     int64_t nloc = 0;
     for (int32_t k : kind) nloc += (k == CDG_KIND_LOCAL);
@@ -109,7 +121,7 @@ extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_par
     int64_t f = 8 * nnzb * ld;                                   // A
     f += es * (nlo + nrows) * ld;                                // L, Dinv
     if (req->precision != CDG_FP64) f += es * nlo * ld;         // U copy
-    if (req->precision != CDG_BF16 && req->nrhs > 1) f += es * nlo * ld;   // W = U Dinv (folded schedule)
+    if (req->precision != CDG_BF16) f += es * nlo * ld;         // W = U Dinv (folded backward sweep)
     if (req->precision == CDG_BF16) f += 4 * (nlo + nrows) * ld; // FP32 factor work
     if (n > 192) f += 8 * nrows * ld;                            // inverse working slots (bound)
     f += 3 * 8 * nrows * 8 * n;                                  // apply partial sums (<= 8 slices)
@@ -126,7 +138,7 @@ namespace {
 int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result &res) {
     const cdg_request &r = job.req;
     std::vector<int32_t> rp, col, kind;
-    hex_box(r.nx, r.ny, r.nz, r.periodic, rp, col, kind);
+    cdg_hex_box(r.nx, r.ny, r.nz, r.periodic, rp, col, kind);
     const int nrows = (int)rp.size() - 1, n = block_n(r);
     const int64_t len = (int64_t)nrows * n;
     cdg_mat *A = nullptr;

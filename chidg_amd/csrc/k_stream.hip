@@ -335,7 +335,7 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
         int e = launch_kslice<S, Acc>(ctx, M, f, r1 - r0);
         if (e) return e;
     }
-    if (M->apply_mode == CDG_APPLY_FOLDED && M->w_ready) {
+    if (M->w_ready && M->apply_mode != CDG_APPLY_LEVELS) {
         // backward with W_ik = U_ik Dinv_k: t_i = y_i - sum_k W_ik t_k, one stage
         // per level, then z = Dinv t for all rows at once
         for (int l = 0; l < M->nblev; ++l) {

@@ -201,11 +201,12 @@ class _Preconditioner:
         return self
 
     def set_schedule(self, schedule: str):
-        """ILU(0) apply schedule: 'levels' (one launch per wavefront level),
-        'dataflow' (one persistent launch) or 'folded' (levels, with Dinv
-        folded into the upper blocks by the factor; for factors that serve
-        many applies; FP64/FP32)."""
-        code = {"levels": 0, "dataflow": 1, "folded": 2}[schedule]
+        """ILU(0) apply schedule: 'levels' (one launch per wavefront level
+        and stage), 'dataflow' (one persistent launch), 'folded' (levels,
+        with Dinv folded into the upper blocks at the first apply after each
+        factor; FP64/FP32) or 'auto' (the default: 'levels' for the first 12
+        applies after each factor, 'folded' from then on)."""
+        code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3}[schedule]
         self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
         self.schedule = schedule
 

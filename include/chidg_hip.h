@@ -151,14 +151,21 @@ int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
  *   CDG_SCHEDULE_LEVELS    one launch per wavefront level and stage
  *   CDG_SCHEDULE_DATAFLOW  one persistent launch, row-to-row dependencies
  *   CDG_SCHEDULE_FOLDED    level schedule with Dinv folded into the upper
- *                          blocks: the factor also forms W = U Dinv (one more
- *                          block GEMM per upper block, nup more block slots),
- *                          and the backward sweep is one stage per level.
- *                          Pays off when one factor serves many applies
- *                          (several right-hand sides); FP64 and FP32 only.  */
+ *                          blocks: the first apply after each factor forms
+ *                          W = U Dinv (one batched block GEMM over the upper
+ *                          blocks, nup more block slots), and the backward
+ *                          sweep is one stage per level.  Pays off when one
+ *                          factor serves many applies; FP64 and FP32 only.
+ *   CDG_SCHEDULE_AUTO      (the default) CDG_SCHEDULE_LEVELS for the first 12
+ *                          applies after each factor, CDG_SCHEDULE_FOLDED from
+ *                          the 13th on (environment CDG_FOLD_AFTER=k sets k;
+ *                          0 never folds), so long solves get the shorter
+ *                          backward sweep and short ones skip the fold.  BF16
+ *                          factors never fold.  */
 #define CDG_SCHEDULE_LEVELS   0
 #define CDG_SCHEDULE_DATAFLOW 1
 #define CDG_SCHEDULE_FOLDED   2
+#define CDG_SCHEDULE_AUTO     3
 int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule);
 /* Factor as FP64 for inspection: L (nnzb*n*n, lower kind-0 slots filled,
  * others zero) and Dinv (nrows*n*n), the LD matrix of precon_ILU0_t. */
@@ -213,6 +220,31 @@ int cdg_server_info(cdg_server *s, int32_t *queued, int32_t *active, int32_t *pe
                     int64_t *budget);
 const char *cdg_server_last_error(cdg_server *s);
 int cdg_server_destroy(cdg_server *s);
+
+/* ---- DG Euler first-Newton-step system (host code) --------------------------
+ * The system ChiDG assembles for the first Newton step of a steady Euler case
+ * (mod_spatial.f90:40-197 residual and Jacobian, type_newton.f90:320-351
+ * pseudo-transient term) on the periodic uniform hex box [0,1]^3 with
+ * nx = ny = nz >= 2 elements per direction and order 0..3: 5 conservative
+ * fields, tensor Legendre modal basis (n = 5 (order+1)^3, block index
+ * field*nterms + mode), Gauss-Legendre quadrature with order+2 points,
+ * central flux plus Roe dissipation with the entropy fix of
+ * euler_roe_operator.f90:200-215, mass/dtau on the diagonal blocks with
+ * dtau = cfl h / (mean |u| + mean c) (type_fluid_pseudo_timestep.f90:95-155).
+ * Blocks follow the hex-box pattern (all directions periodic; the wrap
+ * neighbours are coupling blocks).  Vectors are nrows*n, element-major.
+ * nthreads <= 0: the host's cores (at most 16). */
+/* q: freestream rho = 1, p = 1/gamma, u = (mach, 0, 0) on mode 0, plus
+ * perturb * N(0,1) (seeded) on every coefficient. */
+int cdg_euler_state(int32_t nx, int32_t ny, int32_t nz, int32_t order, double mach, uint64_t seed, double perturb,
+                    double *q);
+/* R(q): sum_faces int phi Fhat.n - int grad(phi).F over each element. */
+int cdg_euler_residual(int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double *R,
+                       int32_t nthreads);
+/* blocks (nnzb*n*n, column-major per block) = dR/dq + mass/dtau (cfl > 0);
+ * R (may be NULL) receives R(q). */
+int cdg_euler_jacobian(int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double cfl,
+                       double *blocks, double *R, int32_t nthreads);
 
 #ifdef __cplusplus
 }

@@ -162,9 +162,35 @@ def test_folded_schedule_set_after_factor(ctx):
     assert rel(M.apply(A, v), z_levels) < 1e-13
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_auto_schedule_folds_after_12_applies(ctx, precision, tol):
+    """The default schedule: the first 12 applies after a factor are the level
+    schedule's bit for bit, the 13th on the folded one's; a new factor starts
+    the count again, and choosing 'levels' afterwards gives the level sweep."""
+    p, blocks = synthetic_system(4, 3, 2, 2, seed=21)
+    A = upload(ctx, p, blocks)
+    Ma = precon_ILU0(precision).init(A)
+    Ml = precon_ILU0(precision, schedule="levels").init(A)
+    Mf = precon_ILU0(precision, schedule="folded").init(A)
+    for M in (Ma, Ml, Mf):
+        M.update(A)
+    v = po.fill_vector(p.vec_len, 6)
+    zl, zf = Ml.apply(A, v), Mf.apply(A, v)
+    assert rel(zf, zl) < tol
+    for _ in range(2):
+        for k in range(12):
+            assert np.array_equal(Ma.apply(A, v), zl), k
+        for k in range(3):
+            assert np.array_equal(Ma.apply(A, v), zf), k
+        Ma.update(A)
+    Ma.set_schedule("levels")
+    for _ in range(14):
+        assert np.array_equal(Ma.apply(A, v), zl)
+
+
 def test_folded_fgmres_full_size(ctx):
-    """16^3 periodic P2, FP32 ILU, folded schedule: FGMRES(30) reaches rtol 1e-8
-    and matches the level schedule's solution."""
+    """16^3 periodic P2, FP32 ILU, folded and auto schedules: FGMRES(30) reaches
+    rtol 1e-8 and matches the level schedule's solution."""
     import torch
     n = pt.block_size(2)
     p = pt.hex_box(16, 16, 16, n)
@@ -173,7 +199,7 @@ def test_folded_fgmres_full_size(ctx):
     dev = torch.empty(0, dtype=torch.float64, device="cuda:0")
     b = A.synthetic_vector(1, like=dev)
     xs = {}
-    for sched in ("levels", "folded"):
+    for sched in ("levels", "folded", "auto"):
         x = torch.zeros_like(b)
         ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
         M = precon_ILU0("fp32", schedule=sched)
@@ -181,5 +207,6 @@ def test_folded_fgmres_full_size(ctx):
         assert ls.residual_norm / float(torch.linalg.norm(b)) < 1e-8
         xs[sched] = x
         M.tear_down()
-    assert float(torch.linalg.norm(xs["folded"] - xs["levels"]) / torch.linalg.norm(xs["levels"])) < 1e-10
+    for sched in ("folded", "auto"):
+        assert float(torch.linalg.norm(xs[sched] - xs["levels"]) / torch.linalg.norm(xs["levels"])) < 1e-10
     A.release()
