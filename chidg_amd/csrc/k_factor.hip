@@ -427,6 +427,17 @@ __device__ __forceinline__ float readlane(float x, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }
 
+// Diagonal-pivot threshold of the checked inverses: the diagonal a_kk is
+// taken as the pivot of column k when |a_kk| >= kPivTau * |a_rk| for every
+// unused row r (threshold partial pivoting with u = 0.1, the usual choice of
+// sparse direct solvers: Duff, Erisman & Reid, Direct Methods for Sparse
+// Matrices, ch. 5), which bounds the growth of a step by 1 + 1/kPivTau.  A
+// block where some step fails the test is inverted with full partial
+// pivoting instead.  Both give the inverse up to rounding; DG Jacobian
+// blocks (dR/dq + mass/dtau) pass with |a_kk| / max_r |a_rk| >= 0.33 where
+// strict partial pivoting would swap a third of the rows.
+constexpr double kPivTau = 0.1;
+
 // Gauss-Jordan with partial pivoting on one 16-column panel, run by one
 // wave.  On entry G[r*gp + c] holds the panel (rows r < NP, zero beyond n and
 // beyond the panel width w) and rowtag[r] != 0 marks rows pivoted in earlier
@@ -439,11 +450,10 @@ __device__ __forceinline__ float readlane(float x, int l) {
 // unrolled; after 16 rotations the columns are back in place.
 //
 // PIVOT = false takes row k as the pivot of column k (the rows pivoted so far
-// being 0..k-1) and only checks that partial pivoting would have chosen the
-// same row: no unused row with a strictly larger |a(r,k)| and a nonzero
-// finite pivot.  The check is off the dependency chain of the elimination;
-// when it holds at every step the arithmetic is exactly that of the pivoting
-// path.  *miss reports a step where it did not hold.
+// being 0..k-1) and only checks it against the threshold: no unused row with
+// kPivTau * |a(r,k)| > |a(k,k)|, and a nonzero finite pivot.  The check is off
+// the dependency chain of the elimination.  *miss reports a step where it did
+// not hold.
 template <typename T, int NR, bool PIVOT>
 __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, int w, int32_t *rowtag,
                                          int32_t *pivrow, T *prow_sh, bool *miss) {
@@ -499,7 +509,7 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
                 if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
 #pragma unroll
                 for (int a = 0; a < NR; ++a)
-                    if (!used[a] && (lane + 64 * a) != p && fabs(pr[a][0]) > ap) viol = true;
+                    if (!used[a] && (lane + 64 * a) != p && T(kPivTau) * fabs(pr[a][0]) > ap) viol = true;
             }
             const T rinv = T(1) / prow[0];
 #pragma unroll
@@ -582,7 +592,7 @@ __device__ __forceinline__ bool panel_gjw(T *G, int gp, int NP, int n, int k0, i
             for (int c = 0; c < 16; ++c) prow[c] = prow_sh[buf][c];
             const T ap = fabs(prow[0]);
             if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
-            if (!used && row != p && fabs(pr[0]) > ap) viol = true;
+            if (!used && row != p && T(kPivTau) * fabs(pr[0]) > ap) viol = true;
             const T rinv = T(1) / prow[0];
 #pragma unroll
             for (int c = 0; c < 16; ++c) prow[c] *= rinv;
@@ -642,9 +652,9 @@ __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 //   outside the pivot block) replays step j as soon as the counter shows it
 //   (acquire), so the two stages overlap.  Wave 0 never waits, so every wait
 //   ends.
-// Both stages check at each step that partial pivoting would have chosen the
-// diagonal: no unused row (row > k0 + j, row < n) with a strictly larger
-// |a(r,j)|, and a nonzero finite pivot.  The arithmetic is that of panel_gjw
+// Both stages check the diagonal pivot of each step against the threshold:
+// no unused row (row > k0 + j, row < n) with kPivTau * |a(r,j)| > |a(j,j)|,
+// and a nonzero finite pivot.  The arithmetic is that of panel_gjw
 // step for step.  On exit G holds C - E_P (E_P: ones at (k0 + c, c), c < w);
 // returns whether some step needed another pivot.  base = 16 * (panel index):
 // the counter only grows during the kernel.
@@ -678,7 +688,7 @@ __device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, in
                 case 2: cr = quad_bcast<2>(a[je]); break;
                 default: cr = quad_bcast<3>(a[je]); break;
                 }
-                if (i > j && k0 + i < n && fabs(cr) > ap) viol = true;
+                if (i > j && k0 + i < n && T(kPivTau) * fabs(cr) > ap) viol = true;
                 const bool isp = i == j, pc = q == jq;
                 T sv[4];
 #pragma unroll
@@ -724,7 +734,7 @@ __device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, in
             for (int c = 0; c < 16; ++c) sv[c] = H[j][c];
             const T ap = fabs(H[j][16]);
             const T cr = pr[j];
-            viol = viol | ((j < w) & unused & (fabs(cr) > ap));
+            viol = viol | ((j < w) & unused & (T(kPivTau) * fabs(cr) > ap));
 #pragma unroll
             for (int c = 0; c < 16; ++c)
                 if (c != j) pr[c] = fma(-cr, sv[c], pr[c]);
@@ -751,9 +761,9 @@ __device__ __forceinline__ bool panel_ab(T *G, int gp, int NP, int n, int k0, in
 //   steps j >= w the panel does not have, which leave the rows unchanged).
 //   Stage B (every lane, row lane * NW + wave outside the pivot block)
 //   replays the 16 steps from H with no barrier between them.
-// Both stages check at each step that partial pivoting would have chosen the
-// diagonal: no unused row (row > k0 + j, row < n) with a strictly larger
-// |a(r,j)|, and a nonzero finite pivot.  The arithmetic is that of panel_gjw
+// Both stages check the diagonal pivot of each step against the threshold:
+// no unused row (row > k0 + j, row < n) with kPivTau * |a(r,j)| > |a(j,j)|,
+// and a nonzero finite pivot.  The arithmetic is that of panel_gjw
 // step for step.  On exit G holds C - E_P (E_P: ones at (k0 + c, c), c < w);
 // returns whether some step needed another pivot.
 template <typename T, int NW>
@@ -783,7 +793,7 @@ __device__ __forceinline__ bool panel_ab_seq(T *G, int gp, int NP, int n, int k0
                 case 2: cr = quad_bcast<2>(a[je]); break;
                 default: cr = quad_bcast<3>(a[je]); break;
                 }
-                if (i > j && k0 + i < n && fabs(cr) > ap) viol = true;
+                if (i > j && k0 + i < n && T(kPivTau) * fabs(cr) > ap) viol = true;
                 const bool isp = i == j, pc = q == jq;
                 T sv[4];
 #pragma unroll
@@ -827,7 +837,7 @@ __device__ __forceinline__ bool panel_ab_seq(T *G, int gp, int NP, int n, int k0
         for (int c = 0; c < 16; ++c) sv[c] = H[j][c];
         const T ap = fabs(H[j][16]);
         const T cr = pr[j];
-        viol = viol | ((j < w) & unused & (fabs(cr) > ap));
+        viol = viol | ((j < w) & unused & (T(kPivTau) * fabs(cr) > ap));
 #pragma unroll
         for (int c = 0; c < 16; ++c)
             if (c != j) pr[c] = fma(-cr, sv[c], pr[c]);
@@ -993,9 +1003,9 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int
 //      tiles take C, formed on MFMA as (C - E_P + E_P) I from a zero
 //      accumulator (an accumulator written by VALU makes the compiler hold
 //      every tile in arch VGPRs and copy them all in each panel).
-// This is the unblocked in-place Gauss-Jordan reorganized, with the pivots
-// partial pivoting chooses whenever the check holds; a block where it fails
-// is left untouched and flagged in redo[] for k_fac_pivot.
+// This is the unblocked in-place Gauss-Jordan reorganized, on diagonal pivots
+// that pass the kPivTau threshold; a block where one fails is left untouched
+// and flagged in redo[] for k_fac_pivot.
 template <typename T, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_fac_binv(T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int n, int32_t *__restrict__ err,

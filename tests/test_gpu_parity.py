@@ -125,6 +125,33 @@ def test_ilu0_factor_apply_parity(ctx, dims, degree):
     assert np.array_equal(M.apply(A, v), z)
 
 
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_factor_blocks_needing_row_pivots(ctx, degree, precision, tol):
+    """Diagonal blocks 2P + 0.1 G/sqrt(n) (P a cyclic shift: the diagonal
+    entries are small next to each column's largest) on every other block
+    row.  Those blocks fail the diagonal-pivot threshold and take the
+    partial-pivoting inverse, the others keep diagonal pivots; factor and
+    apply match the oracle's (which pivots every block partially)."""
+    p, blocks = synthetic_system(3, 3, 2, degree, seed=31)
+    n = p.n
+    B = blocks.reshape(p.nnzb, n, n)            # per block: [column][row]
+    rng = np.random.default_rng(8)
+    rows = np.repeat(np.arange(p.nrows), np.diff(p.row_ptr))
+    shift = np.zeros((n, n))
+    shift[np.arange(n), (np.arange(n) + 1) % n] = 2.0
+    for b in np.flatnonzero((p.kind == pt.KIND_LOCAL) & (p.col == rows) & (rows % 2 == 0)):
+        B[b] = (shift + 0.1 * rng.standard_normal((n, n)) / np.sqrt(n)).T
+    A = upload(ctx, p, blocks)
+    M = precon_ILU0(precision).init(A)
+    M.update(A)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    _, D = M.factors(A)
+    assert rel(D, D_ref) < tol
+    v = po.fill_vector(p.vec_len, 12)
+    assert rel(M.apply(A, v), po.ilu0_apply(p, blocks, L_ref, D_ref, v)) < tol
+
+
 @pytest.mark.parametrize("orth", ["CGS", "MGS"])
 def test_fgmres_parity_config1(ctx, orth):
     """Config 1 shape: 4^3 periodic P1, FGMRES(30)+ILU0, rtol 1e-8, inner off."""
