@@ -465,10 +465,10 @@ static int df_setup(cdg_ctx *ctx, cdg_prec *M) {
     const int64_t need = 3 * (int64_t)M->nrows * S * M->n;
     if (need > 3 * M->part_len) {      // partial storage shared with the level schedule
         double *np = nullptr;
-        e = dev_alloc(ctx, (void **)&np, sizeof(double) * (size_t)need);
+        e = dev_alloc(ctx, (void **)&np, sizeof(double) * (size_t)(need + 2));   // +2 as at init
         if (e) return e;
         for (DevMem &m : M->owned)
-            if (m.p == M->d_part) { dev_free(ctx, m.p, m.bytes); m.p = np; m.bytes = sizeof(double) * (size_t)need; }
+            if (m.p == M->d_part) { dev_free(ctx, m.p, m.bytes); m.p = np; m.bytes = sizeof(double) * (size_t)(need + 2); }
         M->d_part = np;
         M->part_len = need / 3;
     }
@@ -615,7 +615,9 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     }
     M->h_lev_rows = forder;
     M->h_blev_rows = border;
-    const int64_t part_doubles = 3 * M->part_len;
+    // (+2: a sweep launch whose rows have no entries issues its 16-byte
+    // block prefetch from the start of a partial array, unused)
+    const int64_t part_doubles = 3 * M->part_len + 2;
     if (!e) e = alloc((void **)&M->d_part, sizeof(double) * (size_t)part_doubles);
     if (!e && M->apply_mode == CDG_APPLY_DATAFLOW) e = df_setup(ctx, M);
     if (!e && M->apply_mode == CDG_APPLY_FOLDED) e = alloc(&M->W, es * (size_t)M->nup * M->ld);

@@ -12,6 +12,7 @@
 #include "cdg_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -69,14 +70,14 @@ template <int V> __device__ __forceinline__ Walk<V> make_walk(int tid, int T, in
     return w;
 }
 
-// acc[v] += sum over this thread's elements of blk(e) * xs(col(e)); U loads
-// of 16 bytes in flight per thread.
+// acc[v] += sum over this thread's elements of blk(e) * xs(col(e)), from
+// walk step m0 on; U loads of 16 bytes in flight per thread.
 template <typename S, typename Acc, int U = 4>
 __device__ __forceinline__ void stream_block(const S *__restrict__ blk, const Acc *__restrict__ xs, int nn,
-                                             const Walk<Ld16<S, Acc>::V> &w, Acc *acc) {
+                                             const Walk<Ld16<S, Acc>::V> &w, Acc *acc, int m0 = 0) {
     constexpr int V = Ld16<S, Acc>::V;
     const int mfull = nn / w.step;           // steps where every element is in range
-    int m = 0;
+    int m = m0;
     for (; m + U <= mfull; m += U) {
         Acc a[U][V];
 #pragma unroll
@@ -166,23 +167,58 @@ k_mv(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ 
 // ============================================================================
 struct VecView {
     const double *base;       // may be null (0)
-    const double *p1, *p2;    // partial arrays [nrows][S][n], may be null
+    const double *p1, *p2;    // partial arrays [nrows][S][n]; p1 never null, p2 may be
     double s1, s2;            // signs of the partial sums
 };
 
-__device__ __forceinline__ double view_at(const VecView &u, int row, int c, int n, int S) {
-    double x = u.base ? u.base[(size_t)row * n + c] : 0.0;
-    if (u.p1) {
-        double t = 0.0;
-        for (int s = 0; s < S; ++s) t += u.p1[((size_t)row * S + s) * n + c];
-        x += u.s1 * t;
-    }
-    if (u.p2) {
-        double t = 0.0;
-        for (int s = 0; s < S; ++s) t += u.p2[((size_t)row * S + s) * n + c];
-        x += u.s2 * t;
-    }
+// Element c of row `row` of a view is gathered in two steps: view_load issues
+// the loads (the base and NB slices of p1 and, with P2, of p2, all from
+// clamped in-bounds addresses and without branches), view_sum adds them.
+// The partials are summed in slice order s = 0, 1, ..., S-1.  Splitting the
+// two lets a kernel put other loads (the first block chunks) in between: the
+// load counter is in order, so those stay in flight while the sums wait.
+template <int NB> struct ViewLd {
+    double b;
+    double q1[NB], q2[NB];
+};
+
+template <int NB, bool P2>
+__device__ __forceinline__ ViewLd<NB> view_load(const VecView &u, int row, int c, int n, int S, int s0) {
+    ViewLd<NB> r;
+    const size_t rb = (size_t)row * S * n + c;
+    r.b = *(u.base ? u.base + (size_t)row * n + c : u.p1 + rb);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) r.q1[i] = u.p1[rb + (size_t)min(s0 + i, S - 1) * n];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) r.q2[i] = P2 ? u.p2[rb + (size_t)min(s0 + i, S - 1) * n] : 0.0;
+    return r;
+}
+
+template <int NB>
+__device__ __forceinline__ void view_add(const ViewLd<NB> &r, int S, int s0, double &t1, double &t2) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (s0 + i < S) {
+            t1 += r.q1[i];
+            t2 += r.q2[i];
+        }
+}
+
+// u(row, c) from the loads of its first NB slices (the rest loaded here)
+template <int NB, bool P2>
+__device__ __forceinline__ double view_sum(const VecView &u, const ViewLd<NB> &r, int row, int c, int n, int S) {
+    double t1 = 0.0, t2 = 0.0;
+    view_add<NB>(r, S, 0, t1, t2);
+    for (int s0 = NB; s0 < S; s0 += NB) view_add<NB>(view_load<NB, P2>(u, row, c, n, S, s0), S, s0, t1, t2);
+    double x = u.base ? r.b : 0.0;
+    x += u.s1 * t1;
+    if (P2) x += u.s2 * t2;
     return x;
+}
+
+template <int NB, bool P2>
+__device__ __forceinline__ double view_at(const VecView &u, int row, int c, int n, int S) {
+    return view_sum<NB, P2>(u, view_load<NB, P2>(u, row, c, n, S, 0), row, c, n, S);
 }
 
 struct KSliceArgs {
@@ -197,7 +233,7 @@ struct KSliceArgs {
     int n, S, w, maxdep, dc;  // block size, slices, slice width (columns), max entries, walk step
 };
 
-template <typename S, typename Acc, int U>
+template <typename S, typename Acc, int U, int NB, bool P2>
 __global__ void __launch_bounds__(kMaxThreads) k_kslice(KSliceArgs a) {
     constexpr int V = Ld16<S, Acc>::V;
     extern __shared__ unsigned char smem_raw[];
@@ -212,29 +248,91 @@ __global__ void __launch_bounds__(kMaxThreads) k_kslice(KSliceArgs a) {
     int d0 = 0, d1 = 1;
     if (a.ptr) { d0 = a.ptr[row]; d1 = a.ptr[row + 1]; }
     const int nd = d1 - d0;
-    for (int e = tid; e < nd * wc; e += T) {
-        const int k = e / wc, c = e - k * wc;
-        const int parent = a.ptr ? a.col[d0 + k] : row;
-        xs[k * a.w + c] = static_cast<Acc>(view_at(a.x, parent, c0 + c, n, a.S));
-    }
-    __syncthreads();
     const Walk<V> wk = make_walk<V>(tid, T, n, a.dc);
     const S *Bp = static_cast<const S *>(a.B);
+    auto slot_of = [&](int k) { return a.ptr ? (a.idx ? a.idx[d0 + k] : d0 + k) : row; };
+    // the slice is the flat range [c0*n, c1*n) of each block; column index
+    // inside the slice = global column - c0, so xs is read at (col - c0).
+    // The first block's first U walk steps are loaded before the gather.
+    // Order of the loads: this thread's first gather element, then the
+    // first block's first U walk steps, then (after the sums) the rest.
+    // Both go out unconditionally from in-bounds addresses (threads beyond
+    // the gather read a valid element, a row without entries reads the
+    // partial array), so no branch separates them from the sums: a branch
+    // would make the compiler's load-counter waits cover the block loads too.
+    const int ne = nd * wc;
+    const bool g0 = tid < ne;
+    const int kg = g0 ? tid / wc : 0, cg = g0 ? tid - kg * wc : 0;
+    const int pg = a.ptr && nd > 0 ? a.col[d0 + kg] : row;
+    const ViewLd<NB> r0 = view_load<NB, P2>(a.x, pg, c0 + cg, n, a.S, 0);
+    const int m0 = nd > 0 ? min(U, wc * n / wk.step) : 0;
+    const S *b0 = nd > 0 ? Bp + (size_t)slot_of(0) * a.ld + (size_t)c0 * n : reinterpret_cast<const S *>(a.x.p1);
+    Acc a0[U][V];
+#pragma unroll
+    for (int u = 0; u < U; ++u) Ld16<S, Acc>::load(b0 + (u < m0 ? wk.e0 + u * wk.step : 0), a0[u]);
+    if (g0) xs[kg * a.w + cg] = static_cast<Acc>(view_sum<NB, P2>(a.x, r0, pg, c0 + cg, n, a.S));
+    for (int e = tid + T; e < ne; e += T) {
+        const int k = e / wc, c = e - k * wc;
+        const int parent = a.ptr ? a.col[d0 + k] : row;
+        xs[k * a.w + c] = static_cast<Acc>(view_at<NB, P2>(a.x, parent, c0 + c, n, a.S));
+    }
+    __syncthreads();
     Acc acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0;
-    // the slice is the flat range [c0*n, c1*n) of each block; column index
-    // inside the slice = global column - c0, so xs is read at (col - c0)
-    for (int k = 0; k < nd; ++k) {
-        const int slot = a.ptr ? (a.idx ? a.idx[d0 + k] : d0 + k) : row;
-        stream_block<S, Acc, U>(Bp + (size_t)slot * a.ld + (size_t)c0 * n, xs + k * a.w, wc * n, wk, acc);
-    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (u < m0)
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] += a0[u][v] * xs[wk.c0[v] + u * wk.dc];
+    for (int k = 0; k < nd; ++k)
+        stream_block<S, Acc, U>(Bp + (size_t)slot_of(k) * a.ld + (size_t)c0 * n, xs + k * a.w, wc * n, wk, acc,
+                                k == 0 ? m0 : 0);
 #pragma unroll
     for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
     __syncthreads();
     const int parts = V * T / n;
     double *out = a.part + ((size_t)row * a.S + sl) * n;
     for (int r = tid; r < n; r += T) out[r] = static_cast<double>(row_sum(red, r, n, parts));
+}
+
+// z_i = Dinv_i t_i for every row i, the folded schedule's last stage (t in
+// its partial-sum view).  One workgroup per row streams the whole block and
+// writes z_i itself; the block's first U walk steps are loaded before t_i is
+// gathered into LDS, so the two latencies overlap.
+template <typename S, typename Acc, int U, int NB>
+__global__ void __launch_bounds__(kMaxThreads) k_dinv_z(const S *__restrict__ D, int64_t ld, VecView t, int Sp,
+                                                        int n, int dc, double *__restrict__ z) {
+    constexpr int V = Ld16<S, Acc>::V;
+    extern __shared__ unsigned char smem_raw[];
+    const int T = blockDim.x, tid = threadIdx.x, row = blockIdx.x;
+    Acc *xs = reinterpret_cast<Acc *>(smem_raw);   // n (n + V*T Acc in all, 16-byte aligned pieces)
+    Acc *red = xs + ((n + 3) & ~3);                // V*T
+    const S *blk = D + (size_t)row * ld;
+    const int nn = n * n;
+    const Walk<V> w = make_walk<V>(tid, T, n, dc);
+    const ViewLd<NB> r0 = view_load<NB, true>(t, row, min(tid, n - 1), n, Sp, 0);
+    const int m0 = min(U, nn / w.step);
+    Acc a0[U][V];
+#pragma unroll
+    for (int u = 0; u < U; ++u) Ld16<S, Acc>::load(blk + (u < m0 ? w.e0 + u * w.step : 0), a0[u]);
+    if (tid < n) xs[tid] = static_cast<Acc>(view_sum<NB, true>(t, r0, row, tid, n, Sp));
+    for (int c = tid + T; c < n; c += T) xs[c] = static_cast<Acc>(view_at<NB, true>(t, row, c, n, Sp));
+    __syncthreads();
+    Acc acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (u < m0)
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] += a0[u][v] * xs[w.c0[v] + u * w.dc];
+    stream_block<S, Acc, U>(blk, xs, nn, w, acc, m0);
+#pragma unroll
+    for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
+    __syncthreads();
+    const int parts = V * T / n;
+    for (int r = tid; r < n; r += T) z[(size_t)row * n + r] = static_cast<double>(row_sum(red, r, n, parts));
 }
 
 // z = sum_s part[s]  (materializes the backward result)
@@ -310,10 +408,17 @@ int launch_kslice(cdg_ctx *ctx, const cdg_prec *M, KSliceArgs a, int nrows_lvl) 
     a.maxdep = M->maxdep;
     a.dc = sh.dc;
     const size_t lds = sizeof(Acc) * ((size_t)a.maxdep * a.w + (size_t)V * sh.T);
-    if (M->unroll == 8)
-        hipLaunchKernelGGL((k_kslice<S, Acc, 8>), dim3(nrows_lvl * M->slices), dim3(sh.T), lds, ctx->stream, a);
-    else
-        hipLaunchKernelGGL((k_kslice<S, Acc, 4>), dim3(nrows_lvl * M->slices), dim3(sh.T), lds, ctx->stream, a);
+    const dim3 grid(nrows_lvl * M->slices), block(sh.T);
+    const bool p2 = a.x.p2 != nullptr, nb4 = a.S <= 4;
+#define CDG_KSLICE(UU, NBB, PP) hipLaunchKernelGGL((k_kslice<S, Acc, UU, NBB, PP>), grid, block, lds, ctx->stream, a)
+    if (M->unroll == 8) {
+        if (nb4) { if (p2) CDG_KSLICE(8, 4, true); else CDG_KSLICE(8, 4, false); }
+        else { if (p2) CDG_KSLICE(8, 8, true); else CDG_KSLICE(8, 8, false); }
+    } else {
+        if (nb4) { if (p2) CDG_KSLICE(4, 4, true); else CDG_KSLICE(4, 4, false); }
+        else { if (p2) CDG_KSLICE(4, 8, true); else CDG_KSLICE(4, 8, false); }
+    }
+#undef CDG_KSLICE
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
@@ -335,7 +440,7 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
         int e = launch_kslice<S, Acc>(ctx, M, f, r1 - r0);
         if (e) return e;
     }
-    if (M->w_ready && M->apply_mode != CDG_APPLY_LEVELS) {
+    if (!std::is_same<S, cdg_bf16>::value && M->w_ready && M->apply_mode != CDG_APPLY_LEVELS) {
         // backward with W_ik = U_ik Dinv_k: t_i = y_i - sum_k W_ik t_k, one stage
         // per level, then z = Dinv t for all rows at once
         for (int l = 0; l < M->nblev; ++l) {
@@ -347,12 +452,18 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
             int e = launch_kslice<S, Acc>(ctx, M, u, r1 - r0);
             if (e) return e;
         }
-        KSliceArgs d{};
-        d.B = M->Dinv; d.ld = M->ld; d.rows = M->d_blev_rows;
-        d.ptr = nullptr; d.idx = nullptr; d.col = nullptr;
-        d.x = t; d.part = PB;
-        int e = launch_kslice<S, Acc>(ctx, M, d, M->nrows);
-        if (e) return e;
+        constexpr int V = Ld16<S, Acc>::V;
+        const StreamShape &sh = M->shape[M->precision];
+        const size_t lds = sizeof(Acc) * ((size_t)((M->n + 3) & ~3) + (size_t)V * sh.T);
+        using SD = typename std::conditional<std::is_same<S, cdg_bf16>::value, float, S>::type;   // no BF16 W
+        const SD *Dv = static_cast<const SD *>(M->Dinv);
+#define CDG_DINVZ(UU, NBB) hipLaunchKernelGGL((k_dinv_z<SD, Acc, UU, NBB>), dim3(M->nrows), dim3(sh.T), lds, \
+                                              ctx->stream, Dv, M->ld, t, M->slices, M->n, sh.dc, z)
+        if (M->unroll == 8) { if (M->slices <= 4) CDG_DINVZ(8, 4); else CDG_DINVZ(8, 8); }
+        else { if (M->slices <= 4) CDG_DINVZ(4, 4); else CDG_DINVZ(4, 8); }
+#undef CDG_DINVZ
+        CDG_HIP_TRY(hipGetLastError());
+        return CDG_OK;
     } else {   // U and Dinv stages per level (BF16: no W)
         for (int l = 0; l < M->nblev; ++l) {
             const int r0 = M->blev_row_ptr[l], r1 = M->blev_row_ptr[l + 1];
