@@ -679,6 +679,7 @@ extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int3
     if (M->type == CDG_PREC_IDENTITY) { M->stamp = A->stamp; M->factored = 1; return CDG_OK; }
     M->factored = 0;
     M->w_ready = 0;
+    M->prev_napply = M->napply;
     M->napply = 0;
     int e = launch_ilu_factor(ctx, M, A, bad_row);
     if (e) return e;
@@ -692,9 +693,10 @@ static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const dou
     if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len, v, z);
     if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
     if (M->apply_mode == CDG_APPLY_DATAFLOW) return launch_ilu_apply_df(ctx, M, A, v, z);
+    ++M->napply;
     const bool fold_now = !M->w_ready && (M->apply_mode == CDG_APPLY_FOLDED ||   // schedule chosen after the factor
                                           (M->apply_mode == CDG_APPLY_AUTO && M->fold_after > 0 &&
-                                           ++M->napply > M->fold_after));
+                                           (M->napply > M->fold_after || M->prev_napply > M->fold_after)));
     if (fold_now) {
         if (!M->W) {
             const size_t bytes = std::max<size_t>(16, (M->precision == CDG_FP64 ? 8 : 4) * (size_t)M->nup * M->ld);
@@ -837,6 +839,9 @@ struct Solver {
     Timers t;
     int nreorth = 0, nrestart = 0;
     int status = CDG_OK;
+    // per nesting level: events before the preconditioner, after it, after
+    // the matvec (timing without synchronizing the stream)
+    hipEvent_t ev[2][3] = {};
 };
 
 // fetch n device scalars from ctx->d_small into ctx->h_small
@@ -846,29 +851,36 @@ int fetch_small(cdg_ctx *ctx, size_t n) {
     return CDG_OK;
 }
 
-// One orthogonalization pass of w against V_0..V_{j}: htmp -> host hcol (+=),
-// returns ||w||^2 after the pass in *nrm2.  d_small layout: [0..j] htmp, [j+1] nrm2.
-int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *hcol, double *nrm2) {
+// One orthogonalization pass of w = V_jj (the slot of the next basis vector)
+// against V_0..V_{jj-1}: host hcol += the products, *nrm2 = ||w||^2 after the
+// pass, and with normb2 also ||w||^2 before it (CGS: w . w is one more
+// product of the same launch).  One device-to-host fetch per pass.
+// d_small layout from d_h: [0, jj) products, [jj] ||w||^2 before, [jj+1] after.
+int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *w, double *hcol, double *nrm2,
+                  double *normb2) {
     cdg_ctx *ctx = S.ctx;
     const int64_t len = S.len;
-    int e = ctx_ensure_small(ctx, (size_t)jj + 8 + ws.m + 8);
+    if (w != ws.V + (size_t)jj * len) return cdg_fail(ctx, CDG_ERR_STATE, "orthogonalize: w is not V_%d", jj);
+    int e = ctx_ensure_small(ctx, (size_t)jj + 2 + 8 + ws.m + 8);
     if (e) return e;
     double *d_h = ctx->d_small + 8 + ws.m + 8;   // beyond the combine scratch
     if (orth == CDG_MGS) {
+        if (normb2) e = kv_sumsq(ctx, len, w, d_h + jj);
         for (int i = 0; i < jj && !e; ++i) {
-            e = kv_multidot(ctx, len, ws.w, ws.V + (size_t)i * len, len, 1, d_h + i);
-            if (!e) e = kv_axpy_dev(ctx, len, ws.w, ws.V + (size_t)i * len, d_h + i);
+            e = kv_multidot(ctx, len, w, ws.V + (size_t)i * len, len, 1, d_h + i);
+            if (!e) e = kv_axpy_dev(ctx, len, w, ws.V + (size_t)i * len, d_h + i);
         }
-        if (!e) e = kv_sumsq(ctx, len, ws.w, d_h + jj);
+        if (!e) e = kv_sumsq(ctx, len, w, d_h + jj + 1);
     } else {
-        e = kv_multidot(ctx, len, ws.w, ws.V, len, jj, d_h);
-        if (!e) e = kv_cgs_update(ctx, len, ws.w, ws.V, len, jj, d_h, d_h + jj);
+        e = kv_multidot(ctx, len, w, ws.V, len, normb2 ? jj + 1 : jj, d_h);
+        if (!e) e = kv_cgs_update(ctx, len, w, ws.V, len, jj, d_h, d_h + jj + 1);
     }
     if (e) return e;
-    CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_h, sizeof(double) * (jj + 1), hipMemcpyDeviceToHost, ctx->stream));
+    CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_h, sizeof(double) * (jj + 2), hipMemcpyDeviceToHost, ctx->stream));
     CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < jj; ++i) hcol[i] += ctx->h_small[i];
-    *nrm2 = ctx->h_small[jj];
+    if (normb2) *normb2 = ctx->h_small[jj];
+    *nrm2 = ctx->h_small[jj + 1];
     return CDG_OK;
 }
 
@@ -924,6 +936,7 @@ int fgmres_core(Solver &S, std::vector<Workspace> &wss, int depth, const double 
             nvecs++;
             double *vj = ws.V + (size_t)j * len, *zj = ws.Z + (size_t)j * len;
             auto ta = clk::now();
+            CDG_HIP_TRY(hipEventRecord(S.ev[depth][0], ctx->stream));
             e = prec_apply_dev(ctx, S.M, S.A, vj, zj);
             if (e) return e;
             if (need_inner) {
@@ -938,39 +951,41 @@ int fgmres_core(Solver &S, std::vector<Workspace> &wss, int depth, const double 
                 e = kv_add(ctx, len, zj, ws.dz);
                 if (e) return e;
             }
-            Workspace &w2 = ws;
-            CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-            auto tb = clk::now();
-            S.t.precon += secs(ta, tb);
-            // w = A z_j ; nb = ||w||   (:189-191)
-            e = launch_mv(ctx, S.A, w2.Z + (size_t)j * len, w2.w);
+            CDG_HIP_TRY(hipEventRecord(S.ev[depth][1], ctx->stream));
+            // w = A z_j, written into the slot of v_{j+1}   (:189)
+            double *wj = ws.V + (size_t)(j + 1) * len;
+            e = launch_mv(ctx, S.A, zj, wj);
             if (e) return e;
-            CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-            auto tc = clk::now();
-            S.t.mv += secs(tb, tc);
-            e = kv_sumsq(ctx, len, w2.w, ctx->d_small);
-            if (!e) e = fetch_small(ctx, 1);
+            CDG_HIP_TRY(hipEventRecord(S.ev[depth][2], ctx->stream));
+            // orthogonalize; nb = ||w|| before it comes with the same fetch   (:191-210)
+            double nrm2 = 0.0, normb2 = 0.0;
+            e = orthogonalize(S, ws, orth, j + 1, wj, &H(0, j), &nrm2, &normb2);
             if (e) return e;
-            const double norm_before = std::sqrt(ctx->h_small[0]);
-            auto td = clk::now();
-            S.t.norm += secs(tc, td);
-            double nrm2 = 0.0;
-            e = orthogonalize(S, w2, orth, j + 1, &H(0, j), &nrm2);
-            if (e) return e;
+            {   // the fetch synchronized the stream: the events are complete
+                float ms_p = 0.f, ms_m = 0.f;
+                CDG_HIP_TRY(hipEventElapsedTime(&ms_p, S.ev[depth][0], S.ev[depth][1]));
+                CDG_HIP_TRY(hipEventElapsedTime(&ms_m, S.ev[depth][1], S.ev[depth][2]));
+                const double tp = 1e-3 * ms_p, tm = 1e-3 * ms_m;
+                S.t.precon += tp;
+                S.t.mv += tm;
+                S.t.dot += std::max(0.0, secs(ta, clk::now()) - tp - tm);
+            }
+            const double norm_before = std::sqrt(normb2);
             // selective re-orthogonalization, Giraud-Langou L_crit = 1   (:212-223)
             double crit = 0.0;
             for (int i = 0; i <= j; ++i) crit += std::fabs(H(i, j));
             crit /= norm_before;
             if (crit >= 1.0) {
-                e = orthogonalize(S, w2, orth, j + 1, &H(0, j), &nrm2);
+                auto tr = clk::now();
+                e = orthogonalize(S, ws, orth, j + 1, wj, &H(0, j), &nrm2, nullptr);
                 if (e) return e;
                 S.nreorth++;
+                S.t.dot += secs(tr, clk::now());
             }
-            auto te = clk::now();
-            S.t.dot += secs(td, te);
             // h(j+1,j) = ||w|| ; v_{j+1} = w / h(j+1,j)   (:229-231)
+            auto te = clk::now();
             H(j + 1, j) = std::sqrt(nrm2);
-            e = kv_div(ctx, len, w2.w, H(j + 1, j), w2.V + (size_t)(j + 1) * len);
+            e = kv_div_inplace(ctx, len, wj, H(j + 1, j));
             if (e) return e;
             S.t.norm += secs(te, clk::now());
             // previous Givens rotations, new rotation, rotate p   (:235-264)
@@ -1038,6 +1053,15 @@ extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, con
     if (!e) e = upload_in(ctx, bi);
     if (e) return e;
     Solver S{ctx, A, M, len};
+    for (auto &lv : S.ev)
+        for (hipEvent_t &ev : lv)
+            if (!e && hipEventCreate(&ev) != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "hipEventCreate failed");
+    auto destroy_events = [&] {
+        for (auto &lv : S.ev)
+            for (hipEvent_t &ev : lv)
+                if (ev) (void)hipEventDestroy(ev);
+    };
+    if (e) { destroy_events(); return e; }
     std::vector<Workspace> wss(2);   // outer and inner level
     int niter = 0;
     e = fgmres_core(S, wss, 0, bi.dev, xo.dev, prm->nkrylov, prm->tol, prm->rtol, prm->nmax, prm->orthogonalization,
@@ -1052,6 +1076,7 @@ extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, con
     }
     if (!e) e = finish_out(ctx, xo);
     for (Workspace &w : wss) ws_free(ctx, w);
+    destroy_events();
     st->niter = niter;
     st->nrestart = S.nrestart;
     st->nreorth = S.nreorth;

@@ -88,8 +88,10 @@ struct cdg_prec {
     void *W = nullptr;
     int32_t w_ready = 0;               // W holds the current factor's products
     // CDG_APPLY_AUTO: W is formed once the factor has served fold_after
-    // applies (0: never), so long solves get the shorter backward sweep
-    int32_t fold_after = 12, napply = 0;
+    // applies (0: never), or at its first apply when the previous factor
+    // served more than fold_after (consecutive solves of a Newton loop take
+    // similar iteration counts), so long solves get the shorter backward sweep
+    int32_t fold_after = 12, napply = 0, prev_napply = 0;
     int32_t *d_up_all = nullptr;       // 0..nup-1 (the W jobs)
     void *Dinv = nullptr;              // nrows slots
     float *L32 = nullptr, *D32 = nullptr;   // FP32 work for the BF16 variant
@@ -163,6 +165,7 @@ int kv_cgs_update(cdg_ctx *ctx, int64_t len, double *w, const double *V, int64_t
                   const double *d_h, double *d_nrm2);
 int kv_axpy_dev(cdg_ctx *ctx, int64_t len, double *w, const double *v, const double *d_h);
 int kv_div(cdg_ctx *ctx, int64_t len, const double *w, double h, double *out);
+int kv_div_inplace(cdg_ctx *ctx, int64_t len, double *w, double h);
 int kv_sub(cdg_ctx *ctx, int64_t len, const double *b, const double *Ax, double *r);   // r = b - Ax
 int kv_add(cdg_ctx *ctx, int64_t len, double *z, const double *dz);                   // z += dz
 int kv_combine(cdg_ctx *ctx, int64_t len, double *x, const double *x0, const double *Z, int64_t zstride,

@@ -87,6 +87,11 @@ __global__ void k_div(int64_t len, const double *__restrict__ w, double h, doubl
         out[k] = w[k] / h;
 }
 
+__global__ void k_div_inplace(int64_t len, double *__restrict__ w, double h) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
+        w[k] = w[k] / h;
+}
+
 __global__ void k_sub(int64_t len, const double *__restrict__ b, const double *__restrict__ ax, double *__restrict__ r) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
         r[k] = b[k] - ax[k];
@@ -162,6 +167,12 @@ int kv_axpy_dev(cdg_ctx *ctx, int64_t len, double *w, const double *v, const dou
 
 int kv_div(cdg_ctx *ctx, int64_t len, const double *w, double h, double *out) {
     hipLaunchKernelGGL(k_div, egrid(len), dim3(256), 0, ctx->stream, len, w, h, out);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_div_inplace(cdg_ctx *ctx, int64_t len, double *w, double h) {
+    hipLaunchKernelGGL(k_div_inplace, egrid(len), dim3(256), 0, ctx->stream, len, w, h);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }

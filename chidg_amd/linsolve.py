@@ -205,7 +205,8 @@ class _Preconditioner:
         and stage), 'dataflow' (one persistent launch), 'folded' (levels,
         with Dinv folded into the upper blocks at the first apply after each
         factor; FP64/FP32) or 'auto' (the default: 'levels' for the first 12
-        applies after each factor, 'folded' from then on)."""
+        applies after each factor, 'folded' from then on, or from the first
+        apply when the previous factor served more than 12)."""
         code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3}[schedule]
         self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
         self.schedule = schedule

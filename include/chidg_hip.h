@@ -158,10 +158,11 @@ int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
  *                          factor serves many applies; FP64 and FP32 only.
  *   CDG_SCHEDULE_AUTO      (the default) CDG_SCHEDULE_LEVELS for the first 12
  *                          applies after each factor, CDG_SCHEDULE_FOLDED from
- *                          the 13th on (environment CDG_FOLD_AFTER=k sets k;
- *                          0 never folds), so long solves get the shorter
- *                          backward sweep and short ones skip the fold.  BF16
- *                          factors never fold.  */
+ *                          the 13th on, or from the first when the previous
+ *                          factor served more than 12 (environment
+ *                          CDG_FOLD_AFTER=k sets k; 0 never folds), so long
+ *                          solves get the shorter backward sweep and short
+ *                          ones skip the fold.  BF16 factors never fold.  */
 #define CDG_SCHEDULE_LEVELS   0
 #define CDG_SCHEDULE_DATAFLOW 1
 #define CDG_SCHEDULE_FOLDED   2

@@ -165,8 +165,10 @@ def test_folded_schedule_set_after_factor(ctx):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
 def test_auto_schedule_folds_after_12_applies(ctx, precision, tol):
     """The default schedule: the first 12 applies after a factor are the level
-    schedule's bit for bit, the 13th on the folded one's; a new factor starts
-    the count again, and choosing 'levels' afterwards gives the level sweep."""
+    schedule's bit for bit, the 13th on the folded one's; after a factor that
+    served more than 12 applies the next folds at its first apply, after one
+    that served fewer it waits for 12 again; choosing 'levels' gives the level
+    sweep."""
     p, blocks = synthetic_system(4, 3, 2, 2, seed=21)
     A = upload(ctx, p, blocks)
     Ma = precon_ILU0(precision).init(A)
@@ -177,12 +179,18 @@ def test_auto_schedule_folds_after_12_applies(ctx, precision, tol):
     v = po.fill_vector(p.vec_len, 6)
     zl, zf = Ml.apply(A, v), Mf.apply(A, v)
     assert rel(zf, zl) < tol
-    for _ in range(2):
-        for k in range(12):
-            assert np.array_equal(Ma.apply(A, v), zl), k
-        for k in range(3):
-            assert np.array_equal(Ma.apply(A, v), zf), k
-        Ma.update(A)
+    for k in range(12):
+        assert np.array_equal(Ma.apply(A, v), zl), k
+    for k in range(3):
+        assert np.array_equal(Ma.apply(A, v), zf), k
+    Ma.update(A)                    # the last factor served 15 applies
+    for k in range(3):
+        assert np.array_equal(Ma.apply(A, v), zf), k
+    Ma.update(A)                    # this one served 3
+    for k in range(12):
+        assert np.array_equal(Ma.apply(A, v), zl), k
+    assert np.array_equal(Ma.apply(A, v), zf)
+    Ma.update(A)
     Ma.set_schedule("levels")
     for _ in range(14):
         assert np.array_equal(Ma.apply(A, v), zl)
