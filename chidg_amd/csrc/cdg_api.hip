@@ -586,7 +586,9 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     // aligned for all element widths), threads per workgroup and loads in
     // flight.  CDG_APPLY_SLICE_W / CDG_APPLY_THREADS / CDG_APPLY_UNROLL
     // override the defaults (used by the tuning sweep).
-    int sw = 40, target_t = 256, unroll = 4;   // measured best on 16^3 P2 (n = 135)
+    // measured best: 32-column slices at n = 135 (16^3 P2; 40 before the
+    // apply's gather/prefetch load order), 40 for the other block sizes
+    int sw = (A->n > 64 && A->n <= 192) ? 32 : 40, target_t = 256, unroll = 4;
     if (const char *e = std::getenv("CDG_APPLY_SLICE_W")) sw = std::max(8, std::atoi(e));
     if (const char *e = std::getenv("CDG_APPLY_THREADS")) target_t = std::max(64, std::atoi(e));
     if (const char *e = std::getenv("CDG_APPLY_UNROLL")) unroll = std::atoi(e) == 8 ? 8 : 4;

@@ -24,16 +24,50 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 }
 
 // part[i*nb + bx] = sum over chunk bx of V_i . w        (dot, operator_chidg_dot.f90:30-89)
+// Workgroup (bx, g) takes vectors 8g..8g+7: w is read once per 8 products.
+// With an even vector stride every thread reads element pairs (16-byte loads).
 __global__ void __launch_bounds__(RT)
-k_dot_part(int64_t len, const double *__restrict__ w, const double *__restrict__ V, int64_t vstride,
+k_dot_part(int64_t len, const double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
            double *__restrict__ part) {
     __shared__ double sh[RT];
-    const int i = blockIdx.y, nb = gridDim.x;
-    const double *vi = V + (size_t)i * vstride;
-    double acc = 0.0;
-    for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)nb * RT) acc += vi[k] * w[k];
-    const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[(size_t)i * nb + blockIdx.x] = s;
+    const int nb = gridDim.x, i0 = 8 * blockIdx.y;
+    const int ni = min(8, j - i0);
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    const bool pairs = ((vstride | (int64_t)(reinterpret_cast<uintptr_t>(V) >> 3) |
+                         (int64_t)(reinterpret_cast<uintptr_t>(w) >> 3)) & 1) == 0;
+    if (pairs) {
+        const int64_t len2 = len / 2;
+        for (int64_t k2 = blockIdx.x * (int64_t)RT + threadIdx.x; k2 < len2; k2 += (int64_t)nb * RT) {
+            const double2 wk = reinterpret_cast<const double2 *>(w)[k2];
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (u < ni) v[u] = reinterpret_cast<const double2 *>(V + (size_t)(i0 + u) * vstride)[k2];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (u < ni) {
+                    acc[u] += v[u].x * wk.x;
+                    acc[u] += v[u].y * wk.y;
+                }
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0 && (len & 1)) {   // odd tail
+            for (int u = 0; u < ni; ++u) acc[u] += V[(size_t)(i0 + u) * vstride + len - 1] * w[len - 1];
+        }
+    } else {
+        for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)nb * RT) {
+            const double wk = w[k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (u < ni) acc[u] += V[(size_t)(i0 + u) * vstride + k] * wk;
+        }
+    }
+    for (int u = 0; u < ni; ++u) {
+        const double s = block_sum(acc[u], sh);
+        if (threadIdx.x == 0) part[(size_t)(i0 + u) * nb + blockIdx.x] = s;
+        __syncthreads();
+    }
 }
 
 __global__ void __launch_bounds__(RT) k_sumsq_part(int64_t len, const double *__restrict__ w, double *__restrict__ part) {
@@ -55,20 +89,51 @@ __global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, 
 }
 
 // w = w - h_0 V_0 - h_1 V_1 - ... (sequential, as CGS's loop w = w - htmp(i)*v(i)),
-// optional partial sums of the updated w^2 for the norm that follows.
+// optional partial sums of the updated w^2 for the norm that follows.  The
+// loads of 8 basis vectors go out before their products; with an even vector
+// stride every thread updates element pairs (16-byte loads and stores).
 __global__ void __launch_bounds__(RT)
 k_cgs_update(int64_t len, double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
              const double *__restrict__ h, double *__restrict__ part) {
     __shared__ double sh[RT];
     double acc = 0.0;
-    for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)gridDim.x * RT) {
-        double wk = w[k];
-        for (int i0 = 0; i0 < j; i0 += 64) {
-            const int ni = min(64, j - i0);
-            for (int i = 0; i < ni; ++i) wk = wk - h[i0 + i] * V[(size_t)(i0 + i) * vstride + k];
+    const bool pairs = ((vstride | (int64_t)(reinterpret_cast<uintptr_t>(V) >> 3) |
+                         (int64_t)(reinterpret_cast<uintptr_t>(w) >> 3)) & 1) == 0;
+    if (pairs) {
+        const int64_t len2 = len / 2;
+        for (int64_t k2 = blockIdx.x * (int64_t)RT + threadIdx.x; k2 < len2; k2 += (int64_t)gridDim.x * RT) {
+            double2 wk = reinterpret_cast<const double2 *>(w)[k2];
+            for (int i0 = 0; i0 < j; i0 += 8) {
+                const int ni = min(8, j - i0);
+                double2 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (u < ni) v[u] = reinterpret_cast<const double2 *>(V + (size_t)(i0 + u) * vstride)[k2];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (u < ni) {
+                        wk.x = wk.x - h[i0 + u] * v[u].x;
+                        wk.y = wk.y - h[i0 + u] * v[u].y;
+                    }
+            }
+            reinterpret_cast<double2 *>(w)[k2] = wk;
+            acc += wk.x * wk.x;
+            acc += wk.y * wk.y;
         }
-        w[k] = wk;
-        acc += wk * wk;
+        if (blockIdx.x == 0 && threadIdx.x == 0 && (len & 1)) {   // odd tail
+            const int64_t k = len - 1;
+            double wk = w[k];
+            for (int i = 0; i < j; ++i) wk = wk - h[i] * V[(size_t)i * vstride + k];
+            w[k] = wk;
+            acc += wk * wk;
+        }
+    } else {
+        for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)gridDim.x * RT) {
+            double wk = w[k];
+            for (int i = 0; i < j; ++i) wk = wk - h[i] * V[(size_t)i * vstride + k];
+            w[k] = wk;
+            acc += wk * wk;
+        }
     }
     if (part) {
         const double s = block_sum(acc, sh);
@@ -131,7 +196,7 @@ int kv_multidot(cdg_ctx *ctx, int64_t len, const double *w, const double *V, int
     const int nb = nparts(len);
     int e = ctx_ensure_part(ctx, (size_t)nb * j);
     if (e) return e;
-    hipLaunchKernelGGL(k_dot_part, dim3(nb, j), dim3(RT), 0, ctx->stream, len, w, V, vstride, ctx->d_part);
+    hipLaunchKernelGGL(k_dot_part, dim3(nb, (j + 7) / 8), dim3(RT), 0, ctx->stream, len, w, V, vstride, j, ctx->d_part);
     hipLaunchKernelGGL(k_reduce, dim3(j), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_out);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
