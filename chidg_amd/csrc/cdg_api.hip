@@ -267,6 +267,24 @@ std::vector<int32_t> order_by_level(const std::vector<int32_t> &lev, int nlev, s
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+int ctx_wait(cdg_ctx *ctx) {
+    if (!ctx->spin || !ctx->done_ev) {
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return CDG_OK;
+    }
+    CDG_HIP_TRY(hipEventRecord(ctx->done_ev, ctx->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t q;
+    while ((q = hipEventQuery(ctx->done_ev)) == hipErrorNotReady) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(5)) {
+            q = hipEventSynchronize(ctx->done_ev);
+            break;
+        }
+    }
+    if (q != hipSuccess) return cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(q));
+    return CDG_OK;
+}
+
 extern "C" int cdg_ctx_create(int32_t device, cdg_ctx **out) {
     if (!out) return CDG_ERR_ARG;
     *out = nullptr;
@@ -281,6 +299,12 @@ extern "C" int cdg_ctx_create(int32_t device, cdg_ctx **out) {
     ctx->device = device;
     ctx->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return CDG_ERR_HIP; }
+    if (hipEventCreateWithFlags(&ctx->done_ev, hipEventDisableTiming) != hipSuccess) {
+        ctx->done_ev = nullptr;
+        cdg_ctx_destroy(ctx);
+        return CDG_ERR_HIP;
+    }
+    if (const char *sp = std::getenv("CDG_SPIN")) ctx->spin = std::atoi(sp) != 0;
     if (ctx_ensure_small(ctx, 4096) || ctx_ensure_part(ctx, 1 << 16)) { cdg_ctx_destroy(ctx); return CDG_ERR_OOM; }
     *out = ctx;
     return CDG_OK;
@@ -297,6 +321,7 @@ extern "C" int cdg_ctx_destroy(cdg_ctx *ctx) {
     for (auto &kv : ctx->alloc_size) (void)hipFree(kv.first);   // blocks of objects never destroyed
     ctx->alloc_size.clear();
     ctx_flush_cache(ctx);
+    if (ctx->done_ev) (void)hipEventDestroy(ctx->done_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CDG_OK;
@@ -849,8 +874,7 @@ struct Solver {
 // fetch n device scalars from ctx->d_small into ctx->h_small
 int fetch_small(cdg_ctx *ctx, size_t n) {
     CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return CDG_OK;
+    return ctx_wait(ctx);
 }
 
 // One orthogonalization pass of w = V_jj (the slot of the next basis vector)
@@ -879,7 +903,8 @@ int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *w, double 
     }
     if (e) return e;
     CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_h, sizeof(double) * (jj + 2), hipMemcpyDeviceToHost, ctx->stream));
-    CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    e = ctx_wait(ctx);
+    if (e) return e;
     for (int i = 0; i < jj; ++i) hcol[i] += ctx->h_small[i];
     if (normb2) *normb2 = ctx->h_small[jj];
     *nrm2 = ctx->h_small[jj + 1];

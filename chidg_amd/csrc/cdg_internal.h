@@ -47,7 +47,15 @@ struct cdg_ctx {
     // staging for host pointers passed to the ABI
     double *d_stage[3] = {nullptr, nullptr, nullptr};
     size_t stage_cap[3] = {0, 0, 0};
+    // completion event of ctx_wait (the solver's per-step waits poll it)
+    hipEvent_t done_ev = nullptr;
+    bool spin = true;
 };
+
+// Waits until the context's stream has finished its work: polls a completion
+// event (a few microseconds of latency instead of a blocking wait's tens), and
+// falls back to a blocking wait after 5 ms.  CDG_SPIN=0 always blocks.
+int ctx_wait(cdg_ctx *ctx);
 
 // Block-row streaming launch shape for one block size / element width.
 struct StreamShape {
