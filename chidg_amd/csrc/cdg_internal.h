@@ -80,6 +80,7 @@ struct cdg_mat {
     std::vector<int32_t> flevel, blevel;
     int32_t nflev = 0, nblev = 0;
     StreamShape shape64;
+    int32_t mv_unroll = 8;             // 16-byte loads in flight per thread (matvec)
 };
 
 struct cdg_prec {

@@ -120,6 +120,7 @@ __device__ __forceinline__ Acc row_sum(const Acc *red, int r, int n, int parts) 
 // ============================================================================
 // K1: y_i = sum_b A_b x_col(b)   (chidg_mv, operator_chidg_mv.f90:200-374)
 // ============================================================================
+template <int U>
 __global__ void __launch_bounds__(kMaxThreads)
 k_mv(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ row_ptr,
      const int32_t *__restrict__ col, const double *__restrict__ x, double *__restrict__ y, int n,
@@ -142,7 +143,7 @@ k_mv(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ 
         }
         __syncthreads();
         for (int k = 0; k < nb; ++k)
-            stream_block<double, double>(blocks + (size_t)(cb + k) * ld, xs + k * n, nn, w, acc);
+            stream_block<double, double, U>(blocks + (size_t)(cb + k) * ld, xs + k * n, nn, w, acc);
     }
     park<double, V>(red, tid, acc);
     __syncthreads();
@@ -389,8 +390,15 @@ int launch_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y) {
     const int V = 2;
     int chunk = std::max(1, std::min<int>(A->max_row_blocks, (32 * 1024) / (8 * A->n)));
     size_t lds = sizeof(double) * ((size_t)chunk * A->n + (size_t)V * s.T);
-    hipLaunchKernelGGL(k_mv, dim3(A->nrows), dim3(s.T), lds, ctx->stream, A->blocks, A->ld, A->d_row_ptr,
-                       A->d_col, x, y, A->n, s.dc, chunk);
+    if (A->mv_unroll == 16)
+        hipLaunchKernelGGL(k_mv<16>, dim3(A->nrows), dim3(s.T), lds, ctx->stream, A->blocks, A->ld, A->d_row_ptr,
+                           A->d_col, x, y, A->n, s.dc, chunk);
+    else if (A->mv_unroll == 8)
+        hipLaunchKernelGGL(k_mv<8>, dim3(A->nrows), dim3(s.T), lds, ctx->stream, A->blocks, A->ld, A->d_row_ptr,
+                           A->d_col, x, y, A->n, s.dc, chunk);
+    else
+        hipLaunchKernelGGL(k_mv<4>, dim3(A->nrows), dim3(s.T), lds, ctx->stream, A->blocks, A->ld, A->d_row_ptr,
+                           A->d_col, x, y, A->n, s.dc, chunk);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
