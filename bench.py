@@ -121,12 +121,22 @@ def measure(args, p, A, b, steps, warmup, kernels=True):
     ctx = A.ctx
     bnorm = float(torch.linalg.norm(b))
     out = {}
+    if args.tune:
+        A.tune(reps=5)
     for prec in ("fp32", "fp64"):
         M = precon_ILU0(prec, schedule=args.schedule).init(A)
         ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False, orthogonalization="CGS")
         x = torch.zeros_like(b)
         for _ in range(warmup):
             ls.solve(A, x, b, M)
+        tune = {}
+        if args.tune:
+            # launch-shape autotuner on the live factor (setup, outside the timed region)
+            if not warmup:
+                M.update(A)
+            t0 = time.perf_counter()
+            M.tune(A, reps=3)
+            tune = dict(apply_shape=M.launch_shape(), tune_s=time.perf_counter() - t0)
         torch.cuda.synchronize()
         lat = []
         t0 = time.perf_counter()
@@ -142,7 +152,7 @@ def measure(args, p, A, b, steps, warmup, kernels=True):
                  p50_ms=1e3 * statistics.median(lat), p99_ms=1e3 * float(np.percentile(lat, 99)),
                  niter=st.niter, residual_rel=st.residual_norm / bnorm, true_residual_rel=true_res,
                  t_factor_ms=1e3 * st.t_factor, t_precon_ms=1e3 * st.t_precon, t_mv_ms=1e3 * st.t_mv,
-                 t_dot_ms=1e3 * st.t_dot, t_norm_ms=1e3 * st.t_norm, wall_s=wall, x=x)
+                 t_dot_ms=1e3 * st.t_dot, t_norm_ms=1e3 * st.t_norm, wall_s=wall, x=x, **tune)
         if kernels:
             nbytes = ilu_apply_bytes(p, 4 if prec == "fp32" else 8)
             v = b / bnorm
@@ -168,6 +178,7 @@ def measure(args, p, A, b, steps, warmup, kernels=True):
         M.tear_down()
     x64, x32 = out["fp64"].pop("x"), out["fp32"].pop("x")
     out["fp32"]["x_rel_vs_fp64"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    out["mv_shape"] = A.launch_shape()
     return out, x64
 
 
@@ -213,12 +224,16 @@ def p3_run(args):
     A = chidg_matrix(p, ctx)
     A.fill_synthetic(SEED_A, 1.0 / np.sqrt(n), 0.1 / np.sqrt(n))
     b = A.synthetic_vector(SEED_B, 1.0, like=torch.empty(0, dtype=torch.float64, device="cuda:0"))
+    if args.tune:
+        A.tune(reps=3)
     out, xs = {}, {}
     for prec in ("fp64", "fp32", "bf16"):
         M = precon_ILU0(prec).init(A)
         ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False)
         x = torch.zeros_like(b)
         ls.solve(A, x, b, M)             # warm-up (allocations)
+        if args.tune:
+            M.tune(A, reps=2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ls.solve(A, x, b, M)
@@ -234,6 +249,30 @@ def p3_run(args):
         out[prec]["x_rel_vs_fp64_ilu"] = float(torch.linalg.norm(xs[prec] - xs["fp64"]) / torch.linalg.norm(xs["fp64"]))
     A.release()
     return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in out.items()}
+
+
+def pretune_small(args):
+    """Launch shapes for the request loop's block sizes (P1-P3, FP32 ILU),
+    tuned on 8^3 synthetic systems; the loop's own contexts start from them."""
+    from chidg_amd import pattern as pt
+    from chidg_amd.linsolve import Context, chidg_matrix, precon_ILU0
+    ctx = Context(0)
+    shapes = {}
+    for order in (1, 2, 3):
+        n = pt.block_size(order, NFIELDS)
+        p = pt.hex_box(8, 8, 8, n)
+        A = chidg_matrix(p, ctx)
+        A.fill_synthetic(SEED_A, 1.0 / np.sqrt(n), 0.1 / np.sqrt(n))
+        A.tune(reps=3)
+        M = precon_ILU0("fp32").init(A)
+        M.update(A)
+        M.tune(A, reps=3)
+        shapes[f"P{order}"] = dict(apply=M.launch_shape(), mv=A.launch_shape())
+        M.tear_down()
+        A.release()
+    ctx.close()
+    log(f"pre-tuned launch shapes: {shapes}")
+    return shapes
 
 
 def batch_run(args):
@@ -327,6 +366,8 @@ def main():
                          "(16^3 P3 mixed precision), batch (64 uneven systems), serve (Poisson request "
                          "loop); empty for none")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests per arrival rate")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="skip the launch-shape autotuner (run before each timed section, outside the timed region)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -353,6 +394,8 @@ def main():
         sections["synthetic_16c_p2"] = _guarded(synthetic_run, args, "synthetic")
     if "p3" in extras:
         sections["p3_16c_mixed_precision"] = _guarded(p3_run, args, "p3")
+    if args.tune and ("batch" in extras or "serve" in extras):
+        sections["pretuned_shapes"] = _guarded(pretune_small, args, "pretune")
     if "batch" in extras:
         sections["batch_64_uneven"] = _guarded(batch_run, args, "batch")
     if "serve" in extras:
@@ -384,6 +427,7 @@ def main():
                    "nrows": p.nrows, "block_n": p.n, "nnzb": p.nnzb, "seed_state": SEED_Q, "seed_b": SEED_B},
         "fp64": {k: sig(v) for k, v in f64.items()},
         "fp32_ilu": {k: sig(v) for k, v in f32.items()},
+        "mv_shape": g.get("mv_shape"),
         "speedup_fp32_vs_fp64": round(f32["solves_per_s"] / f64["solves_per_s"], 3),
         "ilu_apply_gbs_fp64": round(ilu_apply_bytes(p, 8) / (f64["apply_folded_ms"] / 1e3) / 1e9, 1),
         "ilu_apply_gbs_fp32": round(gbs, 1),

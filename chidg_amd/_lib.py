@@ -24,7 +24,7 @@ EXPORTS = [
     "cdg_mat_create", "cdg_mat_destroy", "cdg_mat_upload", "cdg_mat_download",
     "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
     "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
-    "cdg_prec_set_schedule",
+    "cdg_prec_set_schedule", "cdg_prec_tune", "cdg_mat_tune", "cdg_prec_launch_shape", "cdg_mat_launch_shape",
     "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
     "cdg_server_last_error", "cdg_server_destroy",
     "cdg_fgmres_solve", "cdg_fgmres_default_params",
@@ -112,6 +112,10 @@ def load() -> C.CDLL:
         "cdg_fgmres_solve": ([vp, vp, vp, dp, dp, C.POINTER(FgmresParams), C.POINTER(FgmresStats)], C.c_int),
         "cdg_fgmres_default_params": ([C.POINTER(FgmresParams)], None),
         "cdg_prec_set_schedule": ([vp, vp, C.c_int32], C.c_int),
+        "cdg_prec_tune": ([vp, vp, vp, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "cdg_mat_tune": ([vp, vp, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "cdg_prec_launch_shape": ([vp, ip], C.c_int),
+        "cdg_mat_launch_shape": ([vp, ip], C.c_int),
         "cdg_server_create": ([C.c_int32, C.c_int32, C.c_int64, C.POINTER(FgmresParams), pp], C.c_int),
         "cdg_server_submit": ([vp, C.POINTER(Request), dp, C.POINTER(C.c_uint64)], C.c_int),
         "cdg_server_wait": ([vp, C.c_uint64, C.POINTER(Result)], C.c_int),

@@ -3,6 +3,7 @@
 #include "cdg_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -11,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 // ---------------------------------------------------------------------------
 // errors and memory
@@ -176,6 +178,64 @@ int finish_out(cdg_ctx *ctx, const VecArg &a) {
 }  // namespace
 
 int64_t slot_elems(int n) { return ((int64_t)n * n + 63) / 64 * 64; }
+
+// ---- launch shapes and their autotuner ----------------------------------------
+// The apply's shape: column-slice width, threads per workgroup (target of
+// stream_shape) and 16-byte loads in flight; the matvec's: threads and loads
+// in flight.  Defaults are the measured best at n = 135; cdg_prec_tune /
+// cdg_mat_tune time candidates on a live factor / matrix and keep the fastest,
+// and remember it per (block size, precision) for objects created afterwards.
+namespace {
+struct ApplyShape {
+    int sw, target, unroll;
+};
+struct MvShape {
+    int target, unroll;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, ApplyShape> g_apply_tuned;   // (n, precision)
+std::map<int, MvShape> g_mv_tuned;                          // n
+
+ApplyShape apply_default(int n) { return {(n > 64 && n <= 192) ? 32 : 40, 256, 4}; }
+MvShape mv_default() { return {540, 8}; }
+
+bool apply_tuned(int n, int precision, ApplyShape *out) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_apply_tuned.find({n, precision});
+    if (it == g_apply_tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+bool mv_tuned(int n, MvShape *out) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_mv_tuned.find(n);
+    if (it == g_mv_tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+// Column slices are multiples of 8 columns, so every slice starts 16-byte
+// aligned for all element widths.  False if the shape does not fit (LDS,
+// threads); M is then unchanged.
+bool apply_shape_set(cdg_prec *M, int precision, const ApplyShape &a) {
+    const int n = M->n;
+    int slices = std::max(1, (n + a.sw - 1) / a.sw);
+    const int slice_w = ((n + slices - 1) / slices + 7) / 8 * 8;
+    slices = (n + slice_w - 1) / slice_w;
+    const StreamShape sh[3] = {stream_shape(n, 2, a.target), stream_shape(n, 4, a.target), stream_shape(n, 8, a.target)};
+    const int V = precision == CDG_FP64 ? 2 : (precision == CDG_FP32 ? 4 : 8);
+    const size_t accsz = precision == CDG_FP64 ? 8 : 4;
+    const size_t lds = accsz * ((size_t)M->maxdep * slice_w + (size_t)V * sh[precision].T);
+    if (sh[precision].T == 0 || lds > 64 * 1024) return false;
+    M->slices = slices;
+    M->slice_w = slice_w;
+    M->unroll = a.unroll == 8 ? 8 : 4;
+    for (int i = 0; i < 3; ++i) M->shape[i] = sh[i];
+    M->part_len = (int64_t)M->nrows * slices * n;
+    return true;
+}
+}  // namespace
 
 namespace {
 
@@ -369,16 +429,16 @@ extern "C" int cdg_mat_create(cdg_ctx *ctx, int32_t nrows, int32_t block_n, cons
     A->col.assign(col, col + nnzb);
     A->kind.assign(kind, kind + nnzb);
     analyze(A);
-    {   // matvec launch shape (CDG_MV_THREADS / CDG_MV_UNROLL override the defaults):
-        // ~540 threads with 8 loads in flight each, the measured best at n = 135
-        // (0.611 ms = 6.85 TB/s, against 0.652 ms with ~1024 threads and 4)
-        int target = 540;
-        if (const char *t = std::getenv("CDG_MV_THREADS")) target = std::max(64, std::atoi(t));
-        A->shape64 = stream_shape(block_n, 2, target);
-        if (const char *u = std::getenv("CDG_MV_UNROLL")) {
-            const int uu = std::atoi(u);
-            A->mv_unroll = uu == 16 ? 16 : uu == 8 ? 8 : 4;
-        }
+    {   // matvec launch shape: tuned for this block size if cdg_mat_tune has run,
+        // else ~540 threads with 8 loads in flight each, the measured best at
+        // n = 135 (0.611 ms = 6.85 TB/s, against 0.652 ms with ~1024 threads and
+        // 4); CDG_MV_THREADS / CDG_MV_UNROLL override either
+        MvShape ms = mv_default();
+        (void)mv_tuned(block_n, &ms);
+        if (const char *t = std::getenv("CDG_MV_THREADS")) ms.target = std::max(64, std::atoi(t));
+        if (const char *u = std::getenv("CDG_MV_UNROLL")) ms.unroll = std::atoi(u);
+        A->shape64 = stream_shape(block_n, 2, ms.target);
+        A->mv_unroll = ms.unroll == 16 ? 16 : ms.unroll == 8 ? 8 : 4;
     }
     int e = dev_alloc(ctx, (void **)&A->blocks, sizeof(double) * (size_t)std::max(1, nnzb) * A->ld);
     if (!e) e = upload_array(ctx, &A->d_row_ptr, A->row_ptr, nullptr);
@@ -506,6 +566,7 @@ static int df_setup(cdg_ctx *ctx, cdg_prec *M) {
             if (m.p == M->d_part) { dev_free(ctx, m.p, m.bytes); m.p = np; m.bytes = sizeof(double) * (size_t)(need + 2); }
         M->d_part = np;
         M->part_len = need / 3;
+        M->part_cap = need + 2;
     }
     e = upload_array(ctx, &M->d_tasks, tasks, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_cnt, std::vector<int32_t>((size_t)3 * M->nrows, 0), &M->owned);
@@ -617,30 +678,17 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
         M->maxdep = std::max(M->maxdep, A->lo_ptr[i + 1] - A->lo_ptr[i]);
         M->maxdep = std::max(M->maxdep, A->up_ptr[i + 1] - A->up_ptr[i]);
     }
-    // Column slices (multiples of 8 columns, so every slice starts 16-byte
-    // aligned for all element widths), threads per workgroup and loads in
-    // flight.  CDG_APPLY_SLICE_W / CDG_APPLY_THREADS / CDG_APPLY_UNROLL
-    // override the defaults (used by the tuning sweep).
-    // measured best: 32-column slices at n = 135 (16^3 P2; 40 before the
-    // apply's gather/prefetch load order), 40 for the other block sizes
-    int sw = (A->n > 64 && A->n <= 192) ? 32 : 40, target_t = 256, unroll = 4;
-    if (const char *e = std::getenv("CDG_APPLY_SLICE_W")) sw = std::max(8, std::atoi(e));
-    if (const char *e = std::getenv("CDG_APPLY_THREADS")) target_t = std::max(64, std::atoi(e));
-    if (const char *e = std::getenv("CDG_APPLY_UNROLL")) unroll = std::atoi(e) == 8 ? 8 : 4;
-    M->unroll = unroll;
-    M->slices = std::max(1, (A->n + sw - 1) / sw);
-    M->slice_w = ((A->n + M->slices - 1) / M->slices + 7) / 8 * 8;
-    M->slices = (A->n + M->slice_w - 1) / M->slice_w;
-    M->shape[CDG_FP64] = stream_shape(A->n, 2, target_t);
-    M->shape[CDG_FP32] = stream_shape(A->n, 4, target_t);
-    M->shape[CDG_BF16] = stream_shape(A->n, 8, target_t);
-    const int V = precision == CDG_FP64 ? 2 : (precision == CDG_FP32 ? 4 : 8);
-    const size_t accsz = precision == CDG_FP64 ? 8 : 4;
-    const size_t lds = accsz * ((size_t)M->maxdep * M->slice_w + (size_t)V * M->shape[precision].T);
-    M->part_len = (int64_t)M->nrows * M->slices * M->n;
-    if (e || lds > 64 * 1024 || M->shape[precision].T == 0) {
+    // Apply launch shape: the tuned one for this block size and precision if
+    // cdg_prec_tune has run, else the default; CDG_APPLY_SLICE_W /
+    // CDG_APPLY_THREADS / CDG_APPLY_UNROLL override either.
+    ApplyShape ash = apply_default(A->n);
+    (void)apply_tuned(A->n, precision, &ash);
+    if (const char *ev = std::getenv("CDG_APPLY_SLICE_W")) ash.sw = std::max(8, std::atoi(ev));
+    if (const char *ev = std::getenv("CDG_APPLY_THREADS")) ash.target = std::max(64, std::atoi(ev));
+    if (const char *ev = std::getenv("CDG_APPLY_UNROLL")) ash.unroll = std::atoi(ev) == 8 ? 8 : 4;
+    if (e || !apply_shape_set(M, precision, ash)) {
         cdg_prec_destroy(ctx, M);
-        return e ? e : cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply needs %zu bytes of LDS", lds);
+        return e ? e : cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "ILU apply launch shape does not fit (block size %d)", A->n);
     }
     // apply schedule (CDG_APPLY_MODE=levels|dataflow|folded|auto overrides the default)
     M->apply_mode = CDG_APPLY_AUTO;
@@ -656,6 +704,7 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     // block prefetch from the start of a partial array, unused)
     const int64_t part_doubles = 3 * M->part_len + 2;
     if (!e) e = alloc((void **)&M->d_part, sizeof(double) * (size_t)part_doubles);
+    M->part_cap = part_doubles;
     if (!e && M->apply_mode == CDG_APPLY_DATAFLOW) e = df_setup(ctx, M);
     if (!e && M->apply_mode == CDG_APPLY_FOLDED) e = alloc(&M->W, es * (size_t)M->nup * M->ld);
     if (const char *f = std::getenv("CDG_FOLD_AFTER")) M->fold_after = std::max(0, std::atoi(f));
@@ -693,6 +742,171 @@ extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule
     M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW
                     : schedule == CDG_SCHEDULE_FOLDED ? CDG_APPLY_FOLDED
                     : schedule == CDG_SCHEDULE_AUTO ? CDG_APPLY_AUTO : CDG_APPLY_LEVELS;
+    return CDG_OK;
+}
+
+// ---- launch-shape autotuner (no reference counterpart) ---------------------------
+namespace {
+// Times fn() `reps` times on the context's stream (one untimed call first).
+template <typename F>
+int time_on_stream(cdg_ctx *ctx, hipEvent_t e0, hipEvent_t e1, int reps, F fn, double *ms_out) {
+    int e = fn();
+    if (e) return e;
+    CDG_HIP_TRY(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps && !e; ++r) e = fn();
+    if (e) return e;
+    CDG_HIP_TRY(hipEventRecord(e1, ctx->stream));
+    CDG_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CDG_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = (double)ms / reps;
+    return CDG_OK;
+}
+
+// Grows the apply's partial-sum storage to the current part_len.
+int prec_part_fit(cdg_ctx *ctx, cdg_prec *M) {
+    const int64_t need = 3 * M->part_len + 2;
+    if (need <= M->part_cap) return CDG_OK;
+    double *np = nullptr;
+    int e = dev_alloc(ctx, (void **)&np, sizeof(double) * (size_t)need);
+    if (e) return e;
+    for (DevMem &m : M->owned)
+        if (m.p == M->d_part) { dev_free(ctx, m.p, m.bytes); m.p = np; m.bytes = sizeof(double) * (size_t)need; }
+    M->d_part = np;
+    M->part_cap = need;
+    return CDG_OK;
+}
+
+struct Scratch {   // tuning vectors and events, released on every path out
+    cdg_ctx *ctx;
+    double *v = nullptr, *z = nullptr;
+    size_t bytes = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    explicit Scratch(cdg_ctx *c) : ctx(c) {}
+    int init(int64_t len) {
+        bytes = sizeof(double) * (size_t)std::max<int64_t>(len, 1);
+        int e = dev_alloc(ctx, (void **)&v, bytes);
+        if (!e) e = dev_alloc(ctx, (void **)&z, bytes);
+        if (!e) e = kv_zero(ctx, std::max<int64_t>(len, 1), v);
+        if (!e && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+            e = cdg_fail(ctx, CDG_ERR_HIP, "hipEventCreate failed");
+        return e;
+    }
+    ~Scratch() {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (v) dev_free(ctx, v, bytes);
+        if (z) dev_free(ctx, z, bytes);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+}  // namespace
+
+extern "C" int cdg_prec_tune(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t reps, double *best_ms) {
+    if (!ctx || !M || !A) return CDG_ERR_ARG;
+    if (best_ms) *best_ms = 0.0;
+    if (M->type != CDG_PREC_ILU0 || M->nrows == 0) return CDG_OK;
+    if (A->nrows != M->nrows || A->n != M->n) return cdg_fail(ctx, CDG_ERR_ARG, "matrix does not match preconditioner");
+    if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "cdg_prec_tune before cdg_prec_update");
+    if (M->apply_mode == CDG_APPLY_DATAFLOW)
+        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "cdg_prec_tune: the level schedules only");
+    reps = std::max(1, std::min<int32_t>(reps, 1000));
+    const int n = M->n, prec = M->precision;
+    Scratch sc(ctx);
+    int e = sc.init((int64_t)M->nrows * n);
+    if (e) return e;
+    const int sws[] = {16, 24, 32, 40, 48, 64, 96, 128, n};
+    const int tgs[] = {128, 256, 512};
+    const int uns[] = {4, 8};
+    std::vector<std::array<int, 4>> seen;
+    double best = std::numeric_limits<double>::max();
+    ApplyShape bestc = apply_default(n);
+    for (int sw : sws)
+        for (int tg : tgs)
+            for (int un : uns) {
+                const ApplyShape c{sw, tg, un};
+                if (!apply_shape_set(M, prec, c)) continue;
+                const std::array<int, 4> key{M->slices, M->slice_w, M->shape[prec].T, M->unroll};
+                if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                seen.push_back(key);
+                e = prec_part_fit(ctx, M);
+                double ms = 0.0;
+                if (!e) e = time_on_stream(ctx, sc.e0, sc.e1, reps,
+                                           [&] { return launch_ilu_apply(ctx, M, A, sc.v, sc.z); }, &ms);
+                if (e) {
+                    (void)apply_shape_set(M, prec, bestc);
+                    return e;
+                }
+                if (ms < best) { best = ms; bestc = c; }
+            }
+    if (!apply_shape_set(M, prec, bestc)) return cdg_fail(ctx, CDG_ERR_STATE, "tuned shape no longer fits");
+    e = prec_part_fit(ctx, M);
+    if (e) return e;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_apply_tuned[{n, prec}] = bestc;
+    }
+    if (best_ms) *best_ms = best;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_tune(cdg_ctx *ctx, cdg_mat *A, int32_t reps, double *best_ms) {
+    if (!ctx || !A) return CDG_ERR_ARG;
+    if (best_ms) *best_ms = 0.0;
+    if (A->nrows == 0) return CDG_OK;
+    reps = std::max(1, std::min<int32_t>(reps, 1000));
+    Scratch sc(ctx);
+    int e = sc.init((int64_t)A->nrows * A->n);
+    if (e) return e;
+    const int tgs[] = {128, 256, 384, 540, 768, 1024};
+    const int uns[] = {4, 8, 16};
+    std::vector<std::array<int, 2>> seen;
+    double best = std::numeric_limits<double>::max();
+    const StreamShape keep = A->shape64;
+    const int32_t keep_u = A->mv_unroll;
+    MvShape bestc{0, keep_u};
+    StreamShape bests = keep;
+    for (int tg : tgs)
+        for (int un : uns) {
+            const StreamShape sh = stream_shape(A->n, 2, tg);
+            if (sh.T == 0) continue;
+            const std::array<int, 2> key{sh.T, un};
+            if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+            seen.push_back(key);
+            A->shape64 = sh;
+            A->mv_unroll = un;
+            double ms = 0.0;
+            e = time_on_stream(ctx, sc.e0, sc.e1, reps, [&] { return launch_mv(ctx, A, sc.v, sc.z); }, &ms);
+            if (e) {
+                A->shape64 = keep;
+                A->mv_unroll = keep_u;
+                return e;
+            }
+            if (ms < best) { best = ms; bestc = {tg, un}; bests = sh; }
+        }
+    A->shape64 = bests;
+    A->mv_unroll = bestc.unroll;
+    if (bestc.target > 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_mv_tuned[A->n] = bestc;
+    }
+    if (best_ms) *best_ms = best;
+    return CDG_OK;
+}
+
+extern "C" int cdg_prec_launch_shape(const cdg_prec *M, int32_t *out) {
+    if (!M || !out) return CDG_ERR_ARG;
+    out[0] = M->slices;
+    out[1] = M->slice_w;
+    out[2] = M->shape[M->precision].T;
+    out[3] = M->unroll;
+    return CDG_OK;
+}
+
+extern "C" int cdg_mat_launch_shape(const cdg_mat *A, int32_t *out) {
+    if (!A || !out) return CDG_ERR_ARG;
+    out[0] = A->shape64.T;
+    out[1] = A->mv_unroll;
     return CDG_OK;
 }
 

@@ -114,6 +114,7 @@ struct cdg_prec {
     int32_t *d_err = nullptr;          // first singular row (factor)
     double *d_part = nullptr;          // apply partial sums: 3 arrays of part_len doubles
     int64_t part_len = 0;              // nrows * slices * n
+    int64_t part_cap = 0;              // doubles allocated at d_part
     // forward levels: rows grouped by level (factor and forward sweep); lower
     // blocks of those rows grouped the same way (factor phase 1)
     std::vector<int32_t> lev_lo_ptr, lev_row_ptr;

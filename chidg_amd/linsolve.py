@@ -153,6 +153,19 @@ class chidg_matrix:
         self.ctx._check(self.ctx.lib.cdg_mat_stamp(self.h, C.byref(v)), "cdg_mat_stamp")
         return v.value
 
+    def tune(self, reps: int = 5) -> float:
+        """Launch-shape autotuner of the matvec: keeps the fastest shape for
+        this matrix and later ones of the same block size.  Returns its time
+        per product in seconds."""
+        ms = C.c_double(0.0)
+        self.ctx._check(self.ctx.lib.cdg_mat_tune(self.ctx.h, self.h, int(reps), C.byref(ms)), "cdg_mat_tune")
+        return ms.value * 1e-3
+
+    def launch_shape(self) -> dict:
+        out = (C.c_int32 * 2)()
+        self.ctx._check(self.ctx.lib.cdg_mat_launch_shape(self.h, out), "cdg_mat_launch_shape")
+        return dict(threads=out[0], loads_in_flight=out[1])
+
     def release(self):
         if getattr(self, "h", None):
             self.ctx.lib.cdg_mat_destroy(self.ctx.h, self.h)
@@ -210,6 +223,20 @@ class _Preconditioner:
         code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3}[schedule]
         self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
         self.schedule = schedule
+
+    def tune(self, A: chidg_matrix, reps: int = 5) -> float:
+        """Launch-shape autotuner of the apply on the current factor (after
+        update): keeps the fastest shape for this preconditioner and for later
+        ones of the same block size and precision.  Returns its time per
+        apply in seconds."""
+        ms = C.c_double(0.0)
+        self.ctx._check(self.ctx.lib.cdg_prec_tune(self.ctx.h, self.h, A.h, int(reps), C.byref(ms)), "cdg_prec_tune")
+        return ms.value * 1e-3
+
+    def launch_shape(self) -> dict:
+        out = (C.c_int32 * 4)()
+        self.ctx._check(self.ctx.lib.cdg_prec_launch_shape(self.h, out), "cdg_prec_launch_shape")
+        return dict(slices=out[0], slice_width=out[1], threads=out[2], loads_in_flight=out[3])
 
     def update(self, A: chidg_matrix, b=None):
         if not self.initialized:

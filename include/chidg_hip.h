@@ -168,6 +168,21 @@ int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
 #define CDG_SCHEDULE_FOLDED   2
 #define CDG_SCHEDULE_AUTO     3
 int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule);
+/* Launch-shape autotuner (no reference counterpart).  cdg_prec_tune times the
+ * apply of M's current factor (level schedules; after cdg_prec_update) under
+ * candidate shapes -- column-slice width, threads per workgroup, 16-byte loads
+ * in flight -- `reps` calls each, keeps the fastest for M and remembers it for
+ * preconditioners of the same block size and precision created later in the
+ * process.  cdg_mat_tune does the same for the matvec (threads, loads in
+ * flight; remembered per block size).  *best_ms (may be NULL) receives the
+ * chosen shape's time per call.  Results do not depend on the shape beyond
+ * rounding (the partial sums regroup). */
+int cdg_prec_tune(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t reps, double *best_ms);
+int cdg_mat_tune(cdg_ctx *ctx, cdg_mat *A, int32_t reps, double *best_ms);
+/* The current shapes: out[4] = {column slices, slice width, threads, loads in
+ * flight} for the apply; out[2] = {threads, loads in flight} for the matvec. */
+int cdg_prec_launch_shape(const cdg_prec *M, int32_t *out);
+int cdg_mat_launch_shape(const cdg_mat *A, int32_t *out);
 /* Factor as FP64 for inspection: L (nnzb*n*n, lower kind-0 slots filled,
  * others zero) and Dinv (nrows*n*n), the LD matrix of precon_ILU0_t. */
 int cdg_prec_download(cdg_ctx *ctx, const cdg_prec *M, double *L, double *Dinv);
