@@ -24,7 +24,10 @@
 #include "cdg_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <new>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -410,9 +413,28 @@ int threads_default(int nthreads) {
     return (int)std::max(1u, std::min(hc, 16u));
 }
 
+// Cubes of 2..256 elements per side (row indices stay 32-bit), P0..P3.
 int check_args(int32_t nx, int32_t ny, int32_t nz, int32_t order) {
-    if (nx < 2 || ny != nx || nz != nx || order < 0 || order > 3) return CDG_ERR_ARG;
+    if (nx < 2 || nx > 256 || ny != nx || nz != nx || order < 0 || order > 3) return CDG_ERR_ARG;
     return CDG_OK;
+}
+
+bool all_finite(const double *v, size_t count) {
+    for (size_t k = 0; k < count; ++k)
+        if (!std::isfinite(v[k])) return false;
+    return true;
+}
+
+// Runs fn() and maps what it may throw (threads, allocations) to status codes:
+// nothing propagates through the C ABI.
+template <typename Fn> int guarded(Fn fn) {
+    try {
+        return fn();
+    } catch (const std::bad_alloc &) {
+        return CDG_ERR_OOM;
+    } catch (...) {
+        return CDG_ERR_ARG;
+    }
 }
 
 uint64_t splitmix(uint64_t &s) {
@@ -446,53 +468,65 @@ extern "C" int cdg_euler_state(int32_t nx, int32_t ny, int32_t nz, int32_t order
 extern "C" int cdg_euler_residual(int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double *R,
                                   int32_t nthreads) {
     if (check_args(nx, ny, nz, order) || !q || !R) return CDG_ERR_ARG;
-    const Disc D(nx, ny, nz, order);
-    parallel_for(D.nrows(), threads_default(nthreads),
-                 [&](int e) { element(D, q, e, R + (size_t)e * D.n, nullptr); });
-    return CDG_OK;
+    return guarded([&] {
+        const Disc D(nx, ny, nz, order);
+        std::atomic<bool> bad{false};
+        parallel_for(D.nrows(), threads_default(nthreads), [&](int e) {
+            element(D, q, e, R + (size_t)e * D.n, nullptr);
+            if (!all_finite(R + (size_t)e * D.n, D.n)) bad = true;
+        });
+        return bad ? CDG_ERR_NAN : CDG_OK;   // e.g. a state with negative pressure
+    });
 }
 
 extern "C" int cdg_euler_jacobian(int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double cfl,
                                   double *blocks, double *R, int32_t nthreads) {
     if (check_args(nx, ny, nz, order) || !q || !blocks) return CDG_ERR_ARG;
-    const Disc D(nx, ny, nz, order);
-    std::vector<int32_t> rp, col, kind, face;
-    cdg_hex_box(nx, ny, nz, 7, rp, col, kind, &face);
-    const int n = D.n;
-    const size_t nn = (size_t)n * n;
-    parallel_for(D.nrows(), threads_default(nthreads), [&](int e) {
-        std::vector<double> Rl(n), Jd(nn);
-        std::vector<std::vector<double>> Jf(6);
-        element(D, q, e, Rl.data(), Jd.data(), &Jf);
-        if (R) std::copy(Rl.begin(), Rl.end(), R + (size_t)e * n);
-        // pseudo-transient term: mass / dtau on every field's diagonal sub-block
-        if (cfl > 0) {
-            const double dtau = cfl * D.h / wave_speed(D, q + (size_t)e * n);
-            for (int f = 0; f < NF; ++f)
-                for (int m = 0; m < D.nt; ++m) {
-                    const size_t r = (size_t)f * D.nt + m;
-                    Jd[r * n + r] += D.mass[m] / dtau;
+    return guarded([&] {
+        const Disc D(nx, ny, nz, order);
+        std::vector<int32_t> rp, col, kind, face;
+        cdg_hex_box(nx, ny, nz, 7, rp, col, kind, &face);
+        const int n = D.n;
+        const size_t nn = (size_t)n * n;
+        std::atomic<bool> bad{false};
+        parallel_for(D.nrows(), threads_default(nthreads), [&](int e) {
+            std::vector<double> Rl(n), Jd(nn);
+            std::vector<std::vector<double>> Jf(6);
+            element(D, q, e, Rl.data(), Jd.data(), &Jf);
+            bool ok = all_finite(Rl.data(), Rl.size()) && all_finite(Jd.data(), nn);
+            for (const auto &f : Jf) ok = ok && all_finite(f.data(), f.size());
+            if (!ok) bad = true;
+            if (R) std::copy(Rl.begin(), Rl.end(), R + (size_t)e * n);
+            // pseudo-transient term: mass / dtau on every field's diagonal sub-block
+            if (cfl > 0) {
+                const double dtau = cfl * D.h / wave_speed(D, q + (size_t)e * n);
+                if (!(dtau > 0) || !std::isfinite(dtau)) bad = true;
+                for (int f = 0; f < NF; ++f)
+                    for (int m = 0; m < D.nt; ++m) {
+                        const size_t r = (size_t)f * D.nt + m;
+                        Jd[r * n + r] += D.mass[m] / dtau;
+                    }
+            }
+            // pattern blocks of row e: the diagonal takes Jd; a local block the
+            // interior face (face id) it came from; a coupling block every
+            // periodic face whose donor it is
+            for (int b = rp[e]; b < rp[e + 1]; ++b) {
+                double *dst = blocks + (size_t)b * nn;
+                if (kind[b] == CDG_KIND_LOCAL && col[b] == e && face[b] == 7) {
+                    std::copy(Jd.begin(), Jd.end(), dst);
+                    continue;
                 }
-        }
-        // pattern blocks of row e: the diagonal takes Jd; a local block the
-        // interior face (face id) it came from; a coupling block every
-        // periodic face whose donor it is
-        for (int b = rp[e]; b < rp[e + 1]; ++b) {
-            double *dst = blocks + (size_t)b * nn;
-            if (kind[b] == CDG_KIND_LOCAL && col[b] == e && face[b] == 7) {
-                std::copy(Jd.begin(), Jd.end(), dst);
-                continue;
+                std::fill(dst, dst + nn, 0.0);
+                for (int fi = 0; fi < 6; ++fi) {
+                    bool wrapped;
+                    const int nb = D.neighbour(e, fi / 2, (fi & 1) ? 1 : -1, wrapped);
+                    const bool take = kind[b] == CDG_KIND_LOCAL ? (!wrapped && face[b] == fi + 1)
+                                                                : (wrapped && nb == col[b]);
+                    if (!take) continue;
+                    for (size_t k = 0; k < nn; ++k) dst[k] += Jf[fi][k];
+                }
             }
-            std::fill(dst, dst + nn, 0.0);
-            for (int fi = 0; fi < 6; ++fi) {
-                bool wrapped;
-                const int nb = D.neighbour(e, fi / 2, (fi & 1) ? 1 : -1, wrapped);
-                const bool take = kind[b] == CDG_KIND_LOCAL ? (!wrapped && face[b] == fi + 1)
-                                                            : (wrapped && nb == col[b]);
-                if (!take) continue;
-                for (size_t k = 0; k < nn; ++k) dst[k] += Jf[fi][k];
-            }
-        }
+        });
+        return bad ? CDG_ERR_NAN : CDG_OK;   // e.g. a state with negative pressure
     });
-    return CDG_OK;
 }

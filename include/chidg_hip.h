@@ -249,7 +249,10 @@ int cdg_server_destroy(cdg_server *s);
  * dtau = cfl h / (mean |u| + mean c) (type_fluid_pseudo_timestep.f90:95-155).
  * Blocks follow the hex-box pattern (all directions periodic; the wrap
  * neighbours are coupling blocks).  Vectors are nrows*n, element-major.
- * nthreads <= 0: the host's cores (at most 16). */
+ * nthreads <= 0: the host's cores (at most 16).  Cubes only: nx = ny = nz in
+ * 2..256, order 0..3, else CDG_ERR_ARG; CDG_ERR_NAN when the state gives a
+ * non-finite residual or Jacobian (a negative pressure leaves the pseudo-time
+ * step undefined); CDG_ERR_OOM when a host allocation fails. */
 /* q: freestream rho = 1, p = 1/gamma, u = (mach, 0, 0) on mode 0, plus
  * perturb * N(0,1) (seeded) on every coefficient. */
 int cdg_euler_state(int32_t nx, int32_t ny, int32_t nz, int32_t order, double mach, uint64_t seed, double perturb,

@@ -109,3 +109,23 @@ def test_first_newton_system_solves():
     assert 1 <= niter < 200
     A = dense(p, blocks)
     assert np.linalg.norm(A @ x + R) / np.linalg.norm(R) < 1e-7
+
+
+def test_invalid_inputs_are_errors():
+    """Sizes outside the generator's range, non-finite states, and states
+    without a real sound speed (negative pressure: the pseudo-time step is
+    undefined) come back as status codes, not NaN blocks."""
+    from chidg_amd import _lib
+    with pytest.raises(_lib.CdgError):
+        euler.state(1, 1)
+    with pytest.raises(_lib.CdgError):
+        euler.state(3, 4)
+    q = euler.state(2, 1, mach=0.3, perturb=0.0)
+    n = pt.block_size(1)
+    q = q.reshape(8, 5, n // 5)
+    q[3, 4, 0] = -1.0            # energy below the kinetic energy: p < 0
+    with pytest.raises(_lib.CdgError):
+        euler.jacobian(2, 1, q.ravel(), cfl=1.0)
+    q[3, 4, 0] = np.nan
+    with pytest.raises(_lib.CdgError):
+        euler.residual(2, 1, q.ravel())
