@@ -417,15 +417,31 @@ int launch_kslice(cdg_ctx *ctx, const cdg_prec *M, KSliceArgs a, int nrows_lvl) 
     a.dc = sh.dc;
     const size_t lds = sizeof(Acc) * ((size_t)a.maxdep * a.w + (size_t)V * sh.T);
     const dim3 grid(nrows_lvl * M->slices), block(sh.T);
-    const bool p2 = a.x.p2 != nullptr, nb4 = a.S <= 4;
-#define CDG_KSLICE(UU, NBB, PP) hipLaunchKernelGGL((k_kslice<S, Acc, UU, NBB, PP>), grid, block, lds, ctx->stream, a)
-    if (M->unroll == 8) {
-        if (nb4) { if (p2) CDG_KSLICE(8, 4, true); else CDG_KSLICE(8, 4, false); }
-        else { if (p2) CDG_KSLICE(8, 8, true); else CDG_KSLICE(8, 8, false); }
-    } else {
-        if (nb4) { if (p2) CDG_KSLICE(4, 4, true); else CDG_KSLICE(4, 4, false); }
-        else { if (p2) CDG_KSLICE(4, 8, true); else CDG_KSLICE(4, 8, false); }
+    // gather batch = the slice count up to 8 (no duplicate loads), batches of 8 beyond
+    const bool p2 = a.x.p2 != nullptr;
+    const int nb = std::min(a.S, 8);
+#define CDG_KSLICE(UU, NBB)                                                                                   \
+    do {                                                                                                      \
+        if (p2) hipLaunchKernelGGL((k_kslice<S, Acc, UU, NBB, true>), grid, block, lds, ctx->stream, a);      \
+        else hipLaunchKernelGGL((k_kslice<S, Acc, UU, NBB, false>), grid, block, lds, ctx->stream, a);        \
+    } while (0)
+#define CDG_KSLICE_NB(UU)                                                                                     \
+    switch (nb) {                                                                                             \
+    case 1: CDG_KSLICE(UU, 1); break;                                                                         \
+    case 2: CDG_KSLICE(UU, 2); break;                                                                         \
+    case 3: CDG_KSLICE(UU, 3); break;                                                                         \
+    case 4: CDG_KSLICE(UU, 4); break;                                                                         \
+    case 5: CDG_KSLICE(UU, 5); break;                                                                         \
+    case 6: CDG_KSLICE(UU, 6); break;                                                                         \
+    case 7: CDG_KSLICE(UU, 7); break;                                                                         \
+    default: CDG_KSLICE(UU, 8); break;                                                                        \
     }
+    if (M->unroll == 8) {
+        CDG_KSLICE_NB(8)
+    } else {
+        CDG_KSLICE_NB(4)
+    }
+#undef CDG_KSLICE_NB
 #undef CDG_KSLICE
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
@@ -467,8 +483,23 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
         const SD *Dv = static_cast<const SD *>(M->Dinv);
 #define CDG_DINVZ(UU, NBB) hipLaunchKernelGGL((k_dinv_z<SD, Acc, UU, NBB>), dim3(M->nrows), dim3(sh.T), lds, \
                                               ctx->stream, Dv, M->ld, t, M->slices, M->n, sh.dc, z)
-        if (M->unroll == 8) { if (M->slices <= 4) CDG_DINVZ(8, 4); else CDG_DINVZ(8, 8); }
-        else { if (M->slices <= 4) CDG_DINVZ(4, 4); else CDG_DINVZ(4, 8); }
+#define CDG_DINVZ_NB(UU)                                                                                      \
+        switch (std::min(M->slices, 8)) {                                                                     \
+        case 1: CDG_DINVZ(UU, 1); break;                                                                      \
+        case 2: CDG_DINVZ(UU, 2); break;                                                                      \
+        case 3: CDG_DINVZ(UU, 3); break;                                                                      \
+        case 4: CDG_DINVZ(UU, 4); break;                                                                      \
+        case 5: CDG_DINVZ(UU, 5); break;                                                                      \
+        case 6: CDG_DINVZ(UU, 6); break;                                                                      \
+        case 7: CDG_DINVZ(UU, 7); break;                                                                      \
+        default: CDG_DINVZ(UU, 8); break;                                                                     \
+        }
+        if (M->unroll == 8) {
+            CDG_DINVZ_NB(8)
+        } else {
+            CDG_DINVZ_NB(4)
+        }
+#undef CDG_DINVZ_NB
 #undef CDG_DINVZ
         CDG_HIP_TRY(hipGetLastError());
         return CDG_OK;
