@@ -42,11 +42,11 @@ SEED_B = 1          # right-hand side
 SEED_A = 20261202   # synthetic systems (P3 section, synthetic P2 section)
 HBM_PEAK_GBS = 8000.0
 # The FP32 ILU(0) apply in the folded schedule (W = U Dinv), one call: 46
-# forward and 46 backward level launches of k_kslice<float,float,4,8,*> and one
-# k_dinv_z<float,float,4,8> (z = Dinv t over all rows), 93 launches (default
+# forward and 46 backward level launches of k_kslice<float,float,4,5,*> and one
+# k_dinv_z<float,float,4,5> (z = Dinv t over all rows), 93 launches (default
 # shape: 5 slices of 32 columns).
-APPLY_KERNEL = ("ILU(0) apply FP32, folded schedule: 92 x k_kslice<float,float,4,8,*> + "
-                "k_dinv_z<float,float,4,8> (one call, 93 launches)")
+APPLY_KERNEL = ("ILU(0) apply FP32, folded schedule: 92 x k_kslice<float,float,4,5,*> + "
+                "k_dinv_z<float,float,4,5> (one call, 93 launches)")
 # HBM bytes per apply call from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) on the
 # Euler 16^3 P2 system with the default shape, the factors being what the same counters
 # read for a 512 MiB device copy (profiles/hbm_traffic_apply_folded_euler_16x16x16_p2.md).
