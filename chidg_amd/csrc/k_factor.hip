@@ -1229,15 +1229,32 @@ template <typename S> __device__ __forceinline__ S narrow(double v);
 template <> __device__ __forceinline__ float narrow<float>(double v) { return static_cast<float>(v); }
 template <> __device__ __forceinline__ cdg_bf16 narrow<cdg_bf16>(double v) { return to_bf16(static_cast<float>(v)); }
 
-// U_u = (S) A_{up_blk[u]}   (the apply's copy of A's upper blocks)
+// Four narrowed values to dst[0..3] in one store (dst 8- or 16-byte aligned).
+__device__ __forceinline__ void store4(float *dst, double a, double b, double c, double d) {
+    *reinterpret_cast<float4 *>(dst) = make_float4((float)a, (float)b, (float)c, (float)d);
+}
+__device__ __forceinline__ void store4(cdg_bf16 *dst, double a, double b, double c, double d) {
+    const uint32_t lo = (uint32_t)narrow<cdg_bf16>(a) | ((uint32_t)narrow<cdg_bf16>(b) << 16);
+    const uint32_t hi = (uint32_t)narrow<cdg_bf16>(c) | ((uint32_t)narrow<cdg_bf16>(d) << 16);
+    *reinterpret_cast<uint2 *>(dst) = make_uint2(lo, hi);
+}
+
+// U_u = (S) A_{up_blk[u]}   (the apply's copy of A's upper blocks).  One
+// workgroup per block: 16-byte loads of two doubles, four values per thread and
+// step (slots start 512-byte aligned in A and 128-byte aligned in U).
 template <typename S>
-__global__ void k_copy_upper(const double *__restrict__ Ab, int64_t lda, S *__restrict__ U, int64_t ld,
-                             const int32_t *__restrict__ up_blk, int nup, int nn) {
-    for (int u = blockIdx.y; u < nup; u += gridDim.y) {
+__global__ void __launch_bounds__(256) k_copy_upper(const double *__restrict__ Ab, int64_t lda, S *__restrict__ U,
+                                                    int64_t ld, const int32_t *__restrict__ up_blk, int nup, int nn) {
+    for (int u = blockIdx.x; u < nup; u += gridDim.x) {
         const double *src = Ab + (size_t)up_blk[u] * lda;
         S *dst = U + (size_t)u * ld;
-        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += gridDim.x * blockDim.x)
-            dst[e] = narrow<S>(src[e]);
+        const int n4 = nn / 4;
+        for (int q = threadIdx.x; q < n4; q += blockDim.x) {
+            const double2 a = reinterpret_cast<const double2 *>(src)[2 * q];
+            const double2 b = reinterpret_cast<const double2 *>(src)[2 * q + 1];
+            store4(dst + 4 * q, a.x, a.y, b.x, b.y);
+        }
+        for (int e = 4 * n4 + threadIdx.x; e < nn; e += blockDim.x) dst[e] = narrow<S>(src[e]);
     }
 }
 
@@ -1392,7 +1409,7 @@ int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_
     // the FP32/BF16 copies of A's upper blocks (read by the backward sweep; the
     // FP32 factor's phase 2 reads its U_q from them too)
     if (M->precision != CDG_FP64 && M->nup > 0) {
-        dim3 grid((nn + 255) / 256 < 64 ? (nn + 255) / 256 : 64, std::min(M->nup, 65535));
+        const dim3 grid(std::min(M->nup, 16384));
         if (M->precision == CDG_FP32)
             hipLaunchKernelGGL(k_copy_upper<float>, grid, dim3(256), 0, ctx->stream, A->blocks, A->ld,
                                static_cast<float *>(M->U), M->ld, M->d_up_blk, M->nup, nn);
