@@ -11,6 +11,8 @@
 // block sizes 40/135/320 with little padding.
 #include "cdg_internal.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cstdlib>
 
@@ -189,10 +191,11 @@ __device__ __forceinline__ void gemm_out(int tm, int tn, int n, typename Mfma<T>
 }
 
 // phase 1: L_q = A_{lo_blk[q]} * Dinv_{lo_col[q]} (with the upper lists in
-// place of the lower ones: W_u = A_{up_blk[u]} * Dinv_{up_col[u]})
-template <typename T>
+// place of the lower ones: W_u = A_{up_blk[u]} * Dinv_{up_col[u]}); the A
+// operand is A's own FP64 blocks or, for the FP32 fold, their FP32 copies
+template <typename T, typename SA = double>
 __global__ void __launch_bounds__(GT)
-k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
+k_fac_p1(const SA *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ jobs, int njobs, const int32_t *__restrict__ lo_blk,
          const int32_t *__restrict__ lo_col, int n, int tiles) {
     __shared__ __attribute__((aligned(16))) ATile<T> As[2];
@@ -204,10 +207,10 @@ k_fac_p1(const double *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T 
     typename Mfma<T>::acc_t acc[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
-    const double *A = Ab + (size_t)lo_blk[q] * lda;
+    const SA *A = Ab + (size_t)lo_blk[q] * lda;
     const T *B = D + (size_t)lo_col[q] * ld;
-    gemm_pairs<T, double, T>([&](int, const double *&a, const T *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As,
-                             Bs, acc);
+    gemm_pairs<T, SA, T>([&](int, const SA *&a, const T *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As, Bs,
+                         acc);
     T *out = L + (size_t)q * ld;
     gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) { out[(size_t)col * n + row] = v; });
 }
@@ -1341,8 +1344,15 @@ int fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const T *D) {
     const int n = M->n, tiles = (n + TM - 1) / TM;
     const int u1 = M->nup;
     if (u1 == 0) return CDG_OK;
-    hipLaunchKernelGGL((k_fac_p1<T>), dim3(xcd_grid(u1, tiles * tiles)), dim3(GT), 0, ctx->stream, A->blocks, A->ld,
-                       static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1, M->d_up_blk, M->d_up_col, n, tiles);
+    if (std::is_same<T, float>::value && M->precision == CDG_FP32 && M->U)
+        // the upper blocks' FP32 copies (slot u; the values the FP64 reads convert to): half the bytes
+        hipLaunchKernelGGL((k_fac_p1<T, float>), dim3(xcd_grid(u1, tiles * tiles)), dim3(GT), 0, ctx->stream,
+                           static_cast<const float *>(M->U), M->ld, static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1,
+                           M->d_up_all, M->d_up_col, n, tiles);
+    else
+        hipLaunchKernelGGL((k_fac_p1<T>), dim3(xcd_grid(u1, tiles * tiles)), dim3(GT), 0, ctx->stream, A->blocks,
+                           A->ld, static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1, M->d_up_blk, M->d_up_col, n,
+                           tiles);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
