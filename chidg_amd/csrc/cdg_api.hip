@@ -1105,19 +1105,26 @@ int fetch_small(cdg_ctx *ctx, size_t n) {
     return ctx_wait(ctx);
 }
 
-// One orthogonalization pass of w = V_jj (the slot of the next basis vector)
-// against V_0..V_{jj-1}: host hcol += the products, *nrm2 = ||w||^2 after the
-// pass, and with normb2 also ||w||^2 before it (CGS: w . w is one more
-// product of the same launch).  One device-to-host fetch per pass.
-// d_small layout from d_h: [0, jj) products, [jj] ||w||^2 before, [jj+1] after.
+// Orthogonalization of w = V_jj (the slot of the next basis vector) against
+// V_0..V_{jj-1}: host hcol += the products, *nrm2 = ||w||^2 after, and with
+// normb2 also ||w||^2 before (CGS: w . w is one more product of the same
+// launch).  With `reorth` (CGS, first pass) the selective re-orthogonalization
+// test runs on the device and its second pass follows at once, gated by the
+// test's flag, so both passes cost one device-to-host fetch; *reorth tells
+// whether the second pass ran.  Otherwise one pass (*reorth = -1 when given:
+// the caller applies the test).
+// d_small layout from d_h: [0, jj) products, [jj] ||w||^2 before, [jj+1]
+// after; with the gated pass also [jj+2] flag, [jj+3, 2jj+3) second
+// products, [2jj+3] ||w||^2 after the second pass.
 int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *w, double *hcol, double *nrm2,
-                  double *normb2) {
+                  double *normb2, int *reorth = nullptr) {
     cdg_ctx *ctx = S.ctx;
     const int64_t len = S.len;
     if (w != ws.V + (size_t)jj * len) return cdg_fail(ctx, CDG_ERR_STATE, "orthogonalize: w is not V_%d", jj);
-    int e = ctx_ensure_small(ctx, (size_t)jj + 2 + 8 + ws.m + 8);
+    int e = ctx_ensure_small(ctx, 2 * (size_t)jj + 4 + 8 + ws.m + 8);
     if (e) return e;
     double *d_h = ctx->d_small + 8 + ws.m + 8;   // beyond the combine scratch
+    const bool gated = reorth && normb2 && orth != CDG_MGS;
     if (orth == CDG_MGS) {
         if (normb2) e = kv_sumsq(ctx, len, w, d_h + jj);
         for (int i = 0; i < jj && !e; ++i) {
@@ -1128,14 +1135,30 @@ int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *w, double 
     } else {
         e = kv_multidot(ctx, len, w, ws.V, len, normb2 ? jj + 1 : jj, d_h);
         if (!e) e = kv_cgs_update(ctx, len, w, ws.V, len, jj, d_h, d_h + jj + 1);
+        if (gated) {
+            double *flag = d_h + jj + 2, *h2 = d_h + jj + 3;
+            if (!e) e = kv_reorth_gate(ctx, d_h, jj, flag);
+            if (!e) e = kv_multidot(ctx, len, w, ws.V, len, jj, h2, flag);
+            if (!e) e = kv_cgs_update(ctx, len, w, ws.V, len, jj, h2, h2 + jj, flag);
+        }
     }
     if (e) return e;
-    CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_h, sizeof(double) * (jj + 2), hipMemcpyDeviceToHost, ctx->stream));
+    const size_t nfetch = gated ? 2 * (size_t)jj + 4 : (size_t)jj + 2;
+    CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_h, sizeof(double) * nfetch, hipMemcpyDeviceToHost, ctx->stream));
     e = ctx_wait(ctx);
     if (e) return e;
-    for (int i = 0; i < jj; ++i) hcol[i] += ctx->h_small[i];
-    if (normb2) *normb2 = ctx->h_small[jj];
-    *nrm2 = ctx->h_small[jj + 1];
+    const double *hs = ctx->h_small;
+    for (int i = 0; i < jj; ++i) hcol[i] += hs[i];
+    if (normb2) *normb2 = hs[jj];
+    *nrm2 = hs[jj + 1];
+    if (reorth) *reorth = -1;
+    if (gated) {
+        *reorth = hs[jj + 2] != 0.0;
+        if (*reorth) {
+            for (int i = 0; i < jj; ++i) hcol[i] += hs[jj + 3 + i];
+            *nrm2 = hs[2 * jj + 3];
+        }
+    }
     return CDG_OK;
 }
 
@@ -1214,7 +1237,8 @@ int fgmres_core(Solver &S, std::vector<Workspace> &wss, int depth, const double 
             CDG_HIP_TRY(hipEventRecord(S.ev[depth][2], ctx->stream));
             // orthogonalize; nb = ||w|| before it comes with the same fetch   (:191-210)
             double nrm2 = 0.0, normb2 = 0.0;
-            e = orthogonalize(S, ws, orth, j + 1, wj, &H(0, j), &nrm2, &normb2);
+            int reorth = -1;
+            e = orthogonalize(S, ws, orth, j + 1, wj, &H(0, j), &nrm2, &normb2, &reorth);
             if (e) return e;
             {   // the fetch synchronized the stream: the events are complete
                 float ms_p = 0.f, ms_m = 0.f;
@@ -1225,12 +1249,17 @@ int fgmres_core(Solver &S, std::vector<Workspace> &wss, int depth, const double 
                 S.t.mv += tm;
                 S.t.dot += std::max(0.0, secs(ta, clk::now()) - tp - tm);
             }
-            const double norm_before = std::sqrt(normb2);
-            // selective re-orthogonalization, Giraud-Langou L_crit = 1   (:212-223)
+            // selective re-orthogonalization, Giraud-Langou L_crit = 1   (:212-223);
+            // CGS ran the test and the second pass on the device already
             double crit = 0.0;
-            for (int i = 0; i <= j; ++i) crit += std::fabs(H(i, j));
-            crit /= norm_before;
-            if (crit >= 1.0) {
+            if (reorth < 0) {
+                const double norm_before = std::sqrt(normb2);
+                for (int i = 0; i <= j; ++i) crit += std::fabs(H(i, j));
+                crit /= norm_before;
+            } else if (reorth) {
+                S.nreorth++;
+            }
+            if (reorth < 0 && crit >= 1.0) {
                 auto tr = clk::now();
                 e = orthogonalize(S, ws, orth, j + 1, wj, &H(0, j), &nrm2, nullptr);
                 if (e) return e;

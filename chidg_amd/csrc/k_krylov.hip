@@ -28,8 +28,9 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 // With an even vector stride every thread reads element pairs (16-byte loads).
 __global__ void __launch_bounds__(RT)
 k_dot_part(int64_t len, const double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
-           double *__restrict__ part) {
+           double *__restrict__ part, const double *__restrict__ gate) {
     __shared__ double sh[RT];
+    if (gate && *gate == 0.0) return;   // a pass the host-side rule skips
     const int nb = gridDim.x, i0 = 8 * blockIdx.y;
     const int ni = min(8, j - i0);
     double acc[8];
@@ -79,8 +80,10 @@ __global__ void __launch_bounds__(RT) k_sumsq_part(int64_t len, const double *__
 }
 
 // out[i] = sum_b part[i*nb + b]
-__global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, int nb, double *__restrict__ out) {
+__global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, int nb, double *__restrict__ out,
+                                               const double *__restrict__ gate = nullptr) {
     __shared__ double sh[RT];
+    if (gate && *gate == 0.0) return;
     const int i = blockIdx.x;
     double acc = 0.0;
     for (int b = threadIdx.x; b < nb; b += RT) acc += part[(size_t)i * nb + b];
@@ -94,8 +97,9 @@ __global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, 
 // stride every thread updates element pairs (16-byte loads and stores).
 __global__ void __launch_bounds__(RT)
 k_cgs_update(int64_t len, double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
-             const double *__restrict__ h, double *__restrict__ part) {
+             const double *__restrict__ h, double *__restrict__ part, const double *__restrict__ gate) {
     __shared__ double sh[RT];
+    if (gate && *gate == 0.0) return;
     double acc = 0.0;
     const bool pairs = ((vstride | (int64_t)(reinterpret_cast<uintptr_t>(V) >> 3) |
                          (int64_t)(reinterpret_cast<uintptr_t>(w) >> 3)) & 1) == 0;
@@ -139,6 +143,17 @@ k_cgs_update(int64_t len, double *__restrict__ w, const double *__restrict__ V, 
         const double s = block_sum(acc, sh);
         if (threadIdx.x == 0) part[blockIdx.x] = s;
     }
+}
+
+// flag = 1 when the selective re-orthogonalization rule asks for a second
+// pass: sum_{i<jj} |h_i| / sqrt(w.w before) >= 1 (type_fgmres.f90:212-223),
+// the same operations in the same order as the host's test.
+__global__ void k_reorth_gate(const double *__restrict__ h, int jj, double *__restrict__ flag) {
+    if (threadIdx.x != 0) return;
+    double crit = 0.0;
+    for (int i = 0; i < jj; ++i) crit += fabs(h[i]);
+    crit /= sqrt(h[jj]);
+    *flag = crit >= 1.0 ? 1.0 : 0.0;
 }
 
 __global__ void k_axpy_dev(int64_t len, double *__restrict__ w, const double *__restrict__ v, const double *__restrict__ h) {
@@ -191,13 +206,15 @@ dim3 egrid(int64_t len) { return dim3((unsigned)std::min<int64_t>(std::max<int64
 
 }  // namespace
 
-int kv_multidot(cdg_ctx *ctx, int64_t len, const double *w, const double *V, int64_t vstride, int j, double *d_out) {
+int kv_multidot(cdg_ctx *ctx, int64_t len, const double *w, const double *V, int64_t vstride, int j, double *d_out,
+                const double *d_gate) {
     if (j <= 0) return CDG_OK;
     const int nb = nparts(len);
     int e = ctx_ensure_part(ctx, (size_t)nb * j);
     if (e) return e;
-    hipLaunchKernelGGL(k_dot_part, dim3(nb, (j + 7) / 8), dim3(RT), 0, ctx->stream, len, w, V, vstride, j, ctx->d_part);
-    hipLaunchKernelGGL(k_reduce, dim3(j), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_out);
+    hipLaunchKernelGGL(k_dot_part, dim3(nb, (j + 7) / 8), dim3(RT), 0, ctx->stream, len, w, V, vstride, j, ctx->d_part,
+                       d_gate);
+    hipLaunchKernelGGL(k_reduce, dim3(j), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_out, d_gate);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
@@ -213,13 +230,19 @@ int kv_sumsq(cdg_ctx *ctx, int64_t len, const double *w, double *d_out) {
 }
 
 int kv_cgs_update(cdg_ctx *ctx, int64_t len, double *w, const double *V, int64_t vstride, int j, const double *d_h,
-                  double *d_nrm2) {
+                  double *d_nrm2, const double *d_gate) {
     const int nb = nparts(len);
     int e = ctx_ensure_part(ctx, (size_t)nb);
     if (e) return e;
     hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(RT), 0, ctx->stream, len, w, V, vstride, j, d_h,
-                       d_nrm2 ? ctx->d_part : nullptr);
-    if (d_nrm2) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_nrm2);
+                       d_nrm2 ? ctx->d_part : nullptr, d_gate);
+    if (d_nrm2) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_nrm2, d_gate);
+    CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+int kv_reorth_gate(cdg_ctx *ctx, const double *d_h, int jj, double *d_flag) {
+    hipLaunchKernelGGL(k_reorth_gate, dim3(1), dim3(64), 0, ctx->stream, d_h, jj, d_flag);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
