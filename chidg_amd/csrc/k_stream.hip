@@ -164,11 +164,14 @@ k_mv(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ 
 //     u = base + sign1 * sum_s part1[s] + sign2 * sum_s part2[s]
 // and rebuilt by their consumers (only the consumer's column slice) in a
 // fixed order, so results are deterministic.  A last elementwise pass
-// materializes z.
+// materializes z.  The partials are stored in the accumulation type (FP32 for
+// the FP32 and BF16 factors: a slice's sum is an FP32 value, so storing it as
+// FP32 loses nothing), and widened to FP64 where they are summed.
 // ============================================================================
 struct VecView {
     const double *base;       // may be null (0)
-    const double *p1, *p2;    // partial arrays [nrows][S][n]; p1 never null, p2 may be
+    const void *p1, *p2;      // partial arrays [nrows][S][n] of the accumulation type;
+                              // p1 never null, p2 may be
     double s1, s2;            // signs of the partial sums
 };
 
@@ -183,15 +186,17 @@ template <int NB> struct ViewLd {
     double q1[NB], q2[NB];
 };
 
-template <int NB, bool P2>
+template <int NB, bool P2, typename PT>
 __device__ __forceinline__ ViewLd<NB> view_load(const VecView &u, int row, int c, int n, int S, int s0) {
     ViewLd<NB> r;
+    const PT *p1 = static_cast<const PT *>(u.p1), *p2 = static_cast<const PT *>(u.p2);
     const size_t rb = (size_t)row * S * n + c;
-    r.b = *(u.base ? u.base + (size_t)row * n + c : u.p1 + rb);
+    // without a base, an in-bounds stand-in inside p1's bytes (value unused)
+    r.b = *(u.base ? u.base + (size_t)row * n + c : reinterpret_cast<const double *>(p1) + rb * sizeof(PT) / 8);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) r.q1[i] = u.p1[rb + (size_t)min(s0 + i, S - 1) * n];
+    for (int i = 0; i < NB; ++i) r.q1[i] = static_cast<double>(p1[rb + (size_t)min(s0 + i, S - 1) * n]);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) r.q2[i] = P2 ? u.p2[rb + (size_t)min(s0 + i, S - 1) * n] : 0.0;
+    for (int i = 0; i < NB; ++i) r.q2[i] = P2 ? static_cast<double>(p2[rb + (size_t)min(s0 + i, S - 1) * n]) : 0.0;
     return r;
 }
 
@@ -206,20 +211,20 @@ __device__ __forceinline__ void view_add(const ViewLd<NB> &r, int S, int s0, dou
 }
 
 // u(row, c) from the loads of its first NB slices (the rest loaded here)
-template <int NB, bool P2>
+template <int NB, bool P2, typename PT>
 __device__ __forceinline__ double view_sum(const VecView &u, const ViewLd<NB> &r, int row, int c, int n, int S) {
     double t1 = 0.0, t2 = 0.0;
     view_add<NB>(r, S, 0, t1, t2);
-    for (int s0 = NB; s0 < S; s0 += NB) view_add<NB>(view_load<NB, P2>(u, row, c, n, S, s0), S, s0, t1, t2);
+    for (int s0 = NB; s0 < S; s0 += NB) view_add<NB>(view_load<NB, P2, PT>(u, row, c, n, S, s0), S, s0, t1, t2);
     double x = u.base ? r.b : 0.0;
     x += u.s1 * t1;
     if (P2) x += u.s2 * t2;
     return x;
 }
 
-template <int NB, bool P2>
+template <int NB, bool P2, typename PT>
 __device__ __forceinline__ double view_at(const VecView &u, int row, int c, int n, int S) {
-    return view_sum<NB, P2>(u, view_load<NB, P2>(u, row, c, n, S, 0), row, c, n, S);
+    return view_sum<NB, P2, PT>(u, view_load<NB, P2, PT>(u, row, c, n, S, 0), row, c, n, S);
 }
 
 struct KSliceArgs {
@@ -230,7 +235,7 @@ struct KSliceArgs {
     const int32_t *idx;       // block slot of entry d (null: slot = d)
     const int32_t *col;       // parent row of entry d
     VecView x;                // vector the parents' slices are read from
-    double *part;             // output partials [nrows][S][n]
+    void *part;               // output partials [nrows][S][n], accumulation type
     int n, S, w, maxdep, dc;  // block size, slices, slice width (columns), max entries, walk step
 };
 
@@ -265,17 +270,17 @@ __global__ void __launch_bounds__(kMaxThreads) k_kslice(KSliceArgs a) {
     const bool g0 = tid < ne;
     const int kg = g0 ? tid / wc : 0, cg = g0 ? tid - kg * wc : 0;
     const int pg = a.ptr && nd > 0 ? a.col[d0 + kg] : row;
-    const ViewLd<NB> r0 = view_load<NB, P2>(a.x, pg, c0 + cg, n, a.S, 0);
+    const ViewLd<NB> r0 = view_load<NB, P2, Acc>(a.x, pg, c0 + cg, n, a.S, 0);
     const int m0 = nd > 0 ? min(U, wc * n / wk.step) : 0;
     const S *b0 = nd > 0 ? Bp + (size_t)slot_of(0) * a.ld + (size_t)c0 * n : reinterpret_cast<const S *>(a.x.p1);
     Acc a0[U][V];
 #pragma unroll
     for (int u = 0; u < U; ++u) Ld16<S, Acc>::load(b0 + (u < m0 ? wk.e0 + u * wk.step : 0), a0[u]);
-    if (g0) xs[kg * a.w + cg] = static_cast<Acc>(view_sum<NB, P2>(a.x, r0, pg, c0 + cg, n, a.S));
+    if (g0) xs[kg * a.w + cg] = static_cast<Acc>(view_sum<NB, P2, Acc>(a.x, r0, pg, c0 + cg, n, a.S));
     for (int e = tid + T; e < ne; e += T) {
         const int k = e / wc, c = e - k * wc;
         const int parent = a.ptr ? a.col[d0 + k] : row;
-        xs[k * a.w + c] = static_cast<Acc>(view_at<NB, P2>(a.x, parent, c0 + c, n, a.S));
+        xs[k * a.w + c] = static_cast<Acc>(view_at<NB, P2, Acc>(a.x, parent, c0 + c, n, a.S));
     }
     __syncthreads();
     Acc acc[V];
@@ -293,8 +298,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_kslice(KSliceArgs a) {
     for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
     __syncthreads();
     const int parts = V * T / n;
-    double *out = a.part + ((size_t)row * a.S + sl) * n;
-    for (int r = tid; r < n; r += T) out[r] = static_cast<double>(row_sum(red, r, n, parts));
+    Acc *out = static_cast<Acc *>(a.part) + ((size_t)row * a.S + sl) * n;
+    for (int r = tid; r < n; r += T) out[r] = row_sum(red, r, n, parts);
 }
 
 // z_i = Dinv_i t_i for every row i, the folded schedule's last stage (t in
@@ -312,13 +317,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_dinv_z(const S *__restrict__ D,
     const S *blk = D + (size_t)row * ld;
     const int nn = n * n;
     const Walk<V> w = make_walk<V>(tid, T, n, dc);
-    const ViewLd<NB> r0 = view_load<NB, true>(t, row, min(tid, n - 1), n, Sp, 0);
+    const ViewLd<NB> r0 = view_load<NB, true, Acc>(t, row, min(tid, n - 1), n, Sp, 0);
     const int m0 = min(U, nn / w.step);
     Acc a0[U][V];
 #pragma unroll
     for (int u = 0; u < U; ++u) Ld16<S, Acc>::load(blk + (u < m0 ? w.e0 + u * w.step : 0), a0[u]);
-    if (tid < n) xs[tid] = static_cast<Acc>(view_sum<NB, true>(t, r0, row, tid, n, Sp));
-    for (int c = tid + T; c < n; c += T) xs[c] = static_cast<Acc>(view_at<NB, true>(t, row, c, n, Sp));
+    if (tid < n) xs[tid] = static_cast<Acc>(view_sum<NB, true, Acc>(t, r0, row, tid, n, Sp));
+    for (int c = tid + T; c < n; c += T) xs[c] = static_cast<Acc>(view_at<NB, true, Acc>(t, row, c, n, Sp));
     __syncthreads();
     Acc acc[V];
 #pragma unroll
@@ -337,7 +342,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_dinv_z(const S *__restrict__ D,
 }
 
 // z = sum_s part[s]  (materializes the backward result)
-__global__ void k_sum_parts(const double *__restrict__ part, int S, int n, int64_t len, double *__restrict__ z) {
+template <typename PT>
+__global__ void k_sum_parts(const PT *__restrict__ part, int S, int n, int64_t len, double *__restrict__ z) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t row = e / n, c = e - row * n;
         double t = 0.0;
@@ -451,7 +457,8 @@ template <typename S, typename Acc>
 int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
     const void *U = M->precision == CDG_FP64 ? static_cast<const void *>(A->blocks) : M->U;
     const int64_t ldu = M->precision == CDG_FP64 ? A->ld : M->ld;
-    double *P = M->d_part, *QA = M->d_part + M->part_len, *PB = M->d_part + 2 * M->part_len;
+    // partial arrays in the accumulation type (the storage holds 3 part_len doubles)
+    Acc *P = reinterpret_cast<Acc *>(M->d_part), *QA = P + M->part_len, *PB = P + 2 * M->part_len;
     const VecView zf{v, P, nullptr, -1.0, 0.0};      // forward result  v - sum P
     const VecView zb{nullptr, PB, nullptr, 1.0, 0.0};  // backward result sum PB
     const VecView t{v, P, QA, -1.0, -1.0};            // stage A result v - sum P - sum QA
@@ -521,7 +528,7 @@ int sweep(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *
         }
     }
     const int64_t len = (int64_t)M->nrows * M->n;
-    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<int64_t>((len + 255) / 256, 4096)), dim3(256), 0,
+    hipLaunchKernelGGL(k_sum_parts<Acc>, dim3((unsigned)std::min<int64_t>((len + 255) / 256, 4096)), dim3(256), 0,
                        ctx->stream, PB, M->slices, M->n, len, z);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
