@@ -50,7 +50,7 @@ APPLY_KERNEL = ("ILU(0) apply FP32, folded schedule: 92 x k_kslice<float,float,4
 # HBM bytes per apply call from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) on the
 # Euler 16^3 P2 system with the default shape, the factors being what the same counters
 # read for a 512 MiB device copy (profiles/hbm_traffic_apply_folded_euler_16x16x16_p2.md).
-APPLY_TRAFFIC_FP32 = 2.390e9
+APPLY_TRAFFIC_FP32 = 2.260e9
 
 
 def sig(v, digits=5):
