@@ -196,7 +196,11 @@ std::mutex g_tune_mu;
 std::map<std::pair<int, int>, ApplyShape> g_apply_tuned;   // (n, precision)
 std::map<int, MvShape> g_mv_tuned;                          // n
 
-ApplyShape apply_default(int n) { return {(n > 64 && n <= 192) ? 32 : 40, 256, 4}; }
+// measured at n = 135: 32-column slices; 8 loads in flight for FP64, 4 for FP32/BF16
+ApplyShape apply_default(int n, int precision) {
+    const bool p2 = n > 64 && n <= 192;
+    return {p2 ? 32 : 40, 256, p2 && precision == CDG_FP64 ? 8 : 4};
+}
 MvShape mv_default() { return {540, 8}; }
 
 bool apply_tuned(int n, int precision, ApplyShape *out) {
@@ -681,7 +685,7 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     // Apply launch shape: the tuned one for this block size and precision if
     // cdg_prec_tune has run, else the default; CDG_APPLY_SLICE_W /
     // CDG_APPLY_THREADS / CDG_APPLY_UNROLL override either.
-    ApplyShape ash = apply_default(A->n);
+    ApplyShape ash = apply_default(A->n, precision);
     (void)apply_tuned(A->n, precision, &ash);
     if (const char *ev = std::getenv("CDG_APPLY_SLICE_W")) ash.sw = std::max(8, std::atoi(ev));
     if (const char *ev = std::getenv("CDG_APPLY_THREADS")) ash.target = std::max(64, std::atoi(ev));
@@ -817,7 +821,7 @@ extern "C" int cdg_prec_tune(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_
     if (e) return e;
     // the default first; another shape must beat the best so far by 2 % (the
     // run-to-run spread of one candidate's timing) to replace it
-    const ApplyShape def = apply_default(n);
+    const ApplyShape def = apply_default(n, prec);
     std::vector<ApplyShape> cands{def};
     for (int sw : {16, 24, 32, 40, 48, 64, 96, 128, n})
         for (int tg : {128, 256, 512})
