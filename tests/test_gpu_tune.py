@@ -85,3 +85,30 @@ def test_tune_before_update_is_an_error(ctx):
     with pytest.raises(_lib.CdgError):
         M.tune(A)
     M.tear_down()
+
+
+@pytest.mark.parametrize("degree,sw,precision,tol", [
+    (2, 16, "fp32", 1e-5),     # 9 slices: a gather batch of 8 and one of 1
+    (2, 16, "fp64", 1e-12),
+    (3, 16, "fp32", 1e-5),     # 20 slices: batches of 8, 8, 4
+    (1, 8, "bf16", 5e-2),      # 5 slices of 8 columns
+])
+def test_many_slices_match_oracle(ctx, monkeypatch, degree, sw, precision, tol):
+    """Narrow column slices (CDG_APPLY_SLICE_W) exercise the multi-batch
+    gather and the accumulation-type partial sums; level and folded schedules
+    against the oracle."""
+    monkeypatch.setenv("CDG_APPLY_SLICE_W", str(sw))
+    p, blocks = synthetic_system(3, 2, 2, degree, seed=41)
+    A = upload(ctx, p, blocks)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    v = po.fill_vector(p.vec_len, 5)
+    z_ref = po.ilu0_apply(p, blocks, L_ref, D_ref, v)
+    for schedule in (("levels",) if precision == "bf16" else ("levels", "folded")):
+        M = precon_ILU0(precision, schedule=schedule).init(A)
+        shape = M.launch_shape()
+        assert shape["slice_width"] == sw and shape["slices"] == -(-p.n // sw)
+        M.update(A)
+        z = M.apply(A, v)
+        assert rel(z, z_ref) < tol, schedule
+        assert np.array_equal(M.apply(A, v), z)
+        M.tear_down()
