@@ -377,8 +377,8 @@ def main():
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one system per GPU; "
-              "sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (each system is solved on "
+              "one GPU)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         import torch

@@ -1,5 +1,5 @@
 """Parity of the gfx950 path (through the C ABI) with the CPU oracle and the
-reference's known answers.  Tolerances (BASELINE north star): matvec 1e-13
+reference's known answers.  Tolerances: matvec 1e-13
 relative, FP64 solution 1e-8 relative, iteration counts within +-1."""
 import numpy as np
 import pytest
