@@ -20,6 +20,9 @@ module mod_chidg_hip
     integer(c_int32_t), parameter, public :: HIP_PREC_IDENTITY = 0, HIP_PREC_ILU0 = 1
     integer(c_int32_t), parameter, public :: HIP_FP64 = 0, HIP_FP32 = 1, HIP_BF16 = 2
     integer(c_int32_t), parameter, public :: HIP_CGS = 0, HIP_MGS = 1
+    ! ILU(0) apply schedules (cdg_prec_set_schedule); AUTO is the default
+    integer(c_int32_t), parameter, public :: HIP_SCHEDULE_LEVELS = 0, HIP_SCHEDULE_DATAFLOW = 1, &
+                                             HIP_SCHEDULE_FOLDED = 2, HIP_SCHEDULE_AUTO = 3
 
     ! cdg_fgmres_params / cdg_fgmres_stats (same layout as the C structs)
     type, bind(c), public :: hip_fgmres_params_t
@@ -76,6 +79,22 @@ module mod_chidg_hip
             type(c_ptr), value :: ctx, mat
             real(c_double), intent(in) :: blocks(*)
         end function
+        integer(c_int) function cdg_mat_download(ctx, mat, blocks) bind(c, name='cdg_mat_download')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, mat
+            real(c_double), intent(inout) :: blocks(*)
+        end function
+        integer(c_int) function cdg_mat_tune(ctx, mat, reps, best_ms) bind(c, name='cdg_mat_tune')
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx, mat
+            integer(c_int32_t), value :: reps
+            real(c_double) :: best_ms
+        end function
+        integer(c_int) function cdg_ctx_mem_used(ctx, bytes) bind(c, name='cdg_ctx_mem_used')
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int64_t) :: bytes
+        end function
         integer(c_int) function cdg_mv(ctx, mat, x, y) bind(c, name='cdg_mv')
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx, mat
@@ -98,6 +117,17 @@ module mod_chidg_hip
             type(c_ptr), value :: ctx, prec, mat
             real(c_double), intent(in) :: v(*)
             real(c_double), intent(inout) :: z(*)
+        end function
+        integer(c_int) function cdg_prec_set_schedule(ctx, prec, schedule) bind(c, name='cdg_prec_set_schedule')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx, prec
+            integer(c_int32_t), value :: schedule
+        end function
+        integer(c_int) function cdg_prec_tune(ctx, prec, mat, reps, best_ms) bind(c, name='cdg_prec_tune')
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx, prec, mat
+            integer(c_int32_t), value :: reps
+            real(c_double) :: best_ms
         end function
         integer(c_int) function cdg_prec_destroy(ctx, prec) bind(c, name='cdg_prec_destroy')
             import :: c_int, c_ptr
@@ -131,6 +161,7 @@ module mod_chidg_hip
         type(c_ptr) :: h = c_null_ptr
     contains
         procedure :: init => ctx_init
+        procedure :: mem_used => ctx_mem_used
         procedure :: release => ctx_release
     end type
 
@@ -142,6 +173,8 @@ module mod_chidg_hip
     contains
         procedure :: init => mat_init
         procedure :: store => mat_store       ! upload + stamp: chidg_matrix%assemble
+        procedure :: fetch => mat_fetch       ! device blocks back to the host
+        procedure :: tune => mat_tune         ! matvec launch-shape autotuner
         procedure :: release => mat_release
     end type
 
@@ -156,6 +189,8 @@ module mod_chidg_hip
         procedure :: init => prec_init
         procedure :: update => prec_update
         procedure :: apply => prec_apply
+        procedure :: set_schedule => prec_set_schedule
+        procedure :: tune => prec_tune        ! apply launch-shape autotuner (after update)
         procedure :: tear_down => prec_tear_down
     end type
 
@@ -205,6 +240,14 @@ contains
         call hip_check(cdg_ctx_create(dev, self%h), 'cdg_ctx_create', c_null_ptr)
     end subroutine
 
+    !> Bytes of device memory the context holds (matrices, factors, Krylov bases).
+    function ctx_mem_used(self) result(bytes)
+        class(hip_context_t), intent(in) :: self
+        integer(c_int64_t) :: bytes
+        bytes = 0
+        call hip_check(cdg_ctx_mem_used(self%h, bytes), 'cdg_ctx_mem_used', self%h)
+    end function
+
     subroutine ctx_release(self)
         class(hip_context_t), intent(inout) :: self
         integer(c_int) :: st
@@ -231,6 +274,22 @@ contains
         real(c_double), intent(in) :: blocks(:)
         call hip_check(cdg_mat_upload(self%ctx, self%h, blocks), 'cdg_mat_upload', self%ctx)
     end subroutine
+
+    subroutine mat_fetch(self, blocks)
+        class(hip_matrix_t), intent(in) :: self
+        real(c_double), intent(inout) :: blocks(:)
+        call hip_check(cdg_mat_download(self%ctx, self%h, blocks), 'cdg_mat_download', self%ctx)
+    end subroutine
+
+    !> Times matvec launch shapes (reps calls each) and keeps the fastest for this
+    !! matrix and later ones of the same block size; returns its ms per call.
+    function mat_tune(self, reps) result(best_ms)
+        class(hip_matrix_t), intent(inout) :: self
+        integer, intent(in) :: reps
+        real(c_double) :: best_ms
+        best_ms = 0.d0
+        call hip_check(cdg_mat_tune(self%ctx, self%h, int(reps, c_int32_t), best_ms), 'cdg_mat_tune', self%ctx)
+    end function
 
     subroutine mat_release(self)
         class(hip_matrix_t), intent(inout) :: self
@@ -273,6 +332,25 @@ contains
         real(c_double), allocatable :: z(:)
         allocate(z(size(v)))
         call hip_check(cdg_prec_apply(A%ctx, self%h, A%h, v, z), 'cdg_prec_apply', A%ctx)
+    end function
+
+    subroutine prec_set_schedule(self, schedule)
+        class(hip_precon_t), intent(inout) :: self
+        integer, intent(in) :: schedule
+        call hip_check(cdg_prec_set_schedule(self%ctx, self%h, int(schedule, c_int32_t)), 'cdg_prec_set_schedule', &
+                       self%ctx)
+    end subroutine
+
+    !> Times apply launch shapes on the current factor (reps calls each) and keeps
+    !! the fastest for this preconditioner and later ones of the same block size
+    !! and precision; returns its ms per call.  Needs a factor (update first).
+    function prec_tune(self, A, reps) result(best_ms)
+        class(hip_precon_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        integer, intent(in) :: reps
+        real(c_double) :: best_ms
+        best_ms = 0.d0
+        call hip_check(cdg_prec_tune(A%ctx, self%h, A%h, int(reps, c_int32_t), best_ms), 'cdg_prec_tune', A%ctx)
     end function
 
     subroutine prec_tear_down(self)

@@ -13,6 +13,7 @@ program known_answers
     type(hip_precon_t)  :: M, I3, I2
     type(hip_fgmres_t)  :: ls
     real(c_double), allocatable :: z(:), x(:)
+    real(c_double) :: blk(7), t_apply, t_mv
     integer :: nfail = 0
 
     call ctx%init(0)
@@ -20,12 +21,26 @@ program known_answers
     ! --- ILU0: A = [[1,-2,0],[2,-5,12],[0,2,-10]], b = [1,2,3] ---------------
     call A%init(ctx, 3, 1, [0, 2, 5, 7], [1, 0, 0, 2, 1, 1, 2], [0, 0, 0, 0, 0, 0, 0])
     call A%store([-2.d0, 1.d0, 2.d0, 12.d0, -5.d0, 2.d0, -10.d0])
+    call A%fetch(blk)
+    call check('mat_fetch', all(blk == [-2.d0, 1.d0, 2.d0, 12.d0, -5.d0, 2.d0, -10.d0]))
+    call check('mem_used', ctx%mem_used() > 0)
     call M%init(A, HIP_PREC_ILU0, HIP_FP64)
     call M%update(A)
     z = M%apply(A, [1.d0, 2.d0, 3.d0])
     call check('ilu0_apply', maxval(abs(z - [43.d0/7.d0, 18.d0/7.d0, 3.d0/14.d0])) < 1.d-12)
     call check('ilu0_apply_reference_tol', abs(z(1) - 6.142857143d0) < 1.d-5 .and. &
                abs(z(2) - 2.57142857d0) < 1.d-5 .and. abs(z(3) - 0.2142857143d0) < 1.d-5)
+    ! the same answer with Dinv folded into the upper blocks, then under tuned launch shapes
+    call M%set_schedule(HIP_SCHEDULE_FOLDED)
+    z = M%apply(A, [1.d0, 2.d0, 3.d0])
+    call check('ilu0_apply_folded', maxval(abs(z - [43.d0/7.d0, 18.d0/7.d0, 3.d0/14.d0])) < 1.d-12)
+    call M%set_schedule(HIP_SCHEDULE_LEVELS)
+    t_apply = M%tune(A, 2)
+    t_mv = A%tune(2)
+    z = M%apply(A, [1.d0, 2.d0, 3.d0])
+    call check('ilu0_apply_tuned', t_apply > 0.d0 .and. t_mv > 0.d0 .and. &
+               maxval(abs(z - [43.d0/7.d0, 18.d0/7.d0, 3.d0/14.d0])) < 1.d-12)
+    call M%set_schedule(HIP_SCHEDULE_AUTO)
 
     ! --- FGMRES 3x3: A = [[4,3,0],[3,4,-1],[0,-1,4]], b = [24,30,-24] --------
     call A%store([3.d0, 4.d0, 3.d0, -1.d0, 4.d0, -1.d0, 4.d0])
