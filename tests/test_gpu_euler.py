@@ -102,3 +102,44 @@ def test_euler_p2_8cube(ctx):
             assert rel(x, x_ref) < 1e-8
         else:
             assert rel(x, x_ref) < 1e-5
+
+
+@pytest.mark.parametrize("orth,inner", [("MGS", False), ("CGS", True), ("MGS", True)])
+def test_euler_fgmres_orthogonalization_and_inner(ctx, config1, orth, inner):
+    """b = -R with modified Gram-Schmidt, and with the reference's default inner
+    FGMRES (nkrylov 100, tol and rtol 1e-1, preconditioned by the same ILU(0))
+    on each outer direction: FP64 against the oracle with the same parameters."""
+    p, blocks, R = config1
+    A = chidg_matrix(p, ctx)
+    A.store(blocks)
+    b = -R
+    ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, orthogonalization=orth, inner_fgmres=inner)
+    x = np.zeros(p.vec_len)
+    ls.solve(A, x, b, precon_ILU0())
+    x_ref, n_ref, _, _ = po.fgmres(p, blocks, b, nkrylov=30, tol=0.0, rtol=1e-8, orthogonalization=orth,
+                                   inner_fgmres=inner)
+    assert abs(ls.niter - n_ref) <= 1
+    assert rel(x, x_ref) < 1e-8
+    assert ls.residual_norm / np.linalg.norm(b) < 1e-8
+
+
+def test_euler_p3_3cube(ctx):
+    """3^3 P3 (320 x 320 blocks: the global-memory block inverse and its
+    partial-pivoting fallback on Euler diagonal blocks), RHS N(0,1), 58 oracle
+    iterations (one restart of FGMRES(30)): FP64 against the oracle, FP32 ILU to
+    the same residual."""
+    p, blocks, _, _ = euler.system(3, 3, mach=0.3, seed=3, perturb=0.01, cfl=1.0)
+    A = chidg_matrix(p, ctx)
+    A.store(blocks)
+    b = np.random.default_rng(13).standard_normal(p.vec_len)
+    x_ref, n_ref, _, _ = po.fgmres(p, blocks, b, nkrylov=30, tol=0.0, rtol=1e-8)
+    for precision in ("fp64", "fp32"):
+        ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False)
+        x = np.zeros(p.vec_len)
+        ls.solve(A, x, b, precon_ILU0(precision))
+        assert ls.residual_norm / np.linalg.norm(b) < 1e-8
+        if precision == "fp64":
+            assert abs(ls.niter - n_ref) <= 1
+            assert rel(x, x_ref) < 1e-8
+        else:
+            assert rel(x, x_ref) < 1e-5
