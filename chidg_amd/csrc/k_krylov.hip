@@ -31,15 +31,26 @@ int nparts(int64_t len) {
     return (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, (len + per - 1) / per));
 }
 
+constexpr int NWAVE = RT / 64;
+
+// Sum over the 64 lanes of a wave (butterfly: every lane gets the total).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup, returned to every thread: wave sums, then the
+// NWAVE wave totals in wave order (sh holds NWAVE values).  A caller that
+// reuses sh synchronizes before the next call.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int s = RT / 2; s > 0; s >>= 1) {
-        if (tid < s) sh[tid] += sh[tid + s];
-        __syncthreads();
-    }
-    return sh[0];
+    double s = sh[0];
+#pragma unroll
+    for (int q = 1; q < NWAVE; ++q) s += sh[q];
+    return s;
 }
 
 // part[i*nb + bx] = sum over chunk bx of V_i . w        (dot, operator_chidg_dot.f90:30-89)
@@ -48,7 +59,6 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 __global__ void __launch_bounds__(RT)
 k_dot_part(int64_t len, const double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
            double *__restrict__ part, const double *__restrict__ gate) {
-    __shared__ double sh[RT];
     if (gate && *gate == 0.0) return;   // a pass the host-side rule skips
     const int nb = gridDim.x, i0 = 8 * blockIdx.y;
     const int ni = min(8, j - i0);
@@ -83,15 +93,26 @@ k_dot_part(int64_t len, const double *__restrict__ w, const double *__restrict__
                 if (u < ni) acc[u] += V[(size_t)(i0 + u) * vstride + k] * wk;
         }
     }
-    for (int u = 0; u < ni; ++u) {
-        const double s = block_sum(acc[u], sh);
-        if (threadIdx.x == 0) part[(size_t)(i0 + u) * nb + blockIdx.x] = s;
-        __syncthreads();
+    // the ni workgroup sums at once: wave sums, then thread u adds the wave totals of vector u
+    __shared__ double shw[NWAVE][8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (u < ni) {
+            const double t = wave_sum(acc[u]);
+            if ((threadIdx.x & 63) == 0) shw[threadIdx.x >> 6][u] = t;
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < ni) {
+        const int u = threadIdx.x;
+        double t = shw[0][u];
+#pragma unroll
+        for (int q = 1; q < NWAVE; ++q) t += shw[q][u];
+        part[(size_t)(i0 + u) * nb + blockIdx.x] = t;
     }
 }
 
 __global__ void __launch_bounds__(RT) k_sumsq_part(int64_t len, const double *__restrict__ w, double *__restrict__ part) {
-    __shared__ double sh[RT];
+    __shared__ double sh[NWAVE];
     double acc = 0.0;
     for (int64_t k = blockIdx.x * (int64_t)RT + threadIdx.x; k < len; k += (int64_t)gridDim.x * RT) acc += w[k] * w[k];
     const double s = block_sum(acc, sh);
@@ -101,7 +122,7 @@ __global__ void __launch_bounds__(RT) k_sumsq_part(int64_t len, const double *__
 // out[i] = sum_b part[i*nb + b]
 __global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, int nb, double *__restrict__ out,
                                                const double *__restrict__ gate = nullptr) {
-    __shared__ double sh[RT];
+    __shared__ double sh[NWAVE];
     if (gate && *gate == 0.0) return;
     const int i = blockIdx.x;
     double acc = 0.0;
@@ -117,7 +138,7 @@ __global__ void __launch_bounds__(RT) k_reduce(const double *__restrict__ part, 
 __global__ void __launch_bounds__(RT)
 k_cgs_update(int64_t len, double *__restrict__ w, const double *__restrict__ V, int64_t vstride, int j,
              const double *__restrict__ h, double *__restrict__ part, const double *__restrict__ gate) {
-    __shared__ double sh[RT];
+    __shared__ double sh[NWAVE];
     if (gate && *gate == 0.0) return;
     double acc = 0.0;
     const bool pairs = ((vstride | (int64_t)(reinterpret_cast<uintptr_t>(V) >> 3) |
