@@ -1137,11 +1137,11 @@ int orthogonalize(Solver &S, Workspace &ws, int orth, int jj, double *w, double 
         }
         if (!e) e = kv_sumsq(ctx, len, w, d_h + jj + 1);
     } else {
+        // the gated pass's flag comes with the first update's norm (the same launches)
+        double *flag = gated ? d_h + jj + 2 : nullptr, *h2 = d_h + jj + 3;
         e = kv_multidot(ctx, len, w, ws.V, len, normb2 ? jj + 1 : jj, d_h);
-        if (!e) e = kv_cgs_update(ctx, len, w, ws.V, len, jj, d_h, d_h + jj + 1);
+        if (!e) e = kv_cgs_update(ctx, len, w, ws.V, len, jj, d_h, d_h + jj + 1, nullptr, flag);
         if (gated) {
-            double *flag = d_h + jj + 2, *h2 = d_h + jj + 3;
-            if (!e) e = kv_reorth_gate(ctx, d_h, jj, flag);
             if (!e) e = kv_multidot(ctx, len, w, ws.V, len, jj, h2, flag);
             if (!e) e = kv_cgs_update(ctx, len, w, ws.V, len, jj, h2, h2 + jj, flag);
         }

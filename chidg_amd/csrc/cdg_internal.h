@@ -171,9 +171,10 @@ int launch_widen(cdg_ctx *ctx, const void *src, int precision, double *dst, int6
 int kv_multidot(cdg_ctx *ctx, int64_t len, const double *w, const double *V, int64_t vstride, int j,
                 double *d_out /* j doubles */, const double *d_gate = nullptr);
 int kv_sumsq(cdg_ctx *ctx, int64_t len, const double *w, double *d_out);
-// w = w - sum_i h_i V_i (i ascending), h on device; also d_nrm2 = ||w_new||^2 when non-null
+// w = w - sum_i h_i V_i (i ascending), h on device; also d_nrm2 = ||w_new||^2 when non-null.
+// d_flag (may be null) receives kv_reorth_gate(d_h, j) computed in the same launches.
 int kv_cgs_update(cdg_ctx *ctx, int64_t len, double *w, const double *V, int64_t vstride, int j,
-                  const double *d_h, double *d_nrm2, const double *d_gate = nullptr);
+                  const double *d_h, double *d_nrm2, const double *d_gate = nullptr, double *d_flag = nullptr);
 // *d_flag = 1.0 when sum_{i<jj} |d_h[i]| / sqrt(d_h[jj]) >= 1 (second CGS pass), else 0.0
 int kv_reorth_gate(cdg_ctx *ctx, const double *d_h, int jj, double *d_flag);
 int kv_axpy_dev(cdg_ctx *ctx, int64_t len, double *w, const double *v, const double *d_h);

@@ -185,15 +185,33 @@ k_cgs_update(int64_t len, double *__restrict__ w, const double *__restrict__ V, 
     }
 }
 
-// flag = 1 when the selective re-orthogonalization rule asks for a second
-// pass: sum_{i<jj} |h_i| / sqrt(w.w before) >= 1 (type_fgmres.f90:212-223),
-// the same operations in the same order as the host's test.
-__global__ void k_reorth_gate(const double *__restrict__ h, int jj, double *__restrict__ flag) {
-    if (threadIdx.x != 0) return;
+// 1 when the selective re-orthogonalization rule asks for a second pass:
+// sum_{i<jj} |h_i| / sqrt(w.w before) >= 1 (type_fgmres.f90:212-223), the
+// same operations in the same order as the host's test.
+__device__ __forceinline__ double reorth_flag(const double *h, int jj) {
     double crit = 0.0;
     for (int i = 0; i < jj; ++i) crit += fabs(h[i]);
     crit /= sqrt(h[jj]);
-    *flag = crit >= 1.0 ? 1.0 : 0.0;
+    return crit >= 1.0 ? 1.0 : 0.0;
+}
+
+__global__ void k_reorth_gate(const double *__restrict__ h, int jj, double *__restrict__ flag) {
+    if (threadIdx.x != 0) return;
+    *flag = reorth_flag(h, jj);
+}
+
+// k_reduce of one result, which also writes the re-orthogonalization flag of
+// h[0..jj] (final before the update that produced the partials ran).
+__global__ void __launch_bounds__(RT) k_reduce_gate(const double *__restrict__ part, int nb, double *__restrict__ out,
+                                                    const double *__restrict__ h, int jj, double *__restrict__ flag) {
+    __shared__ double sh[NWAVE];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RT) acc += part[b];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        *out = s;
+        *flag = reorth_flag(h, jj);
+    }
 }
 
 __global__ void k_axpy_dev(int64_t len, double *__restrict__ w, const double *__restrict__ v, const double *__restrict__ h) {
@@ -270,13 +288,18 @@ int kv_sumsq(cdg_ctx *ctx, int64_t len, const double *w, double *d_out) {
 }
 
 int kv_cgs_update(cdg_ctx *ctx, int64_t len, double *w, const double *V, int64_t vstride, int j, const double *d_h,
-                  double *d_nrm2, const double *d_gate) {
+                  double *d_nrm2, const double *d_gate, double *d_flag) {
     const int nb = nparts(len);
     int e = ctx_ensure_part(ctx, (size_t)nb);
     if (e) return e;
     hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(RT), 0, ctx->stream, len, w, V, vstride, j, d_h,
                        d_nrm2 ? ctx->d_part : nullptr, d_gate);
-    if (d_nrm2) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_nrm2, d_gate);
+    if (d_nrm2 && d_flag && !d_gate) {
+        hipLaunchKernelGGL(k_reduce_gate, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_nrm2, d_h, j, d_flag);
+    } else {
+        if (d_nrm2) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, ctx->stream, ctx->d_part, nb, d_nrm2, d_gate);
+        if (d_flag) hipLaunchKernelGGL(k_reorth_gate, dim3(1), dim3(64), 0, ctx->stream, d_h, j, d_flag);
+    }
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
