@@ -11,9 +11,10 @@ Inputs are resident in HBM when timing starts.
 
 ``value`` is the FP32-ILU form's solves/s (ILU factor + apply in FP32, Arnoldi,
 residual and solution in FP64); the plain FP64 form is measured beside it.
-The roofline object describes the ILU(0) apply (the metric's "ILU apply
-GB/s") in the schedule a long solve settles in (Dinv folded into the upper
-blocks), timed with HIP events on the library's own stream.  The CPU baseline
+The BF16-stored-ILU form is measured the same way.  The roofline object
+describes the ILU(0) apply (the metric's "ILU apply GB/s") in the schedule a
+long solve settles in (the stream schedule: one persistent launch over row
+slices of the factor), timed with HIP events on the library's own stream.  The CPU baseline
 runs the C oracle (the reference algorithm restated) on the host cores, on the
 same system.
 """
@@ -41,16 +42,14 @@ SEED_Q = 1          # state perturbation
 SEED_B = 1          # right-hand side
 SEED_A = 20261202   # synthetic systems (P3 section, synthetic P2 section)
 HBM_PEAK_GBS = 8000.0
-# The FP32 ILU(0) apply in the folded schedule (W = U Dinv), one call: 46
-# forward and 46 backward level launches of k_kslice<float,float,4,5,*> and one
-# k_dinv_z<float,float,4,5> (z = Dinv t over all rows), 93 launches (default
-# shape: 5 slices of 32 columns).
-APPLY_KERNEL = ("ILU(0) apply FP32, folded schedule: 92 x k_kslice<float,float,4,5,*> + "
-                "k_dinv_z<float,float,4,5> (one call, 93 launches)")
-# HBM bytes per apply call from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) on the
-# Euler 16^3 P2 system with the default shape, the factors being what the same counters
-# read for a 512 MiB device copy (profiles/hbm_traffic_apply_folded_euler_16x16x16_p2.md).
-APPLY_TRAFFIC_FP32 = 2.260e9
+# The FP32 ILU(0) apply in the stream schedule, one call = one launch of the
+# persistent row-slice sweep k_rs_sweep<float,...> (both triangular sweeps,
+# 40,960 tasks of (block row, 27-row slice)).
+APPLY_KERNEL = "ILU(0) apply FP32, stream schedule: k_rs_sweep<float,4,R,E> (one launch per call)"
+# HBM bytes per apply call from rocprofv3 FETCH_SIZE + WRITE_SIZE on the Euler 16^3 P2
+# system, each scaled by what the same counter reads for a 512 MiB device copy; None
+# until measured for this kernel.
+APPLY_TRAFFIC_FP32 = None
 
 
 def sig(v, digits=5):
@@ -113,10 +112,15 @@ def time_first(ctx, fn):
     return e0.elapsed_time(e1) / 1e3
 
 
-def measure(args, p, A, b, steps, warmup, kernels=True):
-    """FP32-ILU and FP64 forms of one system: solves/s over ``steps`` timed
-    solves (default schedule), and with ``kernels`` the apply under each
-    schedule, the fold cost and the matvec, timed on the library stream."""
+ITEMSIZE = {"fp64": 8, "fp32": 4, "bf16": 2}
+
+
+def measure(args, p, A, b, steps, warmup, kernels=True, precisions=("fp32", "fp64", "bf16")):
+    """The FP32-ILU, FP64 and BF16-stored-ILU forms of one system: solves/s
+    over ``steps`` timed solves (default schedule), and with ``kernels`` the
+    apply under the level, stream and folded schedules, the stream schedule's
+    preparation (W = Dinv U and the row-slice copies) and the matvec, timed on
+    the library stream."""
     import torch
     from chidg_amd.linsolve import chidg_mv_async, fgmres, precon_ILU0
     ctx = A.ctx
@@ -124,7 +128,7 @@ def measure(args, p, A, b, steps, warmup, kernels=True):
     out = {}
     if args.tune:
         A.tune(reps=5)
-    for prec in ("fp32", "fp64"):
+    for prec in precisions:
         M = precon_ILU0(prec, schedule=args.schedule).init(A)
         ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False, orthogonalization="CGS")
         x = torch.zeros_like(b)
@@ -132,7 +136,8 @@ def measure(args, p, A, b, steps, warmup, kernels=True):
             ls.solve(A, x, b, M)
         tune = {}
         if args.tune:
-            # launch-shape autotuner on the live factor (setup, outside the timed region)
+            # launch-shape autotuner of the level schedule on the live factor
+            # (setup, outside the timed region)
             if not warmup:
                 M.update(A)
             t0 = time.perf_counter()
@@ -155,30 +160,44 @@ def measure(args, p, A, b, steps, warmup, kernels=True):
                  t_factor_ms=1e3 * st.t_factor, t_precon_ms=1e3 * st.t_precon, t_mv_ms=1e3 * st.t_mv,
                  t_dot_ms=1e3 * st.t_dot, t_norm_ms=1e3 * st.t_norm, wall_s=wall, x=x, **tune)
         if kernels:
-            nbytes = ilu_apply_bytes(p, 4 if prec == "fp32" else 8)
+            nbytes = ilu_apply_bytes(p, ITEMSIZE[prec])
             v = b / bnorm
             z = torch.empty_like(b)
             y = torch.empty_like(b)
+            torch.cuda.synchronize()
+            # inputs are complete (synchronized above): enqueue without cross-stream waits
+            app = lambda: M.apply_async(A, v, z, order=False)   # noqa: E731
             M.update(A)
             M.set_schedule("levels")
-            t_lv = time_async(ctx, lambda: M.apply_async(A, v, z), args.kreps)
-            M.set_schedule("folded")            # W formed by the first apply after the factor
-            t_first = time_first(ctx, lambda: M.apply_async(A, v, z))
-            t_fd = time_async(ctx, lambda: M.apply_async(A, v, z), args.kreps)
-            M.set_schedule(args.schedule or "auto")
-            t_mv = time_async(ctx, lambda: chidg_mv_async(A, v, y), args.kreps)
+            t_lv = time_async(ctx, app, args.kreps)
+            M.set_schedule("stream")            # W = Dinv U and the row-slice copies at the first apply
+            t_first = time_first(ctx, app)
+            t_st = time_async(ctx, app, args.kreps)
             r.update(apply_levels_ms=1e3 * t_lv, apply_levels_gbs=nbytes / t_lv / 1e9,
-                     apply_folded_ms=1e3 * t_fd, apply_folded_gbs=nbytes / t_fd / 1e9,
-                     fold_ms=1e3 * (t_first - t_fd), mv_ms=1e3 * t_mv, mv_gbs=mv_bytes(p) / t_mv / 1e9)
+                     apply_stream_ms=1e3 * t_st, apply_stream_gbs=nbytes / t_st / 1e9,
+                     stream_prep_ms=1e3 * (t_first - t_st))
+            if prec != "bf16" and args.folded:
+                M.update(A)
+                M.set_schedule("folded")
+                t_ff = time_first(ctx, app)
+                t_fd = time_async(ctx, app, args.kreps)
+                r.update(apply_folded_ms=1e3 * t_fd, apply_folded_gbs=nbytes / t_fd / 1e9,
+                         fold_ms=1e3 * (t_ff - t_fd))
+            M.set_schedule(args.schedule or "auto")
+            t_mv = time_async(ctx, lambda: chidg_mv_async(A, v, y, order=False), args.kreps)
+            r.update(mv_ms=1e3 * t_mv, mv_gbs=mv_bytes(p) / t_mv / 1e9)
         out[prec] = r
         log(f"{prec}: {r['solves_per_s']:.2f} solves/s, niter {st.niter}, res {r['residual_rel']:.2e}, "
             f"factor {r['t_factor_ms']:.2f} ms, precon {r['t_precon_ms']:.2f} ms, mv {r['t_mv_ms']:.2f} ms"
-            + (f"; apply levels {r['apply_levels_ms']:.3f} ms ({r['apply_levels_gbs']:.0f} GB/s), folded "
-               f"{r['apply_folded_ms']:.3f} ms ({r['apply_folded_gbs']:.0f} GB/s), fold {r['fold_ms']:.2f} ms, "
-               f"mv {r['mv_ms']:.3f} ms ({r['mv_gbs']:.0f} GB/s)" if kernels else ""))
+            + (f"; apply levels {r['apply_levels_ms']:.3f} ms ({r['apply_levels_gbs']:.0f} GB/s), stream "
+               f"{r['apply_stream_ms']:.3f} ms ({r['apply_stream_gbs']:.0f} GB/s), prep {r['stream_prep_ms']:.2f} "
+               f"ms, mv {r['mv_ms']:.3f} ms ({r['mv_gbs']:.0f} GB/s)" if kernels else ""))
         M.tear_down()
-    x64, x32 = out["fp64"].pop("x"), out["fp32"].pop("x")
-    out["fp32"]["x_rel_vs_fp64"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    x64 = out["fp64"].pop("x")
+    for prec in precisions:
+        if prec != "fp64":
+            xp = out[prec].pop("x")
+            out[prec]["x_rel_vs_fp64"] = float(torch.linalg.norm(xp - x64) / torch.linalg.norm(x64))
     out["mv_shape"] = A.launch_shape()
     return out, x64
 
@@ -207,7 +226,7 @@ def synthetic_run(args):
     A = chidg_matrix(p, ctx)
     A.fill_synthetic(SEED_A, 1.0 / np.sqrt(n), 0.1 / np.sqrt(n))
     b = A.synthetic_vector(SEED_B, 1.0, like=torch.empty(0, dtype=torch.float64, device="cuda:0"))
-    out, _ = measure(args, p, A, b, 5, 1, kernels=False)
+    out, _ = measure(args, p, A, b, 5, 1, kernels=False, precisions=("fp32", "fp64"))
     A.release()
     return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in out.items()}
 
@@ -322,13 +341,24 @@ def _mv(A, x):
 
 
 def cpu_baseline(p, blocks, b, x_gpu):
-    """Oracle (reference algorithm restated in C, OpenMP over rows of a
-    wavefront level and over matvec rows) on the same system: ILU(0) update +
-    FGMRES(30) to rtol 1e-8, repeated until about 10 s of work (at most 4
-    solves).  Also the GPU FP64 solution's distance from the oracle's."""
+    """The oracle (the reference algorithm restated in C) on the same system,
+    ILU(0) update + FGMRES(30) to rtol 1e-8, on the host's cores:
+
+    * all cores of this process's CPU share (OpenMP over the rows of a
+      wavefront level and over matvec rows; the order of every sum is the
+      sequential one, so iteration counts do not change): full solves, repeated
+      until about 10 s of work (at most 4);
+    * one core (the reference at one MPI rank, the only rank count whose ILU(0)
+      is the full ILU(0)): one ILU(0) update and the first 8 FGMRES
+      iterations, timed; the solve time is the update plus 8 iterations'
+      time scaled to the all-core run's iteration count (a bounded sample: the
+      full one-core solve takes about a minute).
+
+    Also the GPU FP64 solution's distance from the oracle's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    share = len(os.sched_getaffinity(0))
+    cores = min(share, 16)
     po.set_threads(cores)
     tf = ts = 0.0
     runs = 0
@@ -345,12 +375,31 @@ def cpu_baseline(p, blocks, b, x_gpu):
     total = (tf + ts) / runs
     log(f"cpu oracle ({cores} threads, {runs} solves): factor {tf / runs:.2f} s, fgmres {ts / runs:.2f} s "
         f"({niter} it)")
+    po.set_threads(1)
+    t0 = time.perf_counter()
+    L, D = po.ilu0_factor(p, blocks)
+    t1 = time.perf_counter()
+    k1 = 8
+    po.fgmres(p, blocks, b, factors=(L, D), nkrylov=30, tol=0.0, rtol=1e-8, nmax=k1)
+    t2 = time.perf_counter()
+    del L, D
+    one_factor, one_iter = t1 - t0, (t2 - t1) / k1
+    one_total = one_factor + niter * one_iter
+    log(f"cpu oracle (1 thread): factor {one_factor:.2f} s, {one_iter * 1e3:.0f} ms per FGMRES iteration "
+        f"-> {one_total:.1f} s per {niter}-iteration solve")
+    po.set_threads(cores)
     parity = dict(oracle_niter=niter, oracle_residual_rel=res / float(np.linalg.norm(b)),
                   gpu_fp64_x_rel_vs_oracle=float(np.linalg.norm(x_gpu - x) / np.linalg.norm(x)))
     return dict(value=round(1.0 / total, 4), unit="solves/s", cores=cores, kind="port",
                 sample=f"{runs} full solve(s) of the same Euler 16^3 P2 system (ILU0 update {tf / runs:.2f} s + "
                        f"FGMRES(30) {niter} iterations {ts / runs:.2f} s each), C oracle with {cores} OpenMP "
-                       f"threads"), parity
+                       f"threads (all cores of the CPU share: sched_getaffinity {share}, os.cpu_count "
+                       f"{os.cpu_count()})",
+                nproc=os.cpu_count(), cpu_share=share,
+                one_core={"value": round(1.0 / one_total, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+                          "sample": f"one ILU0 update ({one_factor:.2f} s) + the first {k1} FGMRES(30) iterations "
+                                    f"({one_iter * 1e3:.0f} ms each) with 1 thread; solve = update + {niter} "
+                                    f"iterations"}), parity
 
 
 def main():
@@ -360,7 +409,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kreps", type=int, default=20, help="kernel timing repetitions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--schedule", choices=["levels", "dataflow", "folded", "auto"], default=None,
+    ap.add_argument("--schedule", choices=["levels", "dataflow", "folded", "stream", "auto"], default=None,
                     help="ILU(0) apply schedule (default: the library's)")
     ap.add_argument("--extras", default="synthetic,p3,batch,serve",
                     help="comma list of extra sections: synthetic (seeded synthetic 16^3 P2 system), p3 "
@@ -373,6 +422,8 @@ def main():
     ap.add_argument("--batch-slots", type=int, default=6, help="request-loop slots (contexts) for the batch section")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the launch-shape autotuner (run before each timed section, outside the timed region)")
+    ap.add_argument("--folded", action="store_true",
+                    help="also time the folded level schedule (W = U Dinv) of the FP64 and FP32 forms")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -411,9 +462,10 @@ def main():
         parity["gpu_fp64_niter"] = f64["niter"]
         log(f"parity: GPU FP64 {f64['niter']} it vs oracle {parity['oracle_niter']} it, "
             f"x rel {parity['gpu_fp64_x_rel_vs_oracle']:.2e}")
-    # the apply a long solve settles in: the folded schedule
-    t_ap = f32["apply_folded_ms"] / 1e3
+    # the apply a long solve settles in: the stream schedule
+    t_ap = f32["apply_stream_ms"] / 1e3
     gbs = ilu_apply_bytes(p, 4) / t_ap / 1e9
+    bf = g.get("bf16")
     roof = {"bound": "hbm", "kernel": APPLY_KERNEL,
             "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -432,10 +484,13 @@ def main():
                    "nrows": p.nrows, "block_n": p.n, "nnzb": p.nnzb, "seed_state": SEED_Q, "seed_b": SEED_B},
         "fp64": {k: sig(v) for k, v in f64.items()},
         "fp32_ilu": {k: sig(v) for k, v in f32.items()},
+        "bf16_ilu": {k: sig(v) for k, v in bf.items()} if bf else None,
         "mv_shape": g.get("mv_shape"),
         "speedup_fp32_vs_fp64": round(f32["solves_per_s"] / f64["solves_per_s"], 3),
-        "ilu_apply_gbs_fp64": round(ilu_apply_bytes(p, 8) / (f64["apply_folded_ms"] / 1e3) / 1e9, 1),
+        "speedup_bf16_vs_fp64": round(bf["solves_per_s"] / f64["solves_per_s"], 3) if bf else None,
+        "ilu_apply_gbs_fp64": round(ilu_apply_bytes(p, 8) / (f64["apply_stream_ms"] / 1e3) / 1e9, 1),
         "ilu_apply_gbs_fp32": round(gbs, 1),
+        "ilu_apply_gbs_bf16": round(ilu_apply_bytes(p, 2) / (bf["apply_stream_ms"] / 1e3) / 1e9, 1) if bf else None,
         "latency_ms": {"p50": round(f32["p50_ms"], 3), "p99": round(f32["p99_ms"], 3)},
         "parity_vs_oracle": parity,
         "roofline": roof,

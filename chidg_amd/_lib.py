@@ -21,6 +21,7 @@ CGS, MGS = 0, 1
 
 EXPORTS = [
     "cdg_ctx_create", "cdg_ctx_destroy", "cdg_last_error", "cdg_ctx_mem_used", "cdg_ctx_stream",
+    "cdg_ctx_wait_stream", "cdg_ctx_signal_stream",
     "cdg_mat_create", "cdg_mat_destroy", "cdg_mat_upload", "cdg_mat_download",
     "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
     "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
@@ -93,6 +94,8 @@ def load() -> C.CDLL:
         "cdg_last_error": ([vp], C.c_char_p),
         "cdg_ctx_mem_used": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "cdg_ctx_stream": ([vp, pp], C.c_int),
+        "cdg_ctx_wait_stream": ([vp, vp], C.c_int),
+        "cdg_ctx_signal_stream": ([vp, vp], C.c_int),
         "cdg_mv_async": ([vp, vp, dp, dp], C.c_int),
         "cdg_prec_apply_async": ([vp, vp, vp, dp, dp], C.c_int),
         "cdg_mat_create": ([vp, C.c_int32, C.c_int32, ip, ip, ip, pp], C.c_int),

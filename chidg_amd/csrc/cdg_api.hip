@@ -370,6 +370,8 @@ extern "C" int cdg_ctx_create(int32_t device, cdg_ctx **out) {
     }
     if (const char *sp = std::getenv("CDG_SPIN")) ctx->spin = std::atoi(sp) != 0;
     if (ctx_ensure_small(ctx, 4096) || ctx_ensure_part(ctx, 1 << 16)) { cdg_ctx_destroy(ctx); return CDG_ERR_OOM; }
+    rs_ctx_count(+1);
+    ctx->counted = true;
     *out = ctx;
     return CDG_OK;
 }
@@ -386,7 +388,9 @@ extern "C" int cdg_ctx_destroy(cdg_ctx *ctx) {
     ctx->alloc_size.clear();
     ctx_flush_cache(ctx);
     if (ctx->done_ev) (void)hipEventDestroy(ctx->done_ev);
+    if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->counted) rs_ctx_count(-1);
     delete ctx;
     return CDG_OK;
 }
@@ -403,6 +407,26 @@ extern "C" int cdg_ctx_stream(const cdg_ctx *ctx, void **stream) {
     if (!ctx || !stream) return CDG_ERR_ARG;
     *stream = static_cast<void *>(ctx->stream);
     return CDG_OK;
+}
+
+// Cross-stream ordering without a host wait: an event recorded on one stream,
+// waited on by the other.
+static int stream_order(cdg_ctx *ctx, hipStream_t before, hipStream_t after) {
+    CDG_HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->order_ev) CDG_HIP_TRY(hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming));
+    CDG_HIP_TRY(hipEventRecord(ctx->order_ev, before));
+    CDG_HIP_TRY(hipStreamWaitEvent(after, ctx->order_ev, 0));
+    return CDG_OK;
+}
+
+extern "C" int cdg_ctx_wait_stream(cdg_ctx *ctx, void *stream) {
+    if (!ctx) return CDG_ERR_ARG;
+    return stream_order(ctx, static_cast<hipStream_t>(stream), ctx->stream);
+}
+
+extern "C" int cdg_ctx_signal_stream(cdg_ctx *ctx, void *stream) {
+    if (!ctx) return CDG_ERR_ARG;
+    return stream_order(ctx, ctx->stream, static_cast<hipStream_t>(stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -701,8 +725,13 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
         else if (!std::strcmp(m, "auto")) M->apply_mode = CDG_APPLY_AUTO;
         else if (!std::strcmp(m, "levels")) M->apply_mode = CDG_APPLY_LEVELS;
         else if (!std::strcmp(m, "folded") && precision != CDG_BF16) M->apply_mode = CDG_APPLY_FOLDED;
+        else if (!std::strcmp(m, "stream")) M->apply_mode = CDG_APPLY_STREAM;
     }
     M->h_lev_rows = forder;
+    M->h_lo_ptr = A->lo_ptr;
+    M->h_lo_col = A->lo_col;
+    M->h_up_ptr = A->up_ptr;
+    M->h_up_col = A->up_col;
     M->h_blev_rows = border;
     // (+2: a sweep launch whose rows have no entries issues its 16-byte
     // block prefetch from the start of a partial array, unused)
@@ -711,8 +740,8 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     M->part_cap = part_doubles;
     if (!e && M->apply_mode == CDG_APPLY_DATAFLOW) e = df_setup(ctx, M);
     if (!e && M->apply_mode == CDG_APPLY_FOLDED) e = alloc(&M->W, es * (size_t)M->nup * M->ld);
+    if (!e && M->apply_mode == CDG_APPLY_STREAM) e = rs_setup(ctx, M);
     if (const char *f = std::getenv("CDG_FOLD_AFTER")) M->fold_after = std::max(0, std::atoi(f));
-    if (precision == CDG_BF16) M->fold_after = 0;
     if (!e) {
         hipError_t he = hipStreamSynchronize(ctx->stream);
         if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
@@ -725,11 +754,16 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
 extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule) {
     if (!ctx || !M) return CDG_ERR_ARG;
     if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW && schedule != CDG_SCHEDULE_FOLDED &&
-        schedule != CDG_SCHEDULE_AUTO)
+        schedule != CDG_SCHEDULE_AUTO && schedule != CDG_SCHEDULE_STREAM)
         return cdg_fail(ctx, CDG_ERR_ARG, "unknown apply schedule %d", schedule);
     if (M->type != CDG_PREC_ILU0) return CDG_OK;
     if (schedule == CDG_SCHEDULE_DATAFLOW) {
         int e = df_setup(ctx, M);
+        if (e) return e;
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    if (schedule == CDG_SCHEDULE_STREAM) {   // storage now; W and the copies at the next apply
+        int e = rs_setup(ctx, M);
         if (e) return e;
         CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
@@ -745,6 +779,7 @@ extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule
     }
     M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW
                     : schedule == CDG_SCHEDULE_FOLDED ? CDG_APPLY_FOLDED
+                    : schedule == CDG_SCHEDULE_STREAM ? CDG_APPLY_STREAM
                     : schedule == CDG_SCHEDULE_AUTO ? CDG_APPLY_AUTO : CDG_APPLY_LEVELS;
     return CDG_OK;
 }
@@ -921,6 +956,15 @@ extern "C" int cdg_mat_launch_shape(const cdg_mat *A, int32_t *out) {
 // A dataflow sweep that gave up waiting leaves its counters unusable: report
 // it and reset them (called from the synchronizing entry points).
 static int df_check(cdg_ctx *ctx, cdg_prec *M) {
+    if (M->d_rs_err) {   // the stream sweep: an abandoned wait left NaN in z
+        int32_t err = 0;
+        CDG_HIP_TRY(hipMemcpyAsync(&err, M->d_rs_err, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (err) {
+            CDG_HIP_TRY(hipMemsetAsync(M->d_rs_err, 0, sizeof(int32_t), ctx->stream));
+            return cdg_fail(ctx, CDG_ERR_HIP, "ILU(0) stream sweep gave up waiting for a dependency");
+        }
+    }
     if (M->apply_mode != CDG_APPLY_DATAFLOW || !M->d_dferr) return CDG_OK;
     int32_t err = 0;
     CDG_HIP_TRY(hipMemcpyAsync(&err, M->d_dferr, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
@@ -938,6 +982,7 @@ extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int3
     if (M->type == CDG_PREC_IDENTITY) { M->stamp = A->stamp; M->factored = 1; return CDG_OK; }
     M->factored = 0;
     M->w_ready = 0;
+    M->rs_ready = 0;
     M->prev_napply = M->napply;
     M->napply = 0;
     int e = launch_ilu_factor(ctx, M, A, bad_row);
@@ -947,15 +992,35 @@ extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int3
     return CDG_OK;
 }
 
+// The stream schedule's storage for the current factor: W = Dinv U and the
+// row-slice copies of L and Dinv (once per factor).
+static int rs_prepare(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
+    if (M->rs_ready) return CDG_OK;
+    int e = rs_setup(ctx, M);
+    if (!e) e = launch_fold_rs(ctx, M, A);
+    if (!e) e = rs_relayout(ctx, M);
+    if (!e) M->rs_ready = 1;
+    return e;
+}
+
 static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z) {
     const int64_t len = (int64_t)M->nrows * M->n;
     if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len, v, z);
     if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
     if (M->apply_mode == CDG_APPLY_DATAFLOW) return launch_ilu_apply_df(ctx, M, A, v, z);
     ++M->napply;
-    const bool fold_now = !M->w_ready && (M->apply_mode == CDG_APPLY_FOLDED ||   // schedule chosen after the factor
-                                          (M->apply_mode == CDG_APPLY_AUTO && M->fold_after > 0 &&
-                                           (M->napply > M->fold_after || M->prev_napply > M->fold_after)));
+    const bool auto_long = M->apply_mode == CDG_APPLY_AUTO && M->fold_after > 0 &&
+                           (M->napply > M->fold_after || M->prev_napply > M->fold_after) && M->rs_ok;
+    if (M->apply_mode == CDG_APPLY_STREAM || auto_long) {
+        int e = rs_prepare(ctx, M, A);
+        if (e && M->apply_mode == CDG_APPLY_AUTO && e == CDG_ERR_UNSUPPORTED) {
+            M->rs_ok = 0;   // this pattern or block size: the level schedule from now on
+        } else {
+            if (e) return e;
+            return launch_ilu_apply_rs(ctx, M, v, z);
+        }
+    }
+    const bool fold_now = !M->w_ready && M->apply_mode == CDG_APPLY_FOLDED;   // schedule chosen after the factor
     if (fold_now) {
         if (!M->W) {
             const size_t bytes = std::max<size_t>(16, (M->precision == CDG_FP64 ? 8 : 4) * (size_t)M->nup * M->ld);

@@ -49,7 +49,9 @@ struct cdg_ctx {
     size_t stage_cap[3] = {0, 0, 0};
     // completion event of ctx_wait (the solver's per-step waits poll it)
     hipEvent_t done_ev = nullptr;
+    hipEvent_t order_ev = nullptr;     // cdg_ctx_wait_stream / cdg_ctx_signal_stream
     bool spin = true;
+    bool counted = false;              // included in the live-context count (stream sweep grid)
 };
 
 // Waits until the context's stream has finished its work: polls a completion
@@ -125,6 +127,7 @@ struct cdg_prec {
     int32_t *d_blev_rows = nullptr;
     int32_t nblev = 0;
     std::vector<int32_t> h_lev_rows, h_blev_rows;   // host copies of the level orders
+    std::vector<int32_t> h_lo_ptr, h_lo_col, h_up_ptr, h_up_col;   // host copies of the kind-0 lists
     int32_t maxdep = 1;               // max lower/upper blocks of a row
     // apply launch shape: each block's columns are cut into `slices` ranges of
     // `slice_w` columns; shape[precision] is the fixed-row walk for that element
@@ -143,6 +146,21 @@ struct cdg_prec {
     int32_t *d_dferr = nullptr;        // raised by a sweep that gave up waiting
     int32_t df_epoch = 0;              // calls since the counters were zeroed
     int32_t df_spin = 1 << 21;         // polls before a dependency wait gives up
+    // stream schedule (k_rsapply.hip): the factor in the row-slice layout
+    // (slices of rs_w rows, slot pitch rs_slot elements), W = Dinv U, the task
+    // list and the tagged exchange buffers
+    void *Lrs = nullptr, *Wrs = nullptr, *Drs = nullptr;
+    int32_t *rs_tasks = nullptr;       // device task records
+    int32_t *d_up_row = nullptr;       // row of each upper entry
+    void *rs_yb = nullptr, *rs_zb = nullptr;
+    int32_t *d_rs_err = nullptr;
+    uint32_t rs_epoch = 0;             // tag of the last call
+    int32_t rs_ready = 0;              // Lrs/Wrs/Drs hold the current factor
+    int32_t rs_ok = 1;                 // 0: the pattern does not fit the stream schedule (AUTO keeps levels)
+    int32_t rs_w = 0, rs_S = 0, rs_T = 0, rs_dc = 0, rs_R = 0, rs_EPT = 0, rs_ntasks = 0, rs_per_cu = 0;
+    int32_t rs_spin = 1 << 20;
+    int64_t rs_SL = 0, rs_slot = 0;
+    size_t rs_lds = 0;
     uint64_t stamp = 0;
     int32_t factored = 0;
     std::vector<DevMem> owned;
@@ -154,7 +172,16 @@ int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *
 int launch_ilu_apply_df(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 int df_configure(cdg_ctx *ctx, cdg_prec *M);
 int df_reset(cdg_ctx *ctx, cdg_prec *M);
-enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2, CDG_APPLY_AUTO = 3 };
+enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2, CDG_APPLY_AUTO = 3, CDG_APPLY_STREAM = 4 };
+// stream schedule (k_rsapply.hip; the fold in k_factor.hip)
+int rs_setup(cdg_ctx *ctx, cdg_prec *M);
+int rs_relayout(cdg_ctx *ctx, cdg_prec *M);
+int launch_fold_rs(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A);
+int launch_ilu_apply_rs(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
+int launch_ilu_apply_rs_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb,
+                            int32_t *err, uint32_t tag);
+int rs_slice_rows(int n, int wmax);
+void rs_ctx_count(int delta);
 // W_u = A_{up_blk[u]} Dinv_{up_col[u]} for all upper blocks (after a factor)
 int launch_fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A);
 // Uniform hex-box pattern in ChiDG row order (cdg_serve.hip); periodic: bit d

@@ -219,6 +219,9 @@ __global__ void __launch_bounds__(kDfMaxThreads, 2) k_df_apply(DfArgs a) {
                 if ((spins & 255) == 0 && ld_co(a.err) != 0) { abort = 1; break; }
                 __builtin_amdgcn_s_sleep(1);
             }
+            // acquire: the partial loads below are ordered after the counter
+            // observation (invalidates this CU's vector L1, not the L2)
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (lane == 0) sh_abort = abort;
         }
         __syncthreads();
@@ -273,7 +276,8 @@ __global__ void __launch_bounds__(kDfMaxThreads, 2) k_df_apply(DfArgs a) {
             for (int q = 0; q < a.dc; ++q) s += red[r + n * q];
             st_co(out + r, static_cast<double>(s));
         }
-        // The partial was written with agent-coherent stores: waiting for
+        // The partial was written with agent-scope atomic stores, which write
+        // through this XCD's L2 to the device-coherent level: waiting for
         // their completion orders them before the counter increment.  (An
         // agent-scope release fence would also write back the whole L2 of
         // this XCD on every task, which serializes the sweep.)

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 namespace {
 
@@ -243,6 +244,45 @@ k_fac_p2(const double *__restrict__ Ab, int64_t lda, const SB *__restrict__ Ub, 
     T *out = D + (size_t)i * ld;
     gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) {
         out[(size_t)col * n + row] = static_cast<T>(a[(size_t)col * n + row]) - v;
+    });
+}
+
+// Storage conversion of a GEMM result (BF16: round to nearest even).
+template <typename SO> __device__ __forceinline__ SO fold_store(double v);
+template <> __device__ __forceinline__ double fold_store<double>(double v) { return v; }
+template <> __device__ __forceinline__ float fold_store<float>(double v) { return static_cast<float>(v); }
+template <> __device__ __forceinline__ cdg_bf16 fold_store<cdg_bf16>(double v) {
+    uint32_t u = __float_as_uint(static_cast<float>(v));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<cdg_bf16>(u >> 16);
+}
+
+// Row-slice fold: W_u = Dinv_{up_row[u]} * U_u for every upper entry u (the
+// backward sweep's z_i = Dinv_i y_i - sum W_ik z_k), written straight into
+// the row-slice layout of the stream schedule: row r of the block goes to
+// slice q = r / w at element q*SL + col*w + (r - q*w) of the slot.  U_u is
+// read from Ub slot up_src[u] (A's FP64 blocks, or their FP32 copies).
+template <typename T, typename SB, typename SO>
+__global__ void __launch_bounds__(GT)
+k_fold_rs(const T *__restrict__ D, int64_t ld, const SB *__restrict__ Ub, int64_t ldu, const int32_t *__restrict__ up_row,
+          const int32_t *__restrict__ up_src, int nup, SO *__restrict__ W, int64_t slot, int w, int SL, int n,
+          int tiles) {
+    __shared__ __attribute__((aligned(16))) ATile<T> As[2];
+    __shared__ __attribute__((aligned(16))) BTile<T> Bs[2];
+    int job, tile;
+    if (!xcd_job(nup, tiles * tiles, job, tile)) return;
+    const int u = job;
+    const int tm = tile % tiles, tn = tile / tiles;
+    typename Mfma<T>::acc_t acc[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    const T *A = D + (size_t)up_row[u] * ld;
+    const SB *B = Ub + (size_t)up_src[u] * ldu;
+    gemm_pairs<T, T, SB>([&](int, const T *&a, const SB *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As, Bs, acc);
+    SO *out = W + (size_t)u * slot;
+    gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) {
+        const int q = row / w;
+        out[(size_t)q * SL + (size_t)col * w + (row - q * w)] = fold_store<SO>(static_cast<double>(v));
     });
 }
 
@@ -1270,21 +1310,24 @@ template <typename T, int NR>
 int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows, int nrows_lvl, int n,
                 int32_t *err) {
     const size_t lds = ginv_lds<T, NR>(n);
-    static bool attr_set = false;
-    if (!attr_set) {
-        CDG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_ginv<T, NR>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_set = lds <= 160 * 1024;
-    }
+    // Both kernels may take the LDS of the largest block this template serves
+    // (n <= 64 NR, at most 320): set once per process, thread-safe (request-loop
+    // workers launch concurrently), whatever block size comes first.
+    static std::once_flag once;
+    static hipError_t attr_err = hipSuccess;
+    std::call_once(once, [] {
+        const int most = (int)ginv_lds<T, NR>(std::min(64 * NR, 320));
+        attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_ginv<T, NR>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, most);
+        if (attr_err == hipSuccess)
+            attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_pivot<T, NR>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, most);
+    });
+    if (attr_err != hipSuccess)
+        return cdg_fail(ctx, CDG_ERR_HIP, "hipFuncSetAttribute (block inverse LDS): %s", hipGetErrorString(attr_err));
     hipLaunchKernelGGL((k_fac_ginv<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,
                        static_cast<T *>(M->inv_w), M->ld, rows, n, M->d_redo);
     CDG_HIP_TRY(hipGetLastError());
-    static bool attr_p = false;
-    if (!attr_p) {
-        CDG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_pivot<T, NR>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_p = lds <= 160 * 1024;
-    }
     hipLaunchKernelGGL((k_fac_pivot<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,
                        static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
     CDG_HIP_TRY(hipGetLastError());
@@ -1453,6 +1496,32 @@ int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_
         if (bad_row) *bad_row = err;
         return cdg_fail(ctx, CDG_ERR_SINGULAR, "singular diagonal block in row %d", err);
     }
+    return CDG_OK;
+}
+
+// W_u = Dinv_{row(u)} U_u in the row-slice layout (stream schedule), one
+// batched launch over the upper entries.  FP64: A's blocks and the FP64 Dinv;
+// FP32: the upper blocks' FP32 copies and Dinv; BF16: A's FP64 blocks (their
+// values rounded to FP32 at the LDS store) and the FP32 working Dinv, the
+// product stored BF16.
+int launch_fold_rs(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
+    const int n = M->n, tiles = (n + TM - 1) / TM, nup = M->nup;
+    if (nup == 0) return CDG_OK;
+    const dim3 grid(xcd_grid(nup, tiles * tiles)), blk(GT);
+    const int w = M->rs_w, SL = (int)M->rs_SL;
+    if (M->precision == CDG_FP64)
+        hipLaunchKernelGGL((k_fold_rs<double, double, double>), grid, blk, 0, ctx->stream,
+                           static_cast<const double *>(M->Dinv), M->ld, A->blocks, A->ld, M->d_up_row, M->d_up_blk, nup,
+                           static_cast<double *>(M->Wrs), M->rs_slot, w, SL, n, tiles);
+    else if (M->precision == CDG_FP32)
+        hipLaunchKernelGGL((k_fold_rs<float, float, float>), grid, blk, 0, ctx->stream,
+                           static_cast<const float *>(M->Dinv), M->ld, static_cast<const float *>(M->U), M->ld,
+                           M->d_up_row, M->d_up_all, nup, static_cast<float *>(M->Wrs), M->rs_slot, w, SL, n, tiles);
+    else
+        hipLaunchKernelGGL((k_fold_rs<float, double, cdg_bf16>), grid, blk, 0, ctx->stream, M->D32, M->ld, A->blocks,
+                           A->ld, M->d_up_row, M->d_up_blk, nup, static_cast<cdg_bf16 *>(M->Wrs), M->rs_slot, w, SL, n,
+                           tiles);
+    CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
 

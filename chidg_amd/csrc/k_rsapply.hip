@@ -1,0 +1,611 @@
+// ILU(0) apply as one persistent sweep over row slices (precon_ILU0.f90:286-411).
+//
+//   forward   y_i = v_i - sum_{j<i} L_ij y_j                 (precon_ILU0.f90:336-359)
+//   backward  z_i = Dinv_i y_i - sum_{k>i} W_ik z_k,  W_ik = Dinv_i U_ik
+//             (the reference's z_i -= U_ik z_k followed by z_i = Dinv_i z_i,
+//              precon_ILU0.f90:363-398, with Dinv_i multiplied in beforehand)
+//
+// Storage ("row-slice" layout, written once per factor by the fold and
+// relayout kernels): every n x n block is cut into S = n / w slices of w
+// rows; slice s holds rows s w .. s w + w - 1 of all n columns, column-major
+// with leading dimension w, at element offset s * SL of the block's slot
+// (SL = w n rounded up to 8 elements, so every slice starts 16-byte aligned).
+// A task (row i, slice s) therefore streams one contiguous range per block
+// and produces final values of rows s w .. s w + w - 1 of y_i or z_i: no
+// partial sums travel between tasks.
+//
+// Scheduling: tasks are listed in a topological order (forward levels, then
+// backward levels) and dealt round-robin to the G workgroups of one launch
+// (task t to workgroup t mod G), each of which takes its tasks in order.  G
+// never exceeds the number of workgroups the device holds at once, so all of
+// them are resident: the lowest unfinished task's inputs are finished and its
+// workgroup has nothing before it, so the sweep always progresses.  Each wait
+// is bounded nevertheless: past the limit a workgroup raises the error word,
+// writes NaN into the outputs of its remaining backward tasks and leaves; the
+// others leave at their next check.
+//
+// Exchange between tasks: every value of y and z sits in a 64-bit word
+// together with a 32-bit tag, the number of the apply call that wrote it
+// (FP64 values take two words, each half with the tag).  A consumer loads the
+// parent words with device-coherent (agent-scope) atomic loads and accepts
+// them once every tag is the current call's; a 64-bit aligned access is
+// single-copy atomic, so value and tag are seen together.  No counters, no
+// fences, nothing to reset between calls.
+//
+// Software pipeline per workgroup: the block slices of task t+G are loaded
+// into registers while task t reduces and stores, so a task that waits for
+// its parents already holds its blocks.
+#include "cdg_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+typedef double rs_f64x2 __attribute__((ext_vector_type(2)));
+typedef float rs_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t rs_u32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte raw vector of each storage type, its V elements and their
+// accumulation type
+template <typename S> struct Raw16;
+template <> struct Raw16<double> {
+    typedef rs_f64x2 T;
+    typedef double Acc;
+    static constexpr int V = 2;
+    __device__ static __forceinline__ void get(const T &r, double (&o)[2]) { o[0] = r.x; o[1] = r.y; }
+};
+template <> struct Raw16<float> {
+    typedef rs_f32x4 T;
+    typedef float Acc;
+    static constexpr int V = 4;
+    __device__ static __forceinline__ void get(const T &r, float (&o)[4]) {
+        o[0] = r.x; o[1] = r.y; o[2] = r.z; o[3] = r.w;
+    }
+};
+template <> struct Raw16<cdg_bf16> {
+    typedef rs_u32x4 T;
+    typedef float Acc;
+    static constexpr int V = 8;
+    __device__ static __forceinline__ void get(const T &r, float (&o)[8]) {
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            o[2 * i] = __uint_as_float(w[i] << 16);
+            o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+        }
+    }
+};
+
+__device__ __forceinline__ uint64_t ld_tag(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_tag(uint64_t *p, uint64_t w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Tagged value formats: Words 64-bit words per value.
+template <typename VT> struct Tagged;
+template <> struct Tagged<float> {
+    static constexpr int Words = 1;
+    __device__ static __forceinline__ void put(uint64_t *p, double v, uint32_t tag) {
+        st_tag(p, ((uint64_t)tag << 32) | __float_as_uint(static_cast<float>(v)));
+    }
+    // value of the loaded words; ok = every word carries `tag`
+    __device__ static __forceinline__ float get(const uint64_t (&w)[1], uint32_t tag, bool &ok) {
+        ok = (uint32_t)(w[0] >> 32) == tag;
+        return __uint_as_float((uint32_t)w[0]);
+    }
+};
+template <> struct Tagged<double> {
+    static constexpr int Words = 2;
+    __device__ static __forceinline__ void put(uint64_t *p, double v, uint32_t tag) {
+        const uint64_t b = (uint64_t)__double_as_longlong(v);
+        st_tag(p, ((uint64_t)tag << 32) | (b & 0xffffffffull));
+        st_tag(p + 1, ((uint64_t)tag << 32) | (b >> 32));
+    }
+    __device__ static __forceinline__ double get(const uint64_t (&w)[2], uint32_t tag, bool &ok) {
+        ok = (uint32_t)(w[0] >> 32) == tag && (uint32_t)(w[1] >> 32) == tag;
+        return __longlong_as_double((long long)((w[0] & 0xffffffffull) | (w[1] << 32)));
+    }
+};
+
+struct RsArgs {
+    const int32_t *tasks;     // kRsRec ints per task (see rs_build_tasks)
+    int32_t ntasks;
+    const void *Lb, *Wb, *Db; // row-slice slots: L (forward), W (backward), Dinv (backward block 0)
+    int64_t slot;             // elements per slot
+    int32_t SL;               // elements per slice
+    int32_t n, w, dc;         // block size, rows per slice, columns per walk step
+    const double *v;          // the vector preconditioned
+    double *z;                // the result
+    uint64_t *yb, *zb;        // tagged forward / backward results
+    uint32_t tag;
+    int32_t *err;
+    int32_t spin_limit;
+};
+
+}  // namespace
+
+// Task record: [0] row, [1] slice | nblocks << 8 | backward << 16,
+// [2 + k] slot of block k, [2 + kRsMaxBlocks + k] parent row of block k.
+// Backward tasks: block 0 is the row's Dinv (parent: the row's own y), blocks
+// 1.. the W of its upper entries (parents: their z).
+constexpr int kRsMaxBlocks = 7;
+constexpr int kRsRec = 16;
+constexpr int kRsMaxThreads = 512;
+
+namespace {
+
+template <typename S, int NB, int R, int EPT>
+__global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
+    typedef Raw16<S> RW;
+    typedef typename RW::Acc Acc;
+    typedef typename RW::T RawT;
+    typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type VT;
+    typedef Tagged<VT> TG;
+    constexpr int V = RW::V, WD = TG::Words;
+    extern __shared__ unsigned char smem_raw[];
+    const int T = blockDim.x, tid = threadIdx.x, n = a.n, w = a.w;
+    Acc *xs = reinterpret_cast<Acc *>(smem_raw);                    // kRsMaxBlocks * n
+    Acc *red = xs + (((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3);  // V*T
+    const int lim = w * n;             // elements of one block slice
+    const int step = V * T;            // = dc * w
+    const int lim_ld = (lim - 1) / V * V;   // last 16-byte group starting inside the slice
+    int col0[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) col0[v] = (V * tid + v) / w;
+    const S *Lb = static_cast<const S *>(a.Lb), *Wb = static_cast<const S *>(a.Wb), *Db = static_cast<const S *>(a.Db);
+    const int G = gridDim.x;
+    int t = blockIdx.x;
+    if (t >= a.ntasks) return;
+
+    auto slice_base = [&](const int32_t *rec, int k) -> const S * {
+        const int nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1, sl = rec[1] & 0xff;
+        const int kk = k < nb ? k : 0;
+        const S *base = bwd ? (kk == 0 ? Db : Wb) : Lb;
+        return base + (size_t)rec[2 + kk] * a.slot + (size_t)sl * a.SL;
+    };
+    RawT raw[NB][R];
+    // Every load is issued, none under a branch (a branch here would make the
+    // compiler wait for all outstanding loads at the join): a block the task
+    // does not have reads the first 16 bytes of block 0, the same address in
+    // every lane (one request per wave), and steps past the slice re-read its
+    // last 16-byte group.
+    auto issue = [&](const int32_t *rec) {
+        const int nb = (rec[1] >> 8) & 0xff;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const S *b = slice_base(rec, k);
+#pragma unroll
+            for (int m = 0; m < R; ++m)
+                raw[k][m] = __builtin_nontemporal_load(
+                    reinterpret_cast<const RawT *>(b + (k < nb ? min(V * tid + m * step, lim_ld) : 0)));
+        }
+    };
+    issue(a.tasks + (size_t)t * kRsRec);
+    int aborted = 0;
+    for (;;) {
+        const int32_t *rec = a.tasks + (size_t)t * kRsRec;
+        const int row = rec[0];
+        const int sl = rec[1] & 0xff, nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1;
+        const int tn = t + G;
+        // ---- 1. parents into LDS: nb vectors of n values, all tags current
+        {
+            // element j = k n + c (thread's j = tid + i T) is value c of parent
+            // k: word offset off[i] in y (forward parents and the backward
+            // row's own y, j < n) or z (backward upper parents, j >= n); the
+            // backward row's own y enters with +, all others with -
+            const int ne = nb * n;
+            // the parents are wave-uniform (scalar loads); each lane picks its
+            // own by selects, not by a per-lane load from the record
+            int par[kRsMaxBlocks];
+#pragma unroll
+            for (int k = 0; k < kRsMaxBlocks; ++k) par[k] = rec[2 + kRsMaxBlocks + k];
+            int off[EPT];
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) {
+                // clamped into [0, ne): a task without parents (ne = 0) reads
+                // value 0 of row 0, unused
+                const int j = max(0, min(tid + i * T, ne - 1));
+                const int k = j / n, c = j - k * n;
+                int pk = par[0];
+#pragma unroll
+                for (int q = 1; q < kRsMaxBlocks; ++q) pk = k == q ? par[q] : pk;
+                off[i] = (pk * n + c) * WD;
+            }
+            uint32_t need = 0;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i)
+                if (tid + i * T < ne) need |= 1u << i;
+            int spins = 0;
+            for (;;) {
+                uint64_t wd[EPT][WD];
+#pragma unroll
+                for (int i = 0; i < EPT; ++i) {
+                    const uint64_t *src = (bwd && tid + i * T >= n ? a.zb : a.yb) + off[i];
+#pragma unroll
+                    for (int q = 0; q < WD; ++q) wd[i][q] = ld_tag(src + q);
+                }
+#pragma unroll
+                for (int i = 0; i < EPT; ++i) {
+                    bool ok;
+                    const VT x = TG::get(wd[i], a.tag, ok);
+                    if (ok && ((need >> i) & 1u)) {
+                        const int j = tid + i * T;
+                        xs[j] = (bwd && j < n) ? static_cast<Acc>(x) : -static_cast<Acc>(x);
+                        need &= ~(1u << i);
+                    }
+                }
+                if (!__syncthreads_or(need != 0)) break;
+                // bounded wait; the error word, set to this call's tag by the
+                // first workgroup that gives up, ends every other one (a word
+                // left by an earlier call holds another tag and stops nothing)
+                ++spins;
+                int stop = spins > a.spin_limit;
+                if (stop && tid == 0)
+                    __hip_atomic_store(a.err, (int32_t)a.tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((spins & 63) == 0)
+                    stop |= __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (int32_t)a.tag;
+                if (__syncthreads_or(stop)) { aborted = 1; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        if (aborted) break;
+        // ---- 2. products: this thread's elements of the nb block slices
+        Acc acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = 0;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (k < nb) {
+                const Acc *xk = xs + k * n;
+#pragma unroll
+                for (int m = 0; m < R; ++m) {
+                    const int e = V * tid + m * step;
+                    Acc x[V];
+                    RW::get(raw[k][m], x);
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * a.dc, n - 1)];
+                }
+            }
+        }
+        // slices longer than R steps, rows with more than NB blocks: the rest from memory
+        if (R * step < lim || nb > NB) {
+            for (int k = 0; k < nb; ++k) {
+                const S *b = slice_base(rec, k);
+                const Acc *xk = xs + k * n;
+                for (int m = k < NB ? R : 0; m * step < lim; ++m) {
+                    const int e = V * tid + m * step;
+                    if (e < lim) {
+                        Acc x[V];
+                        RW::get(__builtin_nontemporal_load(reinterpret_cast<const RawT *>(b + e)), x);
+#pragma unroll
+                        for (int v = 0; v < V; ++v)
+                            if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * a.dc, n - 1)];
+                    }
+                }
+            }
+        }
+        // ---- 3. the next task's blocks go out now (registers are free)
+        if (tn < a.ntasks) issue(a.tasks + (size_t)tn * kRsRec);
+        // ---- 4. row sums (fixed order) and the outputs of this task
+#pragma unroll
+        for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
+        __syncthreads();
+        for (int r = tid; r < w; r += T) {
+            Acc s = 0;
+            for (int q = 0; q < a.dc; ++q) s += red[r + w * q];
+            const size_t gi = (size_t)row * n + (size_t)sl * w + r;
+            if (!bwd) {
+                const double y = a.v[gi] + static_cast<double>(s);
+                TG::put(a.yb + gi * WD, y, a.tag);
+            } else {
+                const VT zz = static_cast<VT>(s);
+                TG::put(a.zb + gi * WD, static_cast<double>(zz), a.tag);
+                a.z[gi] = static_cast<double>(zz);
+            }
+        }
+        __syncthreads();
+        t = tn;
+        if (t >= a.ntasks) break;
+    }
+    if (aborted) {   // this workgroup's backward outputs from here on never come: NaN
+        for (; t < a.ntasks; t += G) {
+            const int32_t *rec = a.tasks + (size_t)t * kRsRec;
+            if (!((rec[1] >> 16) & 1)) continue;
+            const size_t g0 = (size_t)rec[0] * n + (size_t)(rec[1] & 0xff) * w;
+            for (int r = tid; r < w; r += T) a.z[g0 + r] = __builtin_nan("");
+        }
+    }
+}
+
+// ---- conversions to the storage type ----------------------------------------
+template <typename SO> __device__ __forceinline__ SO to_store(double v);
+template <> __device__ __forceinline__ double to_store<double>(double v) { return v; }
+template <> __device__ __forceinline__ float to_store<float>(double v) { return static_cast<float>(v); }
+template <> __device__ __forceinline__ cdg_bf16 to_store<cdg_bf16>(double v) {
+    uint32_t u = __float_as_uint(static_cast<float>(v));
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return static_cast<cdg_bf16>(u >> 16);
+}
+
+// Column-major n x n slots -> row-slice slots (one workgroup per block).
+template <typename SI, typename SO>
+__global__ void __launch_bounds__(256) k_to_rs(const SI *__restrict__ src, int64_t ld_in, SO *__restrict__ dst,
+                                               int64_t slot, int count, int n, int w, int SL) {
+    const int nn = n * n;
+    for (int b = blockIdx.x; b < count; b += gridDim.x) {
+        const SI *s = src + (size_t)b * ld_in;
+        SO *d = dst + (size_t)b * slot;
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            const int c = e / n, r = e - c * n, q = r / w;
+            d[(size_t)q * SL + (size_t)c * w + (r - q * w)] = to_store<SO>(static_cast<double>(s[e]));
+        }
+    }
+}
+
+}  // namespace
+
+// ---- host side ------------------------------------------------------------------
+// Rows per slice: a divisor of n (equal slices) no larger than wmax.
+int rs_slice_rows(int n, int wmax) {
+    int best = 1;
+    for (int d = 1; d <= n; ++d)
+        if (n % d == 0 && d <= wmax) best = d;
+    return best;
+}
+
+namespace {
+
+template <typename S>
+int R_needed(int lim, int step) { return (lim + step - 1) / step; }
+
+// kernel table: S x R in {2, 3, 4, 6, 8} x EPT in {2, 4, 8}, NB = 4
+template <typename S>
+const void *rs_kernel(int R, int EPT) {
+#define CDG_RS_K(RR, EE) reinterpret_cast<const void *>(&k_rs_sweep<S, 4, RR, EE>)
+#define CDG_RS_E(RR)                                  \
+    if (EPT <= 2) return CDG_RS_K(RR, 2);             \
+    if (EPT <= 4) return CDG_RS_K(RR, 4);             \
+    return CDG_RS_K(RR, 8);
+    if (R <= 2) { CDG_RS_E(2) }
+    if (R <= 3) { CDG_RS_E(3) }
+    if (R <= 4) { CDG_RS_E(4) }
+    if (R <= 6) { CDG_RS_E(6) }
+    CDG_RS_E(8)
+#undef CDG_RS_E
+#undef CDG_RS_K
+}
+
+int round_pow(int x) { return x <= 2 ? 2 : (x <= 4 ? 4 : 8); }
+int round_r(int x) { return x <= 2 ? 2 : x <= 3 ? 3 : x <= 4 ? 4 : x <= 6 ? 6 : 8; }
+
+std::atomic<int> g_live_sweepers{0};
+
+}  // namespace
+
+// Live contexts that may run persistent sweeps concurrently (each takes a
+// share of the device's resident workgroups, so concurrent sweeps on other
+// streams can all be resident at once).
+void rs_ctx_count(int delta) { g_live_sweepers += delta; }
+
+// Layout, task list, launch geometry and storage of the row-slice schedule
+// for M (once per preconditioner).
+int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
+    if (M->rs_tasks) return CDG_OK;
+    const int n = M->n, prec = M->precision;
+    if (M->maxdep + 1 > kRsMaxBlocks)
+        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream schedule: a row has more than %d lower or upper blocks",
+                        kRsMaxBlocks - 1);
+    const int V = prec == CDG_FP64 ? 2 : (prec == CDG_FP32 ? 4 : 8);
+    const size_t es = prec == CDG_FP64 ? 8 : (prec == CDG_FP32 ? 4 : 2);
+    const size_t accsz = prec == CDG_FP64 ? 8 : 4;
+    int wmax = prec == CDG_FP64 ? 16 : 32;
+    if (const char *e = std::getenv("CDG_RS_W")) wmax = std::max(1, std::atoi(e));
+    int target = 256;
+    if (const char *e = std::getenv("CDG_RS_THREADS")) target = std::max(64, std::atoi(e));
+    const int w = rs_slice_rows(n, wmax);
+    const StreamShape sh = stream_shape(w, V, target, kRsMaxThreads);
+    if (sh.T == 0) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream schedule: no launch shape for n=%d", n);
+    const int T = sh.T;
+    const int S = n / w;
+    const int64_t SL = ((int64_t)w * n + 7) / 8 * 8;
+    const int64_t slot = (S * SL + 63) / 64 * 64;
+    const int R = round_r(R_needed<int>(w * n, V * T));
+    const int EPT = round_pow((kRsMaxBlocks * n + T - 1) / T);
+    if ((kRsMaxBlocks * n + T - 1) / T > 8)
+        return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream schedule: %d threads cannot gather %d parents of n=%d", T,
+                        kRsMaxBlocks, n);
+    const void *fn = prec == CDG_FP64 ? rs_kernel<double>(R, EPT)
+                   : (prec == CDG_FP32 ? rs_kernel<float>(R, EPT) : rs_kernel<cdg_bf16>(R, EPT));
+    const size_t lds = accsz * ((((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3) + (size_t)V * T);
+    int per_cu = 0;
+    CDG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, T, lds));
+    if (per_cu < 1) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream sweep does not fit a CU");
+    // task list: forward levels, then backward levels; rows in level order, slices ascending
+    std::vector<int32_t> tasks;
+    tasks.reserve((size_t)2 * M->nrows * S * kRsRec);
+    auto push = [&](int row, int sl, bool bwd) {
+        int32_t rec[kRsRec] = {0};
+        rec[0] = row;
+        int nb = 0;
+        if (!bwd) {
+            for (int q = M->h_lo_ptr[row]; q < M->h_lo_ptr[row + 1]; ++q, ++nb) {
+                rec[2 + nb] = q;
+                rec[2 + kRsMaxBlocks + nb] = M->h_lo_col[q];
+            }
+        } else {
+            rec[2] = row;
+            rec[2 + kRsMaxBlocks] = row;
+            nb = 1;
+            for (int u = M->h_up_ptr[row]; u < M->h_up_ptr[row + 1]; ++u, ++nb) {
+                rec[2 + nb] = u;
+                rec[2 + kRsMaxBlocks + nb] = M->h_up_col[u];
+            }
+        }
+        rec[1] = sl | (nb << 8) | ((bwd ? 1 : 0) << 16);
+        tasks.insert(tasks.end(), rec, rec + kRsRec);
+    };
+    for (int l = 0; l < M->nlev; ++l)
+        for (int k = M->lev_row_ptr[l]; k < M->lev_row_ptr[l + 1]; ++k)
+            for (int sl = 0; sl < S; ++sl) push(M->h_lev_rows[k], sl, false);
+    for (int l = 0; l < M->nblev; ++l)
+        for (int k = M->blev_row_ptr[l]; k < M->blev_row_ptr[l + 1]; ++k)
+            for (int sl = 0; sl < S; ++sl) push(M->h_blev_rows[k], sl, true);
+    const int ntasks = (int)(tasks.size() / kRsRec);
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        bytes = std::max<size_t>(bytes, 256);
+        int e = dev_alloc(ctx, p, bytes);
+        if (!e) M->owned.push_back({*p, bytes});
+        if (!e) {
+            hipError_t he = hipMemsetAsync(*p, 0, bytes, ctx->stream);
+            if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
+        }
+        return e;
+    };
+    const int WD = prec == CDG_FP64 ? 2 : 1;
+    const size_t vec_words = (size_t)M->nrows * n * WD;
+    int e = alloc(&M->Lrs, es * (size_t)M->nlo * slot);
+    if (!e) e = alloc(&M->Wrs, es * (size_t)M->nup * slot);
+    if (!e) e = alloc(&M->Drs, es * (size_t)M->nrows * slot);
+    if (!e) e = alloc(&M->rs_yb, 8 * vec_words);
+    if (!e) e = alloc(&M->rs_zb, 8 * vec_words);
+    if (!e) e = alloc((void **)&M->d_rs_err, sizeof(int32_t));
+    if (!e) {
+        int32_t *dt = nullptr;
+        const size_t bytes = sizeof(int32_t) * std::max<size_t>(tasks.size(), 1);
+        e = dev_alloc(ctx, (void **)&dt, bytes);
+        if (!e) {
+            M->owned.push_back({dt, bytes});
+            hipError_t he = hipMemcpyAsync(dt, tasks.data(), sizeof(int32_t) * tasks.size(), hipMemcpyHostToDevice,
+                                           ctx->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
+            M->rs_tasks = dt;
+        }
+    }
+    if (!e) {   // the up entry -> row map (the fold's left operand)
+        std::vector<int32_t> up_row(std::max(M->nup, 1), 0);
+        for (int i = 0; i < M->nrows; ++i)
+            for (int u = M->h_up_ptr[i]; u < M->h_up_ptr[i + 1]; ++u) up_row[u] = i;
+        int32_t *d = nullptr;
+        const size_t bytes = sizeof(int32_t) * up_row.size();
+        e = dev_alloc(ctx, (void **)&d, bytes);
+        if (!e) {
+            M->owned.push_back({d, bytes});
+            hipError_t he = hipMemcpyAsync(d, up_row.data(), bytes, hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
+            M->d_up_row = d;
+        }
+    }
+    if (e) return e;
+    M->rs_w = w;
+    M->rs_S = S;
+    M->rs_SL = SL;
+    M->rs_slot = slot;
+    M->rs_T = T;
+    M->rs_dc = sh.dc;
+    M->rs_R = R;
+    M->rs_EPT = EPT;
+    M->rs_lds = lds;
+    M->rs_ntasks = ntasks;
+    M->rs_per_cu = per_cu;
+    M->rs_ready = 0;
+    M->rs_spin = 1 << 20;
+    if (const char *ev = std::getenv("CDG_RS_SPIN")) M->rs_spin = std::max(1, std::atoi(ev));
+    return CDG_OK;
+}
+
+// Resident workgroups for one sweep: the device's capacity shared by the
+// live contexts (CDG_RS_GRID caps it).
+static int rs_grid(cdg_ctx *ctx, const cdg_prec *M) {
+    const int live = std::max(1, g_live_sweepers.load());
+    int g = std::max(1, M->rs_per_cu * ctx->num_cus / live);
+    if (const char *e = std::getenv("CDG_RS_GRID")) g = std::max(1, std::min(g, std::atoi(e)));
+    return std::min(g, M->rs_ntasks);
+}
+
+// L and Dinv to the row-slice layout (the fold writes W itself).
+int rs_relayout(cdg_ctx *ctx, cdg_prec *M) {
+    const int n = M->n, w = M->rs_w, SL = (int)M->rs_SL;
+    const dim3 blk(256);
+    auto run = [&](const void *src, void *dst, int count) -> int {
+        if (count == 0) return CDG_OK;
+        const dim3 grid((unsigned)std::min(count, 65535));
+        if (M->precision == CDG_FP64)
+            hipLaunchKernelGGL((k_to_rs<double, double>), grid, blk, 0, ctx->stream, static_cast<const double *>(src),
+                               M->ld, static_cast<double *>(dst), M->rs_slot, count, n, w, SL);
+        else if (M->precision == CDG_FP32)
+            hipLaunchKernelGGL((k_to_rs<float, float>), grid, blk, 0, ctx->stream, static_cast<const float *>(src),
+                               M->ld, static_cast<float *>(dst), M->rs_slot, count, n, w, SL);
+        else
+            hipLaunchKernelGGL((k_to_rs<float, cdg_bf16>), grid, blk, 0, ctx->stream, static_cast<const float *>(src),
+                               M->ld, static_cast<cdg_bf16 *>(dst), M->rs_slot, count, n, w, SL);
+        CDG_HIP_TRY(hipGetLastError());
+        return CDG_OK;
+    };
+    const void *Lsrc = M->precision == CDG_BF16 ? static_cast<const void *>(M->L32) : M->L;
+    const void *Dsrc = M->precision == CDG_BF16 ? static_cast<const void *>(M->D32) : M->Dinv;
+    int e = run(Lsrc, M->Lrs, M->nlo);
+    if (!e) e = run(Dsrc, M->Drs, M->nrows);
+    return e;
+}
+
+namespace {
+template <typename S>
+int rs_launch_t(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb, int32_t *err,
+                uint32_t tag) {
+    RsArgs a{};
+    a.tasks = M->rs_tasks;
+    a.ntasks = M->rs_ntasks;
+    a.Lb = M->Lrs;
+    a.Wb = M->Wrs;
+    a.Db = M->Drs;
+    a.slot = M->rs_slot;
+    a.SL = (int32_t)M->rs_SL;
+    a.n = M->n;
+    a.w = M->rs_w;
+    a.dc = M->rs_dc;
+    a.v = v;
+    a.z = z;
+    a.yb = yb;
+    a.zb = zb;
+    a.tag = tag;
+    a.err = err;
+    a.spin_limit = M->rs_spin;
+    const void *fn = rs_kernel<S>(M->rs_R, M->rs_EPT);
+    void *args[] = {&a};
+    CDG_HIP_TRY(hipLaunchKernel(fn, dim3(rs_grid(ctx, M)), dim3(M->rs_T), args, M->rs_lds, ctx->stream));
+    return CDG_OK;
+}
+}  // namespace
+
+// One apply with the row-slice sweep on the given exchange buffers (words of
+// nrows*n tagged values each) and error word; `tag` must differ from every
+// tag the buffers hold (the caller's call counter).
+int launch_ilu_apply_rs_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb,
+                            int32_t *err, uint32_t tag) {
+    if (M->nrows == 0) return CDG_OK;
+    if (M->precision == CDG_FP64) return rs_launch_t<double>(ctx, M, v, z, yb, zb, err, tag);
+    if (M->precision == CDG_FP32) return rs_launch_t<float>(ctx, M, v, z, yb, zb, err, tag);
+    return rs_launch_t<cdg_bf16>(ctx, M, v, z, yb, zb, err, tag);
+}
+
+int launch_ilu_apply_rs(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z) {
+    if (M->rs_epoch == 0xffffffffu) {   // tags wrap: clear the buffers, restart at 1
+        const size_t words = (size_t)M->nrows * M->n * (M->precision == CDG_FP64 ? 2 : 1);
+        CDG_HIP_TRY(hipMemsetAsync(M->rs_yb, 0, 8 * words, ctx->stream));
+        CDG_HIP_TRY(hipMemsetAsync(M->rs_zb, 0, 8 * words, ctx->stream));
+        M->rs_epoch = 0;
+    }
+    M->rs_epoch += 1;
+    return launch_ilu_apply_rs_buf(ctx, M, v, z, static_cast<uint64_t *>(M->rs_yb), static_cast<uint64_t *>(M->rs_zb),
+                                   M->d_rs_err, M->rs_epoch);
+}

@@ -22,7 +22,8 @@ module mod_chidg_hip
     integer(c_int32_t), parameter, public :: HIP_CGS = 0, HIP_MGS = 1
     ! ILU(0) apply schedules (cdg_prec_set_schedule); AUTO is the default
     integer(c_int32_t), parameter, public :: HIP_SCHEDULE_LEVELS = 0, HIP_SCHEDULE_DATAFLOW = 1, &
-                                             HIP_SCHEDULE_FOLDED = 2, HIP_SCHEDULE_AUTO = 3
+                                             HIP_SCHEDULE_FOLDED = 2, HIP_SCHEDULE_AUTO = 3, &
+                                             HIP_SCHEDULE_STREAM = 4
 
     ! cdg_fgmres_params / cdg_fgmres_stats (same layout as the C structs)
     type, bind(c), public :: hip_fgmres_params_t
@@ -150,6 +151,11 @@ module mod_chidg_hip
             import :: hip_fgmres_params_t
             type(hip_fgmres_params_t), intent(inout) :: params
         end subroutine
+        integer(c_int) function cdg_mat_stamp(mat, stamp) bind(c, name='cdg_mat_stamp')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: mat
+            integer(c_int64_t) :: stamp
+        end function
         integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -175,7 +181,25 @@ module mod_chidg_hip
         procedure :: store => mat_store       ! upload + stamp: chidg_matrix%assemble
         procedure :: fetch => mat_fetch       ! device blocks back to the host
         procedure :: tune => mat_tune         ! matvec launch-shape autotuner
+        procedure :: stamp => mat_stamp       ! chidg_matrix_t%stamp: changes with every store
         procedure :: release => mat_release
+    end type
+
+    !> One block row in the layout of domain_matrix_t (type_domain_matrix.f90:32-79):
+    !! the lblks(ielem,itime) entries -- densematrix_vector_t%data_(:), each with
+    !! its local parent element eparent_l and its n x n matrix mat -- followed by
+    !! the coupling-only entries (chi_blks / bc_blks).  Parents are 1-based
+    !! element indices, as in the reference.
+    type, public :: hip_block_row_t
+        integer(c_int32_t), allocatable :: lparent(:)     ! lblks: eparent_l of data_(k)
+        real(c_double),     allocatable :: lmat(:,:,:)    ! lblks: data_(k)%mat = lmat(:,:,k)
+        integer(c_int32_t), allocatable :: cparent(:)     ! coupling entries: parent elements
+        real(c_double),     allocatable :: cmat(:,:,:)    ! coupling entries: matrices
+    end type
+
+    !> One element vector (densevector_t%vec, type_densevector.f90:21-69).
+    type, public :: hip_densevector_t
+        real(c_double), allocatable :: vec(:)
     end type
 
     !> Preconditioner (precon_ILU0_t / precon_identity_t).
@@ -185,6 +209,7 @@ module mod_chidg_hip
         integer(c_int32_t) :: ptype = HIP_PREC_ILU0
         integer(c_int32_t) :: precision = HIP_FP64
         logical :: initialized = .false.
+        integer(c_int64_t) :: stamp = 0       ! stamp of the matrix the factor was computed from
     contains
         procedure :: init => prec_init
         procedure :: update => prec_update
@@ -192,6 +217,20 @@ module mod_chidg_hip
         procedure :: set_schedule => prec_set_schedule
         procedure :: tune => prec_tune        ! apply launch-shape autotuner (after update)
         procedure :: tear_down => prec_tear_down
+    end type
+
+    !> solver_controller_t (type_solver_controller.f90:25-124): refactor when the
+    !! lhs changed since the last call, when forced, or when the preconditioner
+    !! was computed from another version of the lhs.
+    type, public :: hip_solver_controller_t
+        logical :: force_update_lhs = .false.
+        logical :: lhs_updated = .false.
+        logical :: force_update_preconditioner = .false.
+        integer(c_int64_t) :: lhs_stamp = 0
+        logical :: preconditioner_updated = .false.
+    contains
+        procedure :: update_lhs => ctl_update_lhs
+        procedure :: update_preconditioner => ctl_update_preconditioner
     end type
 
     !> fgmres_t: parameters as in linear_solver_t (type_linear_solver.f90:21-40).
@@ -206,6 +245,7 @@ module mod_chidg_hip
     end type
 
     public :: hip_mv, hip_check
+    public :: hip_pattern_from_rows, hip_pack_blocks, hip_get_packed, hip_set_packed
 
 contains
 
@@ -291,6 +331,13 @@ contains
         call hip_check(cdg_mat_tune(self%ctx, self%h, int(reps, c_int32_t), best_ms), 'cdg_mat_tune', self%ctx)
     end function
 
+    function mat_stamp(self) result(stamp)
+        class(hip_matrix_t), intent(in) :: self
+        integer(c_int64_t) :: stamp
+        stamp = 0
+        call hip_check(cdg_mat_stamp(self%h, stamp), 'cdg_mat_stamp', self%ctx)
+    end function
+
     subroutine mat_release(self)
         class(hip_matrix_t), intent(inout) :: self
         integer(c_int) :: st
@@ -323,6 +370,7 @@ contains
         type(hip_matrix_t), intent(in) :: A
         integer(c_int32_t) :: bad
         call hip_check(cdg_prec_update(A%ctx, self%h, A%h, bad), 'cdg_prec_update', A%ctx)
+        self%stamp = A%stamp()
     end subroutine
 
     function prec_apply(self, A, v) result(z)
@@ -366,19 +414,152 @@ contains
         call cdg_fgmres_default_params(self%params)
     end subroutine
 
-    !> solve(A,x,b,M): x overwritten (zero initial guess); M updated first
-    !! unless the controller semantics (params%update_preconditioner = 0)
-    !! find its factor current.
-    subroutine fgmres_solve(self, A, x, b, M)
+    !> solve(A,x,b,M[,solver_controller]): x overwritten (zero initial guess).
+    !! Without a controller M is updated first, as in the reference; with one,
+    !! when solver_controller%update_preconditioner says so
+    !! (type_fgmres.f90:118-122).
+    subroutine fgmres_solve(self, A, x, b, M, solver_controller)
         class(hip_fgmres_t), intent(inout) :: self
         type(hip_matrix_t), intent(in) :: A
         real(c_double), intent(inout) :: x(:)
         real(c_double), intent(in) :: b(:)
         type(hip_precon_t), intent(inout) :: M
+        type(hip_solver_controller_t), intent(inout), optional :: solver_controller
         if (.not. M%initialized) call M%init(A)
+        self%params%update_preconditioner = 1
+        if (present(solver_controller)) then
+            self%params%update_preconditioner = 0
+            if (solver_controller%update_preconditioner(A, M)) self%params%update_preconditioner = 1
+        end if
         call hip_check(cdg_fgmres_solve(A%ctx, A%h, M%h, b, x, self%params, self%stats), 'cdg_fgmres_solve', A%ctx)
         self%niter = self%stats%niter
         self%residual_norm = self%stats%residual_norm
+        if (self%stats%factored /= 0) M%stamp = A%stamp()
     end subroutine
+
+    !> Default: always update the lhs (type_solver_controller.f90:58-74).
+    function ctl_update_lhs(self, A, niter, residual_ratio) result(update)
+        class(hip_solver_controller_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        integer, intent(in) :: niter
+        real(c_double), intent(in) :: residual_ratio
+        logical :: update
+        update = .true.
+        self%lhs_updated = .true.
+    end function
+
+    !> type_solver_controller.f90:89-124.  The reference reduces the decision
+    !! with a logical OR over the ranks of its communicator; one GPU is one
+    !! rank, whose decision that reduction returns unchanged.
+    function ctl_update_preconditioner(self, lhs, preconditioner) result(update)
+        class(hip_solver_controller_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: lhs
+        type(hip_precon_t), intent(in) :: preconditioner
+        logical :: update
+        integer(c_int64_t) :: stamp
+        stamp = lhs%stamp()
+        update = (self%lhs_stamp /= stamp) .or. self%force_update_preconditioner .or. &
+                 (stamp /= preconditioner%stamp)
+        self%lhs_stamp = stamp
+        self%force_update_preconditioner = .false.
+        self%preconditioner_updated = update
+    end function
+
+    ! ---- bridging from the reference containers' layout ---------------------
+
+    !> row_ptr/col/kind (0-based, C) of rows(:) in ChiDG row order: per row the
+    !! lblks entries in their stored order (face order, then the diagonal,
+    !! type_domain_matrix.f90:134,253-305), then the coupling entries.
+    subroutine hip_pattern_from_rows(rows, row_ptr, col, kind)
+        type(hip_block_row_t), intent(in) :: rows(:)
+        integer(c_int32_t), allocatable, intent(out) :: row_ptr(:), col(:), kind(:)
+        integer :: i, k, nb, nl, nc
+        allocate(row_ptr(size(rows) + 1))
+        row_ptr(1) = 0
+        do i = 1, size(rows)
+            row_ptr(i + 1) = row_ptr(i) + nentries(rows(i), nl, nc)
+        end do
+        nb = row_ptr(size(rows) + 1)
+        allocate(col(max(nb, 1)), kind(max(nb, 1)))
+        do i = 1, size(rows)
+            nb = row_ptr(i) + nentries(rows(i), nl, nc)
+            do k = 1, nl
+                col(row_ptr(i) + k) = rows(i)%lparent(k) - 1
+                kind(row_ptr(i) + k) = HIP_KIND_LOCAL
+            end do
+            do k = 1, nc
+                col(row_ptr(i) + nl + k) = rows(i)%cparent(k) - 1
+                kind(row_ptr(i) + nl + k) = HIP_KIND_COUPLING
+            end do
+        end do
+    end subroutine
+
+    !> All blocks of rows(:) concatenated in pattern order, each n x n
+    !! column-major (densematrix_t%mat's own storage order), ready for store.
+    subroutine hip_pack_blocks(rows, n, blocks)
+        type(hip_block_row_t), intent(in) :: rows(:)
+        integer, intent(in) :: n
+        real(c_double), allocatable, intent(out) :: blocks(:)
+        integer :: i, k, nl, nc, nb, pos
+        nb = 0
+        do i = 1, size(rows)
+            nb = nb + nentries(rows(i), nl, nc)
+        end do
+        allocate(blocks(max(nb, 1) * n * n))
+        pos = 0
+        do i = 1, size(rows)
+            nb = nentries(rows(i), nl, nc)
+            do k = 1, nl
+                blocks(pos + 1:pos + n * n) = reshape(rows(i)%lmat(:, :, k), [n * n])
+                pos = pos + n * n
+            end do
+            do k = 1, nc
+                blocks(pos + 1:pos + n * n) = reshape(rows(i)%cmat(:, :, k), [n * n])
+                pos = pos + n * n
+            end do
+        end do
+    end subroutine
+
+    !> x = the element vectors of a domain_vector_t%vecs(:) one after another
+    !! (element-major; within an element the reference's (ifield-1)*nterms+iterm
+    !! order, type_densevector.f90:214).
+    subroutine hip_get_packed(vecs, x)
+        type(hip_densevector_t), intent(in) :: vecs(:)
+        real(c_double), allocatable, intent(out) :: x(:)
+        integer :: i, pos, total
+        total = 0
+        do i = 1, size(vecs)
+            total = total + size(vecs(i)%vec)
+        end do
+        allocate(x(total))
+        pos = 0
+        do i = 1, size(vecs)
+            x(pos + 1:pos + size(vecs(i)%vec)) = vecs(i)%vec
+            pos = pos + size(vecs(i)%vec)
+        end do
+    end subroutine
+
+    !> The inverse of hip_get_packed: x scattered back into the (allocated)
+    !! element vectors.
+    subroutine hip_set_packed(x, vecs)
+        real(c_double), intent(in) :: x(:)
+        type(hip_densevector_t), intent(inout) :: vecs(:)
+        integer :: i, pos
+        pos = 0
+        do i = 1, size(vecs)
+            vecs(i)%vec = x(pos + 1:pos + size(vecs(i)%vec))
+            pos = pos + size(vecs(i)%vec)
+        end do
+    end subroutine
+
+    integer function nentries(row, nl, nc)
+        type(hip_block_row_t), intent(in) :: row
+        integer, intent(out) :: nl, nc
+        nl = 0
+        nc = 0
+        if (allocated(row%lparent)) nl = size(row%lparent)
+        if (allocated(row%cparent)) nc = size(row%cparent)
+        nentries = nl + nc
+    end function
 
 end module mod_chidg_hip

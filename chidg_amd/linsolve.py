@@ -48,6 +48,29 @@ def _empty_like(x, n):
     return np.empty(n, dtype=np.float64)
 
 
+def _on_gpu(*xs) -> bool:
+    return any(x is not None and _is_torch(x) and x.is_cuda for x in xs)
+
+
+def _torch_stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _after_torch(ctx, *xs) -> None:
+    """The library works on its own stream: order its next work after what
+    torch's current stream has queued (the producers of device inputs, and the
+    last users of memory the caching allocator hands out as outputs)."""
+    if _on_gpu(*xs):
+        ctx._check(ctx.lib.cdg_ctx_wait_stream(ctx.h, _torch_stream()), "cdg_ctx_wait_stream")
+
+
+def _before_torch(ctx) -> None:
+    """Order torch's later work after the library's queued work (outputs of
+    the enqueue-only calls)."""
+    ctx._check(ctx.lib.cdg_ctx_signal_stream(ctx.h, _torch_stream()), "cdg_ctx_signal_stream")
+
+
 class Context:
     """One HIP device + stream (``cdg_ctx``)."""
 
@@ -128,6 +151,7 @@ class chidg_matrix:
             blocks = np.ascontiguousarray(blocks, dtype=np.float64)
             if blocks.size != self.nnzb * self.n * self.n:
                 raise ValueError("block array has the wrong size")
+        _after_torch(self.ctx, blocks)
         self.ctx._check(self.ctx.lib.cdg_mat_upload(self.ctx.h, self.h, _ptr(blocks)), "cdg_mat_upload")
 
     assemble = store
@@ -138,6 +162,7 @@ class chidg_matrix:
 
     def synthetic_vector(self, seed: int, scale: float = 1.0, like=None):
         v = _empty_like(like if like is not None else np.empty(0), self.vec_len)
+        _after_torch(self.ctx, v)
         self.ctx._check(self.ctx.lib.cdg_vec_fill_synthetic(self.ctx.h, self.h, seed, scale, _ptr(v)),
                         "cdg_vec_fill_synthetic")
         return v
@@ -181,13 +206,21 @@ class chidg_matrix:
 def chidg_mv(A: chidg_matrix, x, out=None):
     """y = A x over every block class (operator_chidg_mv.f90:46-61)."""
     y = out if out is not None else _empty_like(x, A.vec_len)
+    _after_torch(A.ctx, x, y)
     A.ctx._check(A.ctx.lib.cdg_mv(A.ctx.h, A.h, _ptr(x), _ptr(y)), "cdg_mv")
     return y
 
 
-def chidg_mv_async(A: chidg_matrix, x, y) -> None:
-    """Enqueue y = A x on the library stream (device tensors only)."""
+def chidg_mv_async(A: chidg_matrix, x, y, order: bool = True) -> None:
+    """Enqueue y = A x on the library stream (device tensors only).  With
+    ``order`` (the default) the product waits for torch's queued work and
+    torch's later work waits for the product; a caller that has synchronized
+    the device already (a timing loop) may pass False."""
+    if order:
+        _after_torch(A.ctx, x, y)
     A.ctx._check(A.ctx.lib.cdg_mv_async(A.ctx.h, A.h, _ptr(x), _ptr(y)), "cdg_mv_async")
+    if order:
+        _before_torch(A.ctx)
 
 
 class _Preconditioner:
@@ -215,12 +248,14 @@ class _Preconditioner:
 
     def set_schedule(self, schedule: str):
         """ILU(0) apply schedule: 'levels' (one launch per wavefront level
-        and stage), 'dataflow' (one persistent launch), 'folded' (levels,
-        with Dinv folded into the upper blocks at the first apply after each
-        factor; FP64/FP32) or 'auto' (the default: 'levels' for the first 12
-        applies after each factor, 'folded' from then on, or from the first
-        apply when the previous factor served more than 12)."""
-        code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3}[schedule]
+        and stage), 'dataflow' (one persistent launch over column slices with
+        partial sums), 'folded' (levels, with Dinv folded into the upper blocks
+        at the first apply after each factor; FP64/FP32), 'stream' (one
+        persistent launch over row slices of the factor, W = Dinv U formed at
+        the first apply after each factor) or 'auto' (the default: 'levels'
+        for the first 12 applies after each factor, 'stream' from then on, or
+        from the first apply when the previous factor served more than 12)."""
+        code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3, "stream": 4}[schedule]
         self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
         self.schedule = schedule
 
@@ -251,13 +286,19 @@ class _Preconditioner:
 
     def apply(self, A: chidg_matrix, v, out=None):
         z = out if out is not None else _empty_like(v, A.vec_len)
+        _after_torch(self.ctx, v, z)
         self.ctx._check(self.ctx.lib.cdg_prec_apply(self.ctx.h, self.h, A.h, _ptr(v), _ptr(z)), "cdg_prec_apply")
         return z
 
-    def apply_async(self, A: chidg_matrix, v, z) -> None:
-        """Enqueue z = M^-1 v on the library stream (device tensors only)."""
+    def apply_async(self, A: chidg_matrix, v, z, order: bool = True) -> None:
+        """Enqueue z = M^-1 v on the library stream (device tensors only);
+        ``order`` as in ``chidg_mv_async``."""
+        if order:
+            _after_torch(self.ctx, v, z)
         self.ctx._check(self.ctx.lib.cdg_prec_apply_async(self.ctx.h, self.h, A.h, _ptr(v), _ptr(z)),
                         "cdg_prec_apply_async")
+        if order:
+            _before_torch(self.ctx)
 
     def factors(self, A: chidg_matrix):
         """(L, Dinv) as FP64 host arrays, L in the matrix's block slots."""
@@ -328,14 +369,17 @@ class fgmres:
                                  int(self.inner_fgmres), self.inner_nkrylov, self.inner_tol, self.inner_rtol,
                                  self.inner_nmax, o[self.inner_orthogonalization], int(update_preconditioner))
 
-    def solve(self, A: chidg_matrix, x, b, M: _Preconditioner, solver_controller=None):
+    def solve(self, A: chidg_matrix, x, b, M: _Preconditioner, solver_controller=None, comm=None):
         """Solve A x = b (x overwritten, zero initial guess).  Without a
         controller the preconditioner is updated every call, as in the
-        reference; with one, only when the matrix stamp changed."""
+        reference; with one, when ``solver_controller.update_preconditioner``
+        says so (type_fgmres.f90:118-122)."""
         if not M.initialized:
             M.init(A)
-        prm = self.params(update_preconditioner=solver_controller is None)
+        update = True if solver_controller is None else solver_controller.update_preconditioner(A, M, comm)
+        prm = self.params(update_preconditioner=update)
         st = _lib.FgmresStats()
+        _after_torch(A.ctx, x, b)
         A.ctx._check(A.ctx.lib.cdg_fgmres_solve(A.ctx.h, A.h, M.h, _ptr(b), _ptr(x), C.byref(prm), C.byref(st)),
                      "cdg_fgmres_solve")
         self.niter = st.niter
@@ -347,4 +391,34 @@ class fgmres:
 
 
 class solver_controller:
-    """Stamp-keyed refactor decision (type_solver_controller.f90:89-124)."""
+    """``solver_controller_t`` (type_solver_controller.f90:25-124): controls
+    the update of the lhs and of the preconditioner inside a nonlinear solver.
+
+    ``update_preconditioner(lhs, preconditioner, comm)`` asks for a refactor
+    when the lhs changed since the last call (its stamp differs from the
+    remembered ``lhs_stamp``), when ``force_update_preconditioner`` is set, or
+    when the preconditioner was last updated with another version of the
+    matrix; it then remembers the lhs stamp and clears the force flag.  The
+    reference reduces the decision with a logical OR over the ranks of
+    ``comm``; on one GPU (one rank) that reduction is the local decision."""
+
+    def __init__(self):
+        self.force_update_lhs = False
+        self.lhs_updated = False
+        self.force_update_preconditioner = False
+        self.lhs_stamp = 0
+        self.preconditioner_updated = False
+
+    def update_lhs(self, A, niter: int = 0, residual_ratio: float = 0.0) -> bool:
+        """Default: always update the lhs (type_solver_controller.f90:58-74)."""
+        self.lhs_updated = True
+        return True
+
+    def update_preconditioner(self, lhs: chidg_matrix, preconditioner: _Preconditioner, comm=None) -> bool:
+        stamp = lhs.stamp
+        update = (self.lhs_stamp != stamp or self.force_update_preconditioner
+                  or stamp != preconditioner.stamp)
+        self.lhs_stamp = stamp
+        self.force_update_preconditioner = False
+        self.preconditioner_updated = bool(update)
+        return self.preconditioner_updated

@@ -104,6 +104,15 @@ int         cdg_ctx_mem_used(const cdg_ctx *ctx, int64_t *bytes);
 /* The context's HIP stream (hipStream_t) so callers can order their own work
  * and record events around the *_async calls.  No reference counterpart. */
 int         cdg_ctx_stream(const cdg_ctx *ctx, void **stream);
+/* Ordering against a caller's stream (a hipStream_t of the context's device,
+ * NULL the legacy default stream), without a host wait; no reference
+ * counterpart.  cdg_ctx_wait_stream: work the context queues from now on
+ * starts after the work already queued on `stream` (call it before passing
+ * device vectors that stream produces).  cdg_ctx_signal_stream: work queued
+ * on `stream` from now on starts after the context's queued work (call it
+ * after a *_async call whose outputs that stream consumes). */
+int         cdg_ctx_wait_stream(cdg_ctx *ctx, void *stream);
+int         cdg_ctx_signal_stream(cdg_ctx *ctx, void *stream);
 
 /* ---- matrix: chidg_matrix_t (type_chidg_matrix.f90:38-83) --------------------
  * Pattern: per block row (element, time level) the lblks entries followed by
@@ -156,17 +165,25 @@ int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
  *                          blocks, nup more block slots), and the backward
  *                          sweep is one stage per level.  Pays off when one
  *                          factor serves many applies; FP64 and FP32 only.
+ *   CDG_SCHEDULE_STREAM    one persistent launch over (row, row slice) tasks:
+ *                          the first apply after each factor forms
+ *                          W = Dinv U (left fold, one batched block GEMM) and
+ *                          copies L and Dinv into a row-slice layout (nlo +
+ *                          nup + nrows more block slots), so every task writes
+ *                          final values and the backward sweep needs no
+ *                          Dinv stage.  All precisions.
  *   CDG_SCHEDULE_AUTO      (the default) CDG_SCHEDULE_LEVELS for the first 12
- *                          applies after each factor, CDG_SCHEDULE_FOLDED from
+ *                          applies after each factor, CDG_SCHEDULE_STREAM from
  *                          the 13th on, or from the first when the previous
  *                          factor served more than 12 (environment
- *                          CDG_FOLD_AFTER=k sets k; 0 never folds), so long
- *                          solves get the shorter backward sweep and short
- *                          ones skip the fold.  BF16 factors never fold.  */
+ *                          CDG_FOLD_AFTER=k sets k; 0 never switches), so long
+ *                          solves get the streaming sweep and short ones skip
+ *                          its preparation.  */
 #define CDG_SCHEDULE_LEVELS   0
 #define CDG_SCHEDULE_DATAFLOW 1
 #define CDG_SCHEDULE_FOLDED   2
 #define CDG_SCHEDULE_AUTO     3
+#define CDG_SCHEDULE_STREAM   4
 int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule);
 /* Launch-shape autotuner (no reference counterpart).  cdg_prec_tune times the
  * apply of M's current factor (level schedules; after cdg_prec_update) under

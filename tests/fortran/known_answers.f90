@@ -12,9 +12,15 @@ program known_answers
     type(hip_matrix_t)  :: A, B
     type(hip_precon_t)  :: M, I3, I2
     type(hip_fgmres_t)  :: ls
-    real(c_double), allocatable :: z(:), x(:)
+    type(hip_matrix_t)  :: C
+    type(hip_precon_t)  :: MC
+    type(hip_solver_controller_t) :: ctl
+    type(hip_block_row_t) :: rows(3)
+    type(hip_densevector_t) :: vecs(3)
+    integer(c_int32_t), allocatable :: rp(:), cl(:), kd(:)
+    real(c_double), allocatable :: z(:), x(:), packed(:), xp(:)
     real(c_double) :: blk(7), t_apply, t_mv
-    integer :: nfail = 0
+    integer :: nfail = 0, k, factored(5)
 
     call ctx%init(0)
 
@@ -40,14 +46,62 @@ program known_answers
     z = M%apply(A, [1.d0, 2.d0, 3.d0])
     call check('ilu0_apply_tuned', t_apply > 0.d0 .and. t_mv > 0.d0 .and. &
                maxval(abs(z - [43.d0/7.d0, 18.d0/7.d0, 3.d0/14.d0])) < 1.d-12)
+    call M%set_schedule(HIP_SCHEDULE_STREAM)
+    z = M%apply(A, [1.d0, 2.d0, 3.d0])
+    call check('ilu0_apply_stream', maxval(abs(z - [43.d0/7.d0, 18.d0/7.d0, 3.d0/14.d0])) < 1.d-12)
     call M%set_schedule(HIP_SCHEDULE_AUTO)
+
+    ! --- the same system from rows in domain_matrix_t layout (lblks parents in
+    !     face order [XI_MIN, XI_MAX, DIAG], 1-based) through the bridging ----
+    allocate(rows(1)%lparent(2), rows(1)%lmat(1, 1, 2))
+    rows(1)%lparent = [2, 1]
+    rows(1)%lmat(1, 1, :) = [-2.d0, 1.d0]
+    allocate(rows(2)%lparent(3), rows(2)%lmat(1, 1, 3))
+    rows(2)%lparent = [1, 3, 2]
+    rows(2)%lmat(1, 1, :) = [2.d0, 12.d0, -5.d0]
+    allocate(rows(3)%lparent(2), rows(3)%lmat(1, 1, 2), rows(3)%cparent(1), rows(3)%cmat(1, 1, 1))
+    rows(3)%lparent = [2, 3]
+    rows(3)%lmat(1, 1, :) = [2.d0, -10.d0]
+    rows(3)%cparent = [1]            ! a coupling-only (chi_blks) entry: matvec only
+    rows(3)%cmat(1, 1, 1) = 0.5d0
+    call hip_pattern_from_rows(rows, rp, cl, kd)
+    call check('bridge_pattern', all(rp == [0, 2, 5, 8]) .and. all(cl == [1, 0, 0, 2, 1, 1, 2, 0]) .and. &
+               all(kd == [0, 0, 0, 0, 0, 0, 0, 1]))
+    call hip_pack_blocks(rows, 1, packed)
+    call check('bridge_pack', all(packed == [-2.d0, 1.d0, 2.d0, 12.d0, -5.d0, 2.d0, -10.d0, 0.5d0]))
+    call C%init(ctx, 3, 1, rp, cl, kd)
+    call C%store(packed)
+    do k = 1, 3
+        allocate(vecs(k)%vec(1))
+        vecs(k)%vec = real(k, c_double)
+    end do
+    call hip_get_packed(vecs, xp)
+    call check('bridge_get_packed', all(xp == [1.d0, 2.d0, 3.d0]))
+    z = hip_mv(C, xp)                ! lblks product plus 0.5 * x(1) into row 3
+    call check('bridge_mv', maxval(abs(z - [-3.d0, 28.d0, -25.5d0])) < 1.d-13)
+    call hip_set_packed(z, vecs)
+    call check('bridge_set_packed', vecs(2)%vec(1) == 28.d0 .and. vecs(3)%vec(1) == -25.5d0)
+
+    ! --- solver_controller: refactor only when the lhs changed or when forced -
+    call MC%init(C, HIP_PREC_ILU0, HIP_FP64)
+    call ls%init()
+    ls%params%inner_fgmres = 0
+    ls%params%tol = 1.d-13
+    allocate(x(3))
+    do k = 1, 5
+        if (k == 3) call C%store(packed)                 ! a new lhs version (re-stamped)
+        if (k == 5) ctl%force_update_preconditioner = .true.
+        call ls%solve(C, x, [1.d0, 2.d0, 3.d0], MC, ctl)
+        factored(k) = ls%stats%factored
+    end do
+    call check('controller_refactor_rule', all(factored == [1, 0, 1, 0, 1]))
+    call check('controller_solution', maxval(abs(hip_mv(C, x) - [1.d0, 2.d0, 3.d0])) < 1.d-10)
 
     ! --- FGMRES 3x3: A = [[4,3,0],[3,4,-1],[0,-1,4]], b = [24,30,-24] --------
     call A%store([3.d0, 4.d0, 3.d0, -1.d0, 4.d0, -1.d0, 4.d0])
     call ls%init()                 ! reference defaults (inner FGMRES on)
     ls%params%tol = 1.d-13
     call I3%init(A, HIP_PREC_IDENTITY)
-    allocate(x(3))
     call ls%solve(A, x, [24.d0, 30.d0, -24.d0], I3)
     call check('fgmres_3x3', maxval(abs(x - [3.d0, 4.d0, -5.d0])) < 1.d-9)
 
@@ -65,6 +119,8 @@ program known_answers
     call check('mv', maxval(abs(z - [24.d0, 30.d0, -24.d0])) < 1.d-13)
 
     call M%tear_down()
+    call MC%tear_down()
+    call C%release()
     call I3%tear_down()
     call I2%tear_down()
     call A%release()

@@ -132,3 +132,30 @@ def synthetic_system(nx, ny, nz, degree, seed, periodic=(True, True, True), nfie
     p = pt.hex_box(nx, ny, nz, n, periodic)
     blocks = po.fill_synthetic(p, seed, 1.0 / np.sqrt(n), s_off_scale / np.sqrt(n))
     return p, blocks
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def synth_block(seed: int, b: int, n: int, is_diag: bool, s_diag: float, s_off: float) -> np.ndarray:
+    """Block ``b`` of the seeded synthetic generator (oracle/synth.c), n*n
+    column-major, computed with numpy's wrapping uint64 arithmetic: every
+    step is exact or one IEEE rounding, so the values are bitwise those of
+    the generator.  Lets a test check single blocks of a matrix too large to
+    generate on the host."""
+    G = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        idx = np.uint64(b) * np.uint64(2097152) + np.arange(n * n, dtype=np.uint64)
+        s = np.uint64(seed)
+        h1 = _mix64(s + G * (np.uint64(2) * idx + np.uint64(1)))
+        h2 = _mix64(s + G * (np.uint64(2) * idx + np.uint64(2)))
+    lo = np.uint64(0xFFFFFFFF)
+    S = (h1 & lo) + (h1 >> np.uint64(32)) + (h2 & lo) + (h2 >> np.uint64(32))
+    g = (S.astype(np.float64) * 2.3283064365386963e-10 - 2.0) * 1.7320508075688772
+    out = g * (s_diag if is_diag else s_off)
+    if is_diag:
+        out[np.arange(n) * (n + 1)] += 2.0
+    return out

@@ -30,6 +30,7 @@ def test_fortran_known_answers_on_gpu():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASS" in r.stdout
-    for name in ("ilu0_apply", "ilu0_apply_folded", "ilu0_apply_tuned", "mat_fetch", "mem_used",
-                 "fgmres_3x3", "fgmres_2x2_skew", "mv"):
+    for name in ("ilu0_apply", "ilu0_apply_folded", "ilu0_apply_tuned", "ilu0_apply_stream", "mat_fetch",
+                 "mem_used", "bridge_pattern", "bridge_pack", "bridge_get_packed", "bridge_mv", "bridge_set_packed",
+                 "controller_refactor_rule", "controller_solution", "fgmres_3x3", "fgmres_2x2_skew", "mv"):
         assert f"PASS {name}" in r.stdout

@@ -52,7 +52,7 @@ def test_gpu_matches_golden(name):
     A = chidg_matrix(p, ctx)
     A.store(f["blocks"])
     assert rel(chidg_mv(A, f["x"]), f["Ax"]) < 1e-13
-    for schedule in ("levels", "dataflow"):
+    for schedule in ("levels", "dataflow", "stream"):
         M = precon_ILU0(schedule=schedule).init(A)
         M.update(A)
         assert rel(M.apply(A, f["b"]), f["ilu_b"]) < 1e-12

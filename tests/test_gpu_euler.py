@@ -53,7 +53,7 @@ def test_euler_ilu0_apply(ctx, config1):
     assert rel(M.apply(A, -R), z_ref) < 1e-12
 
 
-@pytest.mark.parametrize("schedule", ["levels", "folded", "auto"])
+@pytest.mark.parametrize("schedule", ["levels", "folded", "stream", "auto"])
 def test_euler_fgmres_config1(ctx, config1, schedule):
     """b = -R: FGMRES(30)+ILU(0) to rtol 1e-8, FP64, against the oracle."""
     p, blocks, R = config1

@@ -345,3 +345,52 @@ def test_tiny_and_empty_systems(ctx):
     A = chidg_matrix(p, ctx)
     A.store(np.zeros(0))
     assert chidg_mv(A, np.zeros(0)).shape == (0,)
+
+
+def test_controller_refactors_only_when_needed(ctx):
+    """fgmres%solve with a solver_controller (type_fgmres.f90:118-122): the
+    second solve on an unchanged matrix reuses the factor; a store() that
+    re-stamps the matrix, and the force flag, each trigger one refactor."""
+    from chidg_amd.linsolve import solver_controller
+    p, blocks = synthetic_system(3, 3, 3, 1, seed=44)
+    A = upload(ctx, p, blocks)
+    M = precon_ILU0().init(A)
+    ctl = solver_controller()
+    ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
+    b = po.fill_vector(p.vec_len, 3)
+    seen = []
+    for action in ("first", "same", "store", "same", "force", "same"):
+        if action == "store":
+            A.store(blocks)
+        elif action == "force":
+            ctl.force_update_preconditioner = True
+        x = np.zeros(p.vec_len)
+        ls.solve(A, x, b, M, solver_controller=ctl)
+        assert ls.residual_norm / np.linalg.norm(b) < 1e-8
+        seen.append(ls.stats.factored)
+    assert seen == [1, 0, 1, 0, 1, 0]
+
+
+def test_torch_producer_ordered_before_library(ctx):
+    """Device inputs written by torch kernels still in flight on torch's
+    stream: the library's stream waits for them (cdg_ctx_wait_stream), and
+    torch consumers of an enqueue-only apply wait for it."""
+    import torch
+    p, blocks = synthetic_system(4, 4, 4, 2, seed=45)
+    A = upload(ctx, p, blocks)
+    M = precon_ILU0().init(A)
+    M.update(A)
+    xh = po.fill_vector(p.vec_len, 5)
+    y_ref = chidg_mv(A, xh)
+    z_ref = M.apply(A, xh)
+    big = torch.randn(4096, 4096, device="cuda:0", dtype=torch.float64)
+    for _ in range(3):
+        x0 = torch.zeros(p.vec_len, device="cuda:0", dtype=torch.float64)
+        _ = big @ big                      # keeps torch's stream busy
+        x = x0 + torch.from_numpy(xh).to("cuda:0")
+        assert np.array_equal(chidg_mv(A, x).cpu().numpy(), y_ref)
+        _ = big @ big
+        v = x * 1.0
+        z = torch.empty_like(v)
+        M.apply_async(A, v, z)
+        assert np.array_equal((z * 1.0).cpu().numpy(), z_ref)
