@@ -23,7 +23,7 @@ EXPORTS = [
     "cdg_ctx_create", "cdg_ctx_destroy", "cdg_last_error", "cdg_ctx_mem_used", "cdg_ctx_stream",
     "cdg_ctx_wait_stream", "cdg_ctx_signal_stream",
     "cdg_mat_create", "cdg_mat_destroy", "cdg_mat_upload", "cdg_mat_download",
-    "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
+    "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_set_partition", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
     "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
     "cdg_prec_set_schedule", "cdg_prec_tune", "cdg_mat_tune", "cdg_prec_launch_shape", "cdg_mat_launch_shape",
     "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
@@ -105,6 +105,7 @@ def load() -> C.CDLL:
         "cdg_mat_fill_synthetic": ([vp, vp, C.c_uint64, C.c_double, C.c_double], C.c_int),
         "cdg_vec_fill_synthetic": ([vp, vp, C.c_uint64, C.c_double, dp], C.c_int),
         "cdg_mat_stamp": ([vp, C.POINTER(C.c_uint64)], C.c_int),
+        "cdg_mat_set_partition": ([vp, vp, ip], C.c_int),
         "cdg_mat_info": ([vp, ip, ip, ip], C.c_int),
         "cdg_mv": ([vp, vp, dp, dp], C.c_int),
         "cdg_prec_create": ([vp, vp, C.c_int32, C.c_int32, pp], C.c_int),

@@ -272,6 +272,7 @@ void analyze(cdg_mat *A) {
             if (A->kind[b] != CDG_KIND_LOCAL) continue;
             const int j = A->col[b];
             if (j == i) { if (A->diag[i] < 0) A->diag[i] = b; }
+            else if (!A->part.empty() && A->part[j] != A->part[i]) continue;   // parent on another rank
             else if (j < i) { A->lo_blk.push_back(b); A->lo_col.push_back(j); }
             else { A->up_blk.push_back(b); A->up_col.push_back(j); }
         }
@@ -530,6 +531,19 @@ extern "C" int cdg_vec_fill_synthetic(cdg_ctx *ctx, const cdg_mat *A, uint64_t s
     if (!e) e = launch_vec_synthetic(ctx, len, seed, scale, o.dev);
     if (!e) e = finish_out(ctx, o);
     return e;
+}
+
+extern "C" int cdg_mat_set_partition(cdg_ctx *ctx, cdg_mat *A, const int32_t *part) {
+    if (!ctx || !A) return CDG_ERR_ARG;
+    if (part) {
+        for (int i = 0; i < A->nrows; ++i)
+            if (part[i] < 0) return cdg_fail(ctx, CDG_ERR_ARG, "part[%d] = %d is negative", i, part[i]);
+        A->part.assign(part, part + A->nrows);
+    } else {
+        A->part.clear();
+    }
+    analyze(A);
+    return CDG_OK;
 }
 
 extern "C" int cdg_mat_stamp(const cdg_mat *A, uint64_t *stamp) {

@@ -74,6 +74,7 @@ struct cdg_mat {
     double *blocks = nullptr;          // nnzb * ld, device
     int32_t *d_row_ptr = nullptr, *d_col = nullptr;
     std::vector<int32_t> row_ptr, col, kind;
+    std::vector<int32_t> part;         // rank of each row (empty: one rank), see cdg_mat_set_partition
     int32_t max_row_blocks = 0;
     uint64_t stamp = 0;
     // kind-0 structure (type_domain_matrix.f90:295-302, 592-627)

@@ -411,6 +411,9 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
     int target = 256;
     if (const char *e = std::getenv("CDG_RS_THREADS")) target = std::max(64, std::atoi(e));
     const int w = rs_slice_rows(n, wmax);
+    // a task gathers at most (maxdep + 1) parent vectors, at most 8 values per thread
+    const int ne_max = (M->maxdep + 1) * n;
+    target = std::max(target, (ne_max + 7) / 8);
     const StreamShape sh = stream_shape(w, V, target, kRsMaxThreads);
     if (sh.T == 0) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream schedule: no launch shape for n=%d", n);
     const int T = sh.T;
@@ -418,10 +421,10 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
     const int64_t SL = ((int64_t)w * n + 7) / 8 * 8;
     const int64_t slot = (S * SL + 63) / 64 * 64;
     const int R = round_r(R_needed<int>(w * n, V * T));
-    const int EPT = round_pow((kRsMaxBlocks * n + T - 1) / T);
-    if ((kRsMaxBlocks * n + T - 1) / T > 8)
+    if ((ne_max + T - 1) / T > 8)
         return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream schedule: %d threads cannot gather %d parents of n=%d", T,
-                        kRsMaxBlocks, n);
+                        M->maxdep + 1, n);
+    const int EPT = round_pow((ne_max + T - 1) / T);
     const void *fn = prec == CDG_FP64 ? rs_kernel<double>(R, EPT)
                    : (prec == CDG_FP32 ? rs_kernel<float>(R, EPT) : rs_kernel<cdg_bf16>(R, EPT));
     const size_t lds = accsz * ((((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3) + (size_t)V * T);

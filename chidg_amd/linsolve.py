@@ -156,6 +156,22 @@ class chidg_matrix:
 
     assemble = store
 
+    def set_partition(self, part) -> None:
+        """Rank of each block row (None: one rank).  ILU(0) then uses only the
+        blocks whose parent row is on the row's rank (the reference's
+        parent_proc == IRANK test, precon_ILU0.f90:180,222,344,368):
+        block-Jacobi ILU(0) over the ranks.  Preconditioners created
+        afterwards follow it; the matvec uses every block."""
+        if part is None:
+            code = self.ctx.lib.cdg_mat_set_partition(self.ctx.h, self.h, None)
+        else:
+            arr = np.ascontiguousarray(part, dtype=np.int32)
+            if arr.shape != (self.nrows,):
+                raise ValueError("part needs one rank per block row")
+            code = self.ctx.lib.cdg_mat_set_partition(self.ctx.h, self.h, arr.ctypes.data_as(C.POINTER(C.c_int32)))
+        self.ctx._check(code, "cdg_mat_set_partition")
+        self.part = None if part is None else np.asarray(part, dtype=np.int32).copy()
+
     def fill_synthetic(self, seed: int, s_diag: float, s_off: float) -> None:
         self.ctx._check(self.ctx.lib.cdg_mat_fill_synthetic(self.ctx.h, self.h, seed, s_diag, s_off),
                         "cdg_mat_fill_synthetic")

@@ -132,6 +132,14 @@ int cdg_mat_fill_synthetic(cdg_ctx *ctx, cdg_mat *A, uint64_t seed, double s_dia
 /* Synthetic vector of length nrows*n: g(seed', index)*scale (same hash). */
 int cdg_vec_fill_synthetic(cdg_ctx *ctx, const cdg_mat *A, uint64_t seed, double scale, double *v);
 int cdg_mat_stamp(const cdg_mat *A, uint64_t *stamp);
+/* Rank of each block row, the reference's parent_proc of every block
+ * (densematrix_t, type_densematrix.f90:35-94); part = NULL: one rank.  ILU(0)
+ * uses a kind-0 block only when its parent row has the row's rank, the
+ * reference's parent_proc == IRANK test (precon_ILU0.f90:180,222,344,368), so
+ * a partitioned matrix gets the block-Jacobi ILU(0) a multi-rank run computes,
+ * all ranks on one GPU.  The matvec uses every block.  Preconditioners created
+ * afterwards follow the partition. */
+int cdg_mat_set_partition(cdg_ctx *ctx, cdg_mat *A, const int32_t *part);
 int cdg_mat_info(const cdg_mat *A, int32_t *nrows, int32_t *block_n, int32_t *nnzb);
 
 /* y = A x over lblks, chi_blks, bc_blks, hb_blks

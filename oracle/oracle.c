@@ -82,6 +82,7 @@ static int struct_build(const orc_pattern *p, orc_struct *s, int need_transpose)
             if (p->kind[b] != 0) continue;
             int j = p->col[b];
             if (j == i) { if (s->diag[i] < 0) s->diag[i] = b; }
+            else if (p->part && p->part[j] != p->part[i]) continue;   /* parent on another rank */
             else if (j < i) s->lo_ptr[i + 1]++;
             else s->up_ptr[i + 1]++;
         }
@@ -98,6 +99,7 @@ static int struct_build(const orc_pattern *p, orc_struct *s, int need_transpose)
         for (int b = p->row_ptr[i]; b < p->row_ptr[i + 1]; ++b) {
             if (p->kind[b] != 0) continue;
             int j = p->col[b];
+            if (p->part && p->part[j] != p->part[i]) continue;
             if (j < i) s->lo_blk[il++] = b;
             else if (j > i) s->up_blk[iu++] = b;
         }

@@ -56,6 +56,10 @@ typedef struct {
     const int32_t *row_ptr;   /* nrows+1                                    */
     const int32_t *col;       /* parent row of each block                   */
     const int32_t *kind;      /* 0 interior/diag, 1 coupling-only           */
+    const int32_t *part;      /* rank of each row, or NULL (one rank): ILU0
+                                 uses a kind-0 block only when its parent row
+                                 has the row's rank (parent_proc == IRANK,
+                                 precon_ILU0.f90:180,222,344,368)          */
 } orc_pattern;
 
 typedef struct {

@@ -29,7 +29,7 @@ class OracleError(RuntimeError):
 class _Pattern(C.Structure):
     _fields_ = [("nrows", C.c_int), ("n", C.c_int), ("nnzb", C.c_int),
                 ("row_ptr", C.POINTER(C.c_int32)), ("col", C.POINTER(C.c_int32)),
-                ("kind", C.POINTER(C.c_int32))]
+                ("kind", C.POINTER(C.c_int32)), ("part", C.POINTER(C.c_int32))]
 
 
 class _Params(C.Structure):
@@ -84,16 +84,20 @@ def _dp(a: np.ndarray):
 
 
 class OraclePattern:
-    """Keeps the int32 arrays alive for the C struct."""
+    """Keeps the int32 arrays alive for the C struct.  ``part`` (rank of each
+    row, or None) restricts ILU(0) to blocks whose parent row has the row's
+    rank."""
 
-    def __init__(self, pattern):
+    def __init__(self, pattern, part=None):
         self.row_ptr = np.ascontiguousarray(pattern.row_ptr, dtype=np.int32)
         self.col = np.ascontiguousarray(pattern.col, dtype=np.int32)
         self.kind = np.ascontiguousarray(pattern.kind, dtype=np.int32)
         self.nrows, self.n, self.nnzb = pattern.nrows, pattern.n, int(self.row_ptr[-1])
         ip = C.POINTER(C.c_int32)
+        self.part = None if part is None else np.ascontiguousarray(part, dtype=np.int32)
+        pp = self.part.ctypes.data_as(ip) if self.part is not None else ip()
         self.s = _Pattern(self.nrows, self.n, self.nnzb, self.row_ptr.ctypes.data_as(ip),
-                          self.col.ctypes.data_as(ip), self.kind.ctypes.data_as(ip))
+                          self.col.ctypes.data_as(ip), self.kind.ctypes.data_as(ip), pp)
 
     @property
     def ptr(self):
@@ -133,8 +137,8 @@ def inv(a: np.ndarray) -> np.ndarray:
     return out.T.copy()
 
 
-def ilu0_factor(pattern, blocks: np.ndarray):
-    op = OraclePattern(pattern)
+def ilu0_factor(pattern, blocks: np.ndarray, part=None):
+    op = OraclePattern(pattern, part)
     nn = op.n * op.n
     L = np.zeros(op.nnzb * nn, dtype=np.float64)
     Dinv = np.zeros(op.nrows * nn, dtype=np.float64)
@@ -147,8 +151,8 @@ def ilu0_factor(pattern, blocks: np.ndarray):
     return L, Dinv
 
 
-def ilu0_apply(pattern, blocks, L, Dinv, v) -> np.ndarray:
-    op = OraclePattern(pattern)
+def ilu0_apply(pattern, blocks, L, Dinv, v, part=None) -> np.ndarray:
+    op = OraclePattern(pattern, part)
     z = np.empty(op.nrows * op.n, dtype=np.float64)
     e = lib().orc_ilu0_apply(op.ptr, _dp(blocks), _dp(L), _dp(Dinv),
                              _dp(np.ascontiguousarray(v, dtype=np.float64)), _dp(z))
@@ -159,11 +163,12 @@ def ilu0_apply(pattern, blocks, L, Dinv, v) -> np.ndarray:
 
 def fgmres(pattern, blocks, b, precon: str = "ilu0", factors=None, nkrylov=30, tol=1e-8,
            rtol=1e-8, nmax=-1, orthogonalization="CGS", inner_fgmres=False, inner_nkrylov=100,
-           inner_tol=1e-1, inner_rtol=1e-1, inner_nmax=100, inner_orthogonalization="CGS"):
-    """Returns (x, niter, residual_norm, nreorth)."""
-    op = OraclePattern(pattern)
+           inner_tol=1e-1, inner_rtol=1e-1, inner_nmax=100, inner_orthogonalization="CGS", part=None):
+    """Returns (x, niter, residual_norm, nreorth).  ``part``: ranks of the
+    rows for the ILU(0) (the matvec uses every block)."""
+    op = OraclePattern(pattern, part)
     if precon == "ilu0":
-        L, Dinv = factors if factors is not None else ilu0_factor(pattern, blocks)
+        L, Dinv = factors if factors is not None else ilu0_factor(pattern, blocks, part)
         pk = ORC_PREC_ILU0
     else:
         L = Dinv = np.zeros(1)
