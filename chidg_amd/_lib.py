@@ -27,7 +27,8 @@ EXPORTS = [
     "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
     "cdg_prec_set_schedule", "cdg_prec_tune", "cdg_mat_tune", "cdg_prec_launch_shape", "cdg_mat_launch_shape",
     "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
-    "cdg_server_last_error", "cdg_server_destroy",
+    "cdg_server_last_error", "cdg_server_destroy", "cdg_server_register", "cdg_server_submit_system",
+    "cdg_server_unregister",
     "cdg_fgmres_solve", "cdg_fgmres_default_params",
     "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian",
 ]
@@ -43,7 +44,7 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("niter", C.c_int32), ("residual_norm", C.c_double),
                 ("t_queue", C.c_double), ("t_run", C.c_double), ("t_latency", C.c_double),
-                ("t_factor", C.c_double), ("slot", C.c_int32)]
+                ("t_factor", C.c_double), ("slot", C.c_int32), ("factored", C.c_int32)]
 
 
 class FgmresParams(C.Structure):
@@ -127,6 +128,9 @@ def load() -> C.CDLL:
         "cdg_server_info": ([vp, ip, ip, ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "cdg_server_last_error": ([vp], C.c_char_p),
         "cdg_server_destroy": ([vp], C.c_int),
+        "cdg_server_register": ([vp, vp, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
+        "cdg_server_submit_system": ([vp, C.c_uint64, dp, C.c_int32, dp, C.POINTER(C.c_uint64)], C.c_int),
+        "cdg_server_unregister": ([vp, C.c_uint64], C.c_int),
         "cdg_euler_state": ([C.c_int32] * 4 + [C.c_double, C.c_uint64, C.c_double, dp], C.c_int),
         "cdg_euler_residual": ([C.c_int32] * 4 + [dp, dp, C.c_int32], C.c_int),
         "cdg_euler_jacobian": ([C.c_int32] * 4 + [dp, C.c_double, dp, dp, C.c_int32], C.c_int),

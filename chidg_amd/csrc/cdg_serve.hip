@@ -76,9 +76,28 @@ double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration
 
 int block_n(const cdg_request &r) { return (r.order + 1) * (r.order + 1) * (r.order + 1) * 5; }
 
+// A caller's matrix registered with the server: its ILU(0) factor lives on
+// the home worker's context from the first request on and is reused while
+// the matrix stamp is unchanged (solver_controller_t,
+// type_solver_controller.f90:89-124).
+struct System {
+    const cdg_mat *A = nullptr;        // the caller's (not owned)
+    int32_t precision = CDG_FP64;
+    int home = 0;                      // worker slot that runs its requests
+    cdg_prec *M = nullptr;             // on the home worker's context
+    int64_t resident = 0;              // bytes reserved while registered (the factor)
+};
+
+enum JobKind { JOB_SYNTHETIC = 0, JOB_SYSTEM = 1, JOB_RELEASE = 2 };
+
 struct Job {
     uint64_t ticket;
+    int kind = JOB_SYNTHETIC;
     cdg_request req;
+    uint64_t system = 0;               // JOB_SYSTEM / JOB_RELEASE
+    int home = -1;                     // worker slot that must run it (-1: any)
+    const double *b = nullptr;         // JOB_SYSTEM: nrhs right-hand sides
+    int32_t nrhs = 1;
     double *x_out;
     int64_t footprint;
     clk::time_point t_submit;
@@ -103,9 +122,30 @@ struct cdg_server {
     int64_t in_use = 0;
     int active = 0, peak_active = 0;
     bool stop = false;
+    int nslots = 0;
     std::vector<std::thread> workers;
     std::string err;
+    std::unordered_map<uint64_t, System> systems;
+    uint64_t next_system = 1;
 };
+
+// Device bytes of an ILU(0) preconditioner beyond the matrix: the factor (L,
+// Dinv), the FP32/BF16 copies of the upper blocks, the BF16 variant's FP32
+// work, the block inverse's working slots, the level schedule's partial sums,
+// and the stream schedule's row-slice copies (L, W = Dinv U, Dinv) with its
+// tagged exchange vectors (nup = nlo: the kind-0 pattern is symmetric).
+static int64_t factor_extra_bytes(int64_t nrows, int64_t nlo, int64_t n, int precision) {
+    const int64_t ld = slot_elems((int)n), len = nrows * n;
+    const int64_t es = precision == CDG_FP64 ? 8 : (precision == CDG_FP32 ? 4 : 2);
+    int64_t f = es * (nlo + nrows) * ld;                         // L, Dinv
+    if (precision != CDG_FP64) f += es * nlo * ld;               // U copy
+    if (precision == CDG_BF16) f += 4 * (nlo + nrows) * ld;      // FP32 factor work
+    if (n > 192) f += 8 * nrows * ld;                            // inverse working slots (bound)
+    f += 3 * 8 * nrows * 8 * n;                                  // apply partial sums (<= 8 slices)
+    f += es * (2 * nlo + nrows) * (ld + 64 * 16);                // stream schedule slots (slice padding bound)
+    f += 2 * 8 * len * (precision == CDG_FP64 ? 2 : 1);          // tagged exchange vectors
+    return f;
+}
 
 extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_params *params, int64_t *bytes) {
     if (!req || !params || !bytes) return CDG_ERR_ARG;
@@ -117,14 +157,8 @@ extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_par
     for (int32_t k : kind) nloc += (k == CDG_KIND_LOCAL);
     const int64_t n = block_n(*req), ld = slot_elems((int)n), len = nrows * n;
     const int64_t nlo = (nloc - nrows) / 2;
-    const int64_t es = req->precision == CDG_FP64 ? 8 : (req->precision == CDG_FP32 ? 4 : 2);
     int64_t f = 8 * nnzb * ld;                                   // A
-    f += es * (nlo + nrows) * ld;                                // L, Dinv
-    if (req->precision != CDG_FP64) f += es * nlo * ld;         // U copy
-    if (req->precision != CDG_BF16) f += es * nlo * ld;         // W = U Dinv (folded backward sweep)
-    if (req->precision == CDG_BF16) f += 4 * (nlo + nrows) * ld; // FP32 factor work
-    if (n > 192) f += 8 * nrows * ld;                            // inverse working slots (bound)
-    f += 3 * 8 * nrows * 8 * n;                                  // apply partial sums (<= 8 slices)
+    f += factor_extra_bytes(nrows, nlo, n, req->precision);
     const int64_t m = std::max(1, params->nkrylov);
     f += 8 * len * (2 * (m + 1) + 6);                            // V, Z, b, x, work vectors
     if (params->inner_fgmres) f += 8 * len * (2 * (std::max(1, params->inner_nkrylov) + 1) + 4);
@@ -150,9 +184,10 @@ int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result 
     if (!e) e = dev_alloc(ctx, (void **)&x, sizeof(double) * len);
     if (!e) e = launch_vec_synthetic(ctx, len, r.seed_b, 1.0, b);
     if (!e) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, r.precision, &M);
-    // several right-hand sides share the factor: fold Dinv into the upper
-    // blocks (one more GEMM per upper block, a shorter backward sweep)
-    if (!e && r.nrhs > 1 && r.precision != CDG_BF16) e = cdg_prec_set_schedule(ctx, M, CDG_SCHEDULE_FOLDED);
+    // several right-hand sides share the factor: the stream schedule (one
+    // more GEMM per upper block and the row-slice copies, then one persistent
+    // launch per apply)
+    if (!e && r.nrhs > 1) e = cdg_prec_set_schedule(ctx, M, CDG_SCHEDULE_STREAM);
     // right-hand sides k = 0.. share the matrix and the factor (updated once)
     const int nrhs = std::max(1, r.nrhs);
     cdg_fgmres_params prm = params;
@@ -181,6 +216,81 @@ int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result 
     return e;
 }
 
+bool on_device(const void *p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice;
+}
+
+// ||b|| of a host or device vector
+int vec_norm(cdg_ctx *ctx, const double *b, int64_t len, double *out) {
+    if (on_device(b)) return ctx_norm2(ctx, len, b, out);
+    double t = 0.0;
+    for (int64_t k = 0; k < len; ++k) t += b[k] * b[k];
+    *out = std::sqrt(t);
+    return CDG_OK;
+}
+
+// A request on a registered system, on its home worker: the factor is
+// created at the first request and refactored only when the matrix stamp
+// changed since (update_preconditioner = 0: cdg_fgmres_solve compares the
+// stamps); the right-hand sides share it.
+int run_system_job(cdg_server *s, cdg_ctx *ctx, Job &job, cdg_result &res) {
+    System *sys;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->systems.find(job.system);
+        if (it == s->systems.end()) return cdg_fail(ctx, CDG_ERR_ARG, "system %llu is not registered",
+                                                    (unsigned long long)job.system);
+        sys = &it->second;   // only its home worker (this thread) touches M
+    }
+    const cdg_mat *A = sys->A;
+    int e = CDG_OK;
+    if (!sys->M) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, sys->precision, &sys->M);
+    const int64_t len = (int64_t)A->nrows * A->n;
+    double *xs = nullptr;
+    if (!e && !job.x_out) e = dev_alloc(ctx, (void **)&xs, sizeof(double) * (size_t)std::max<int64_t>(len, 1));
+    cdg_fgmres_params prm = s->params;
+    prm.update_preconditioner = 0;
+    res.niter = 0;
+    res.residual_norm = 0.0;
+    res.factored = 0;
+    for (int k = 0; k < std::max(1, job.nrhs) && !e; ++k) {
+        const double *bk = job.b + (size_t)k * len;
+        double *xk = job.x_out ? job.x_out + (size_t)k * len : xs;
+        cdg_fgmres_stats st{};
+        e = cdg_fgmres_solve(ctx, A, sys->M, bk, xk, &prm, &st);
+        double bn = 0.0;
+        if (!e) e = vec_norm(ctx, bk, len, &bn);
+        res.niter += st.niter;
+        res.residual_norm = std::max(res.residual_norm, bn > 0 ? st.residual_norm / bn : st.residual_norm);
+        res.t_factor += st.t_factor;
+        res.factored += st.factored;
+    }
+    if (xs) dev_free(ctx, xs, 0);
+    return e;
+}
+
+// Unregistering: the factor goes with the home worker's context.
+int release_system(cdg_server *s, cdg_ctx *ctx, Job &job) {
+    cdg_prec *M = nullptr;
+    int64_t resident = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->systems.find(job.system);
+        if (it == s->systems.end()) return CDG_OK;
+        M = it->second.M;
+        resident = it->second.resident;
+        s->systems.erase(it);
+        s->in_use -= resident;
+    }
+    if (M) cdg_prec_destroy(ctx, M);
+    return CDG_OK;
+}
+
 void worker(cdg_server *s, int slot) {
     cdg_ctx *ctx = nullptr;
     const int ce = cdg_ctx_create(s->device, &ctx);
@@ -188,15 +298,20 @@ void worker(cdg_server *s, int slot) {
         Job job;
         {
             std::unique_lock<std::mutex> lk(s->mu);
+            // the first queued job this worker may run (any worker, or this
+            // one as a registered system's home), admitted when it fits
+            std::deque<Job>::iterator pick;
             s->cv_work.wait(lk, [&] {
-                if (s->stop && s->queue.empty()) return true;
-                if (s->queue.empty()) return false;
-                const int64_t f = s->queue.front().footprint;
-                return s->in_use + f <= s->budget || s->active == 0;
+                pick = s->queue.end();
+                for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+                    if (it->home < 0 || it->home == slot) { pick = it; break; }
+                if (s->stop && pick == s->queue.end()) return true;
+                if (pick == s->queue.end()) return false;
+                return s->in_use + pick->footprint <= s->budget || s->active == 0;
             });
-            if (s->queue.empty()) break;
-            job = s->queue.front();
-            s->queue.pop_front();
+            if (pick == s->queue.end()) break;
+            job = *pick;
+            s->queue.erase(pick);
             s->in_use += job.footprint;
             s->active += 1;
             s->peak_active = std::max(s->peak_active, s->active);
@@ -208,6 +323,8 @@ void worker(cdg_server *s, int slot) {
         int e;
         if (ce) e = ce;
         else if (job.footprint > s->budget) e = CDG_ERR_OOM;
+        else if (job.kind == JOB_SYSTEM) e = run_system_job(s, ctx, job, res);
+        else if (job.kind == JOB_RELEASE) e = release_system(s, ctx, job);
         else e = run_job(ctx, s->params, job, res);
         const clk::time_point t_end = clk::now();
         res.status = e;
@@ -224,6 +341,14 @@ void worker(cdg_server *s, int slot) {
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
+    }
+    {   // factors of systems still registered at shutdown live on this context
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto &kv : s->systems)
+            if (kv.second.home == slot && kv.second.M) {
+                cdg_prec_destroy(ctx, kv.second.M);
+                kv.second.M = nullptr;
+            }
     }
     if (ctx) cdg_ctx_destroy(ctx);
 }
@@ -248,6 +373,7 @@ extern "C" int cdg_server_create(int32_t device, int32_t nslots, int64_t hbm_bud
         hbm_budget = (int64_t)(0.8 * (double)fr);
     }
     s->budget = hbm_budget;
+    s->nslots = nslots;
     for (int i = 0; i < nslots; ++i) s->workers.emplace_back(worker, s, i);
     *out = s;
     return CDG_OK;
@@ -269,8 +395,79 @@ extern "C" int cdg_server_submit(cdg_server *s, const cdg_request *req, double *
         s->queue.push_back(job);
         *ticket = job.ticket;
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return CDG_OK;
+}
+
+extern "C" int cdg_server_register(cdg_server *s, const cdg_mat *A, int32_t precision, uint64_t *system) {
+    if (!s || !A || !system) return CDG_ERR_ARG;
+    if (precision < CDG_FP64 || precision > CDG_BF16) return CDG_ERR_ARG;
+    if (A->struct_err != CDG_OK) return A->struct_err;
+    System sys;
+    sys.A = A;
+    sys.precision = precision;
+    sys.resident = factor_extra_bytes(A->nrows, (int64_t)A->lo_blk.size(), A->n, precision);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return CDG_ERR_STATE;
+    if (s->in_use + sys.resident > s->budget) return CDG_ERR_OOM;
+    const uint64_t id = s->next_system++;
+    sys.home = (int)((id - 1) % (uint64_t)s->nslots);
+    s->in_use += sys.resident;
+    s->systems[id] = sys;
+    *system = id;
+    return CDG_OK;
+}
+
+extern "C" int cdg_server_submit_system(cdg_server *s, uint64_t system, const double *b, int32_t nrhs, double *x_out,
+                                        uint64_t *ticket) {
+    if (!s || !b || !ticket || nrhs < 1) return CDG_ERR_ARG;
+    Job job;
+    job.kind = JOB_SYSTEM;
+    job.system = system;
+    job.b = b;
+    job.nrhs = nrhs;
+    job.x_out = x_out;
+    job.t_submit = clk::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return CDG_ERR_STATE;
+        auto it = s->systems.find(system);
+        if (it == s->systems.end()) return CDG_ERR_ARG;
+        job.home = it->second.home;
+        const cdg_mat *A = it->second.A;
+        const int64_t len = (int64_t)A->nrows * A->n, m = std::max(1, s->params.nkrylov);
+        job.footprint = 8 * len * (2 * (m + 1) + 8);            // Krylov basis, staging, work vectors
+        if (s->params.inner_fgmres) job.footprint += 8 * len * (2 * (std::max(1, s->params.inner_nkrylov) + 1) + 4);
+        job.ticket = s->next_ticket++;
+        s->results[job.ticket] = Slot{};
+        s->queue.push_back(job);
+        *ticket = job.ticket;
+    }
+    s->cv_work.notify_all();
+    return CDG_OK;
+}
+
+extern "C" int cdg_server_unregister(cdg_server *s, uint64_t system) {
+    if (!s) return CDG_ERR_ARG;
+    Job job;
+    job.kind = JOB_RELEASE;
+    job.system = system;
+    job.x_out = nullptr;
+    job.footprint = 0;
+    job.t_submit = clk::now();
+    uint64_t ticket;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->systems.find(system);
+        if (it == s->systems.end()) return CDG_ERR_ARG;
+        job.home = it->second.home;
+        job.ticket = ticket = s->next_ticket++;
+        s->results[ticket] = Slot{};
+        s->queue.push_back(job);   // behind the system's queued requests
+    }
+    s->cv_work.notify_all();
+    cdg_result res{};
+    return cdg_server_wait(s, ticket, &res);
 }
 
 extern "C" int cdg_server_wait(cdg_server *s, uint64_t ticket, cdg_result *res) {

@@ -60,6 +60,7 @@ class Result:
     t_latency: float
     t_factor: float
     slot: int
+    factored: int = 0
     x: np.ndarray | None = None
 
 
@@ -78,6 +79,8 @@ class Server:
             raise _lib.CdgError(code, "cdg_server_create", "")
         self.h = h
         self._out = {}
+        self._args = {}      # inputs of queued requests (kept alive until their wait)
+        self._keep = {}      # matrices of registered systems
 
     def footprint(self, req: Request) -> int:
         b = C.c_int64()
@@ -104,11 +107,45 @@ class Server:
         if code != _lib.CDG_OK:
             raise _lib.CdgError(code, "cdg_server_wait", "")
         x = self._out.pop(ticket, None)
-        res = Result(r.status, r.niter, r.residual_norm, r.t_queue, r.t_run, r.t_latency, r.t_factor, r.slot, x)
+        self._args.pop(ticket, None)
+        res = Result(r.status, r.niter, r.residual_norm, r.t_queue, r.t_run, r.t_latency, r.t_factor, r.slot,
+                     r.factored, x)
         if res.status != _lib.CDG_OK:
             msg = (self.lib.cdg_server_last_error(self.h) or b"").decode()
             raise _lib.CdgError(res.status, "request", msg)
         return res
+
+    def register(self, A, precision: str = "fp32") -> int:
+        """A caller's matrix (``linsolve.chidg_matrix``, stored) as a resident
+        system: its ILU(0) factor stays on the device between requests and is
+        recomputed only when the matrix is stored again (a new stamp)."""
+        sid = C.c_uint64()
+        code = self.lib.cdg_server_register(self.h, A.h, _PRECISION[precision], C.byref(sid))
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_register", "")
+        self._keep.setdefault(sid.value, []).append(A)
+        return sid.value
+
+    def submit_system(self, system: int, b, want_x: bool = False) -> int:
+        """Right-hand sides b (one vector, or nrhs of them as rows of a 2-D
+        array; host float64) on a registered system."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        x = np.empty_like(b) if want_x else None
+        t = C.c_uint64()
+        xp = x.ctypes.data_as(C.c_void_p) if x is not None else None
+        code = self.lib.cdg_server_submit_system(self.h, system, b.ctypes.data_as(C.c_void_p), nrhs, xp, C.byref(t))
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_submit_system", "")
+        self._out[t.value] = x
+        self._args[t.value] = b
+        return t.value
+
+    def unregister(self, system: int) -> None:
+        code = self.lib.cdg_server_unregister(self.h, system)
+        if code != _lib.CDG_OK:
+            raise _lib.CdgError(code, "cdg_server_unregister", "")
+        self._keep.pop(system, None)
 
     def info(self) -> dict:
         q, a, pk = C.c_int32(), C.c_int32(), C.c_int32()

@@ -251,6 +251,8 @@ typedef struct {
     double t_latency;          /* seconds from submit to completion */
     double t_factor;           /* ILU(0) update inside t_run */
     int32_t slot;              /* worker that ran the request */
+    int32_t factored;          /* ILU(0) updates the request made (a registered
+                                  system's resident factor: 0 while its stamp holds) */
 } cdg_result;
 int cdg_server_create(int32_t device, int32_t nslots, int64_t hbm_budget, const cdg_fgmres_params *params,
                       cdg_server **out);
@@ -260,6 +262,24 @@ int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_params *params
 int cdg_server_info(cdg_server *s, int32_t *queued, int32_t *active, int32_t *peak_active, int64_t *in_use,
                     int64_t *budget);
 const char *cdg_server_last_error(cdg_server *s);
+/* Requests on a caller's matrix.  cdg_server_register: A (created and stored
+ * by the caller, any context of the server's device) becomes a registered
+ * system with an ILU(0) factor of `precision` that stays resident on the
+ * device between requests; its bytes are reserved against the budget while
+ * registered (CDG_ERR_OOM when they do not fit).  The factor is computed at
+ * the first request and again only when A's stamp changed (cdg_mat_upload),
+ * the rule of solver_controller_t (type_solver_controller.f90:89-124).  The
+ * requests of one system run in submission order on one worker; the caller
+ * does not store into A while its requests run, and keeps A alive until
+ * cdg_server_unregister returns.  cdg_server_submit_system: nrhs right-hand
+ * sides (nrhs * nrows * n doubles, host or device, valid until the wait
+ * returns) sharing the factor; x_out (NULL: discarded) receives the nrhs
+ * solutions.  cdg_server_unregister waits for the system's queued requests
+ * and frees its factor. */
+int cdg_server_register(cdg_server *s, const cdg_mat *A, int32_t precision, uint64_t *system);
+int cdg_server_submit_system(cdg_server *s, uint64_t system, const double *b, int32_t nrhs, double *x_out,
+                             uint64_t *ticket);
+int cdg_server_unregister(cdg_server *s, uint64_t system);
 int cdg_server_destroy(cdg_server *s);
 
 /* ---- DG Euler first-Newton-step system (host code) --------------------------

@@ -58,17 +58,95 @@ def test_poisson_burst_completes():
 
 
 def test_multi_rhs_shares_factor():
-    """nrhs right-hand sides on one factor (folded schedule: Dinv folded into
-    the upper blocks): the first agrees with a one-RHS request up to the
-    re-association of the backward products, every one converges, and the
-    same request gives the same bits again."""
+    """nrhs right-hand sides on one factor (stream schedule: W = Dinv U and
+    the row-slice copies formed once): the first agrees with a one-RHS request
+    (level schedule) to the distance of two solutions at rtol 1e-8, every one
+    converges, and the same request gives the same bits again."""
     one = Request(6, 6, 6, 2, seed_A=301, seed_b=11)
     three = Request(6, 6, 6, 2, seed_A=301, seed_b=11, nrhs=3)
     with Server(nslots=2, **KW) as s:
         a = s.wait(s.submit(one, want_x=True))
         b = s.wait(s.submit(three, want_x=True))
         c = s.wait(s.submit(three, want_x=True))
-    assert np.linalg.norm(a.x - b.x) / np.linalg.norm(a.x) < 1e-10
+    assert np.linalg.norm(a.x - b.x) / np.linalg.norm(a.x) < 1e-6
     assert np.array_equal(b.x, c.x)
     assert b.niter >= 3 * 1 and b.niter >= a.niter
     assert b.residual_norm < 1e-8
+
+
+def test_registered_system_keeps_its_factor():
+    """A caller's matrix registered with the server: the first request
+    factors, later ones reuse the resident factor, a store()
+    that re-stamps the matrix refactors; multi-RHS requests share it; the
+    solutions match the oracle."""
+    import pyoracle as po
+    from chidg_amd import euler
+    from chidg_amd.linsolve import Context, chidg_matrix
+    p, blocks, R, _ = euler.system(4, 1, mach=0.3, seed=3, perturb=0.01, cfl=1.0)
+    ctx = Context(0)
+    A = chidg_matrix(p, ctx)
+    A.store(blocks)
+    rng = np.random.default_rng(4)
+    bs = rng.standard_normal((3, p.vec_len))
+    x_ref = [po.fgmres(p, blocks, bk, nkrylov=30, tol=0.0, rtol=1e-8)[0] for bk in bs]
+    with Server(nslots=3, **KW) as s:
+        sid = s.register(A, "fp64")
+        r1 = s.wait(s.submit_system(sid, bs[0], want_x=True))
+        r2 = s.wait(s.submit_system(sid, bs[0], want_x=True))
+        A.store(blocks)                                   # a new stamp
+        r3 = s.wait(s.submit_system(sid, bs[0], want_x=True))
+        r4 = s.wait(s.submit_system(sid, bs, want_x=True))   # three right-hand sides
+        s.unregister(sid)
+    assert (r1.factored, r2.factored, r3.factored, r4.factored) == (1, 0, 1, 0)
+    # the same factor; the apply schedule may differ between the solves (the
+    # default switches to the stream sweep once a factor has served 12 applies)
+    for r in (r2, r3):
+        assert np.linalg.norm(r.x - r1.x) / np.linalg.norm(r1.x) < 1e-12
+    for k in range(3):
+        assert np.linalg.norm(r4.x[k] - x_ref[k]) / np.linalg.norm(x_ref[k]) < 1e-8
+    assert max(r.residual_norm for r in (r1, r2, r3, r4)) < 1e-8
+    A.release()
+    ctx.close()
+
+
+def test_registered_systems_mixed_with_synthetic_load():
+    """Two registered systems (FP32 and BF16 factors) interleaved with
+    synthetic requests on four slots: every result converges, each system
+    factors once, and its repeated requests agree with the first."""
+    from chidg_amd import pattern as pt
+    from chidg_amd.linsolve import Context, chidg_matrix
+    import pyoracle as po
+    ctx = Context(0)
+    mats = []
+    for k, (N, P) in enumerate(((5, 2), (6, 1))):
+        n = pt.block_size(P)
+        pk = pt.hex_box(N, N, N, n)
+        Ak = chidg_matrix(pk, ctx)
+        Ak.store(po.fill_synthetic(pk, 900 + k, 1 / np.sqrt(n), 0.1 / np.sqrt(n)))
+        mats.append((pk, Ak))
+    rng = np.random.default_rng(8)
+    with Server(nslots=4, **KW) as s:
+        sids = [s.register(mats[0][1], "fp32"), s.register(mats[1][1], "bf16")]
+        bs = [rng.standard_normal(m[0].vec_len) for m in mats]
+        tickets = []
+        for rep in range(3):
+            for sid, b in zip(sids, bs):
+                tickets.append(("sys", sid, s.submit_system(sid, b, want_x=True)))
+            for r in sweep_requests(3, rng, meshes=(4, 6)):
+                tickets.append(("syn", None, s.submit(r)))
+        first = {}
+        for kind, sid, t in tickets:
+            res = s.wait(t)
+            assert res.residual_norm < 1e-8
+            if kind == "sys":
+                if sid in first:
+                    assert res.factored == 0
+                    assert np.linalg.norm(res.x - first[sid]) / np.linalg.norm(first[sid]) < 1e-6
+                else:
+                    assert res.factored == 1
+                    first[sid] = res.x
+        for sid in sids:
+            s.unregister(sid)
+    for _, Ak in mats:
+        Ak.release()
+    ctx.close()
