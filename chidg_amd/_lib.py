@@ -15,7 +15,7 @@ CDG_OK = 0
 ERRORS = {1: "CDG_ERR_ARG", 2: "CDG_ERR_HIP", 3: "CDG_ERR_SINGULAR", 4: "CDG_ERR_NO_TRANSPOSE",
           5: "CDG_ERR_OOM", 6: "CDG_ERR_NAN", 7: "CDG_ERR_STATE", 8: "CDG_ERR_UNSUPPORTED"}
 KIND_LOCAL, KIND_COUPLING = 0, 1
-PREC_IDENTITY, PREC_ILU0 = 0, 1
+PREC_IDENTITY, PREC_ILU0, PREC_JACOBI = 0, 1, 2
 FP64, FP32, BF16 = 0, 1, 2
 CGS, MGS = 0, 1
 

@@ -634,7 +634,8 @@ extern "C" int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M) {
 extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int32_t precision, cdg_prec **out) {
     if (!ctx || !A || !out) return CDG_ERR_ARG;
     *out = nullptr;
-    if (type != CDG_PREC_IDENTITY && type != CDG_PREC_ILU0) return cdg_fail(ctx, CDG_ERR_ARG, "unknown preconditioner %d", type);
+    if (type != CDG_PREC_IDENTITY && type != CDG_PREC_ILU0 && type != CDG_PREC_JACOBI)
+        return cdg_fail(ctx, CDG_ERR_ARG, "unknown preconditioner %d", type);
     if (precision < CDG_FP64 || precision > CDG_BF16) return cdg_fail(ctx, CDG_ERR_ARG, "unknown precision %d", precision);
     cdg_prec *M = new cdg_prec();
     M->type = type;
@@ -642,14 +643,34 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     M->nrows = A->nrows;
     M->n = A->n;
     if (type == CDG_PREC_IDENTITY) { *out = M; return CDG_OK; }
-    if (A->struct_err != CDG_OK) {
+    // The kind-0 structure the factor follows: A's, or for block Jacobi
+    // (precon_jacobi.f90:95-215) the diagonal alone -- ILU(0) of a matrix
+    // whose every row is its own rank: L = 0, U = 0, Dinv_i = inv(A_ii), and
+    // the apply is z_i = Dinv_i v_i.
+    cdg_mat jview;
+    const cdg_mat *S = A;
+    if (type == CDG_PREC_JACOBI) {
+        jview.nrows = A->nrows;
+        jview.n = A->n;
+        jview.nnzb = A->nnzb;
+        jview.row_ptr = A->row_ptr;
+        jview.col = A->col;
+        jview.kind = A->kind;
+        jview.part.resize(A->nrows);
+        for (int i = 0; i < A->nrows; ++i) jview.part[i] = i;
+        analyze(&jview);
+        S = &jview;
+        M->type = CDG_PREC_ILU0;   // the ILU(0) machinery with no off-diagonal blocks
+        M->jacobi = 1;
+    }
+    if (S->struct_err != CDG_OK) {
         delete M;
-        return cdg_fail(ctx, A->struct_err, A->struct_err == CDG_ERR_NO_TRANSPOSE
+        return cdg_fail(ctx, S->struct_err, S->struct_err == CDG_ERR_NO_TRANSPOSE
                                                 ? "lblks entry without a transposed partner"
                                                 : "block row without a diagonal block");
     }
-    M->nlo = (int32_t)A->lo_blk.size();
-    M->nup = (int32_t)A->up_blk.size();
+    M->nlo = (int32_t)S->lo_blk.size();
+    M->nup = (int32_t)S->up_blk.size();
     M->ld = slot_elems(A->n);
     const size_t es = precision == CDG_FP64 ? 8 : (precision == CDG_FP32 ? 4 : 2);
     auto alloc = [&](void **p, size_t bytes) -> int {
@@ -667,37 +688,37 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     }
     // structure arrays
     std::vector<int32_t> up_idx(M->nup);
-    for (int u = 0; u < M->nup; ++u) up_idx[u] = precision == CDG_FP64 ? A->up_blk[u] : u;
-    if (!e) e = upload_array(ctx, &M->d_lo_ptr, A->lo_ptr, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_lo_blk, A->lo_blk, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_lo_col, A->lo_col, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_lo_tr, A->lo_tr, &M->owned);
+    for (int u = 0; u < M->nup; ++u) up_idx[u] = precision == CDG_FP64 ? S->up_blk[u] : u;
+    if (!e) e = upload_array(ctx, &M->d_lo_ptr, S->lo_ptr, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lo_blk, S->lo_blk, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lo_col, S->lo_col, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_lo_tr, S->lo_tr, &M->owned);
     if (!e && precision != CDG_FP64) {
         // A_ji (j < i) is an upper block of row j, so every lo_tr[q] has a U slot
-        std::vector<int32_t> slot_of(std::max(A->nnzb, 1), -1), lo_tru(A->lo_tr.size());
-        for (int u = 0; u < M->nup; ++u) slot_of[A->up_blk[u]] = u;
-        for (size_t q = 0; q < lo_tru.size(); ++q) lo_tru[q] = slot_of[A->lo_tr[q]];
+        std::vector<int32_t> slot_of(std::max(A->nnzb, 1), -1), lo_tru(S->lo_tr.size());
+        for (int u = 0; u < M->nup; ++u) slot_of[S->up_blk[u]] = u;
+        for (size_t q = 0; q < lo_tru.size(); ++q) lo_tru[q] = slot_of[S->lo_tr[q]];
         e = upload_array(ctx, &M->d_lo_tru, lo_tru, &M->owned);
     }
-    if (!e) e = upload_array(ctx, &M->d_up_ptr, A->up_ptr, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_up_blk, A->up_blk, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_ptr, S->up_ptr, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_blk, S->up_blk, &M->owned);
     if (!e) e = upload_array(ctx, &M->d_up_idx, up_idx, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_up_col, A->up_col, &M->owned);
-    if (!e) e = upload_array(ctx, &M->d_diag, A->diag, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_up_col, S->up_col, &M->owned);
+    if (!e) e = upload_array(ctx, &M->d_diag, S->diag, &M->owned);
     // forward levels (factor schedule and forward sweep), backward levels
     std::vector<int32_t> fptr, bptr;
-    std::vector<int32_t> forder = order_by_level(A->flevel, A->nflev, &fptr);
-    std::vector<int32_t> border = order_by_level(A->blevel, A->nblev, &bptr);
-    M->nlev = A->nflev;
+    std::vector<int32_t> forder = order_by_level(S->flevel, S->nflev, &fptr);
+    std::vector<int32_t> border = order_by_level(S->blevel, S->nblev, &bptr);
+    M->nlev = S->nflev;
     M->lev_row_ptr = fptr;
-    M->nblev = A->nblev;
+    M->nblev = S->nblev;
     M->blev_row_ptr = bptr;
     std::vector<int32_t> lev_lo;
     M->lev_lo_ptr.assign(M->nlev + 1, 0);
     for (int l = 0; l < M->nlev; ++l) {
         for (int k = fptr[l]; k < fptr[l + 1]; ++k) {
             const int i = forder[k];
-            for (int q = A->lo_ptr[i]; q < A->lo_ptr[i + 1]; ++q) lev_lo.push_back(q);
+            for (int q = S->lo_ptr[i]; q < S->lo_ptr[i + 1]; ++q) lev_lo.push_back(q);
         }
         M->lev_lo_ptr[l + 1] = (int32_t)lev_lo.size();
     }
@@ -717,8 +738,8 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
     if (e) { cdg_prec_destroy(ctx, M); return e; }
     M->maxdep = 1;
     for (int i = 0; i < A->nrows; ++i) {
-        M->maxdep = std::max(M->maxdep, A->lo_ptr[i + 1] - A->lo_ptr[i]);
-        M->maxdep = std::max(M->maxdep, A->up_ptr[i + 1] - A->up_ptr[i]);
+        M->maxdep = std::max(M->maxdep, S->lo_ptr[i + 1] - S->lo_ptr[i]);
+        M->maxdep = std::max(M->maxdep, S->up_ptr[i + 1] - S->up_ptr[i]);
     }
     // Apply launch shape: the tuned one for this block size and precision if
     // cdg_prec_tune has run, else the default; CDG_APPLY_SLICE_W /
@@ -742,10 +763,10 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
         else if (!std::strcmp(m, "stream")) M->apply_mode = CDG_APPLY_STREAM;
     }
     M->h_lev_rows = forder;
-    M->h_lo_ptr = A->lo_ptr;
-    M->h_lo_col = A->lo_col;
-    M->h_up_ptr = A->up_ptr;
-    M->h_up_col = A->up_col;
+    M->h_lo_ptr = S->lo_ptr;
+    M->h_lo_col = S->lo_col;
+    M->h_up_ptr = S->up_ptr;
+    M->h_up_col = S->up_col;
     M->h_blev_rows = border;
     // (+2: a sweep launch whose rows have no entries issues its 16-byte
     // block prefetch from the start of a partial array, unused)

@@ -88,6 +88,7 @@ struct cdg_mat {
 
 struct cdg_prec {
     int32_t type = CDG_PREC_IDENTITY;
+    int32_t jacobi = 0;                // created as CDG_PREC_JACOBI (ILU(0) with no off-diagonal blocks)
     int32_t precision = CDG_FP64;
     int32_t nrows = 0, n = 0, nlo = 0, nup = 0;
     int64_t ld = 0;                    // elements per factor slot

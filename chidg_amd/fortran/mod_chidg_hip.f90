@@ -17,7 +17,7 @@ module mod_chidg_hip
     private
 
     integer(c_int32_t), parameter, public :: HIP_KIND_LOCAL = 0, HIP_KIND_COUPLING = 1
-    integer(c_int32_t), parameter, public :: HIP_PREC_IDENTITY = 0, HIP_PREC_ILU0 = 1
+    integer(c_int32_t), parameter, public :: HIP_PREC_IDENTITY = 0, HIP_PREC_ILU0 = 1, HIP_PREC_JACOBI = 2
     integer(c_int32_t), parameter, public :: HIP_FP64 = 0, HIP_FP32 = 1, HIP_BF16 = 2
     integer(c_int32_t), parameter, public :: HIP_CGS = 0, HIP_MGS = 1
     ! ILU(0) apply schedules (cdg_prec_set_schedule); AUTO is the default

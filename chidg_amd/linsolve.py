@@ -346,11 +346,19 @@ class precon_ILU0(_Preconditioner):
     kind = _lib.PREC_ILU0
 
 
+class precon_jacobi(_Preconditioner):
+    """Block Jacobi (``precon_jacobi_t``, precon_jacobi.f90:95-215):
+    ``update`` inverts every diagonal block, ``apply`` is z_i = Dinv_i v_i."""
+    kind = _lib.PREC_JACOBI
+
+
 def create_preconditioner(name: str, precision: str = "fp64") -> _Preconditioner:
     """Factory with the reference's names (mod_preconditioner.f90:49-108)."""
     key = name.strip().lower()
     if key == "ilu0":
         return precon_ILU0(precision)
+    if key in ("jacobi", "blockjacobi"):
+        return precon_jacobi(precision)
     if key == "identity":
         return precon_identity(precision)
     raise ValueError(f"unknown preconditioner {name!r}")

@@ -47,6 +47,8 @@ extern "C" {
 /* ---- preconditioners (mod_preconditioner.f90:49-108 factory names) -------- */
 #define CDG_PREC_IDENTITY    0   /* 'identity'                                    */
 #define CDG_PREC_ILU0        1   /* 'ILU0'                                        */
+#define CDG_PREC_JACOBI      2   /* 'jacobi' (precon_jacobi.f90:95-215): block
+                                    Jacobi, Dinv_i = inv(A_ii), z_i = Dinv_i v_i  */
 
 /* ---- storage precision of the ILU(0) factor -------------------------------- */
 #define CDG_FP64             0   /* reference precision                           */
