@@ -991,12 +991,12 @@ extern "C" int cdg_mat_launch_shape(const cdg_mat *A, int32_t *out) {
 // A dataflow sweep that gave up waiting leaves its counters unusable: report
 // it and reset them (called from the synchronizing entry points).
 static int df_check(cdg_ctx *ctx, cdg_prec *M) {
-    if (M->d_rs_err) {   // the stream sweep: an abandoned wait left NaN in z
-        int32_t err = 0;
-        CDG_HIP_TRY(hipMemcpyAsync(&err, M->d_rs_err, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
+    if (M->d_rs_ctl) {   // the stream sweep: an abandoned wait left NaN in z
+        uint32_t err = 0;
+        CDG_HIP_TRY(hipMemcpyAsync(&err, M->d_rs_ctl, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
         CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (err) {
-            CDG_HIP_TRY(hipMemsetAsync(M->d_rs_err, 0, sizeof(int32_t), ctx->stream));
+            CDG_HIP_TRY(hipMemsetAsync(M->d_rs_ctl, 0, sizeof(uint32_t), ctx->stream));
             return cdg_fail(ctx, CDG_ERR_HIP, "ILU(0) stream sweep gave up waiting for a dependency");
         }
     }

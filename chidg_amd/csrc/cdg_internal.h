@@ -155,7 +155,7 @@ struct cdg_prec {
     int32_t *rs_tasks = nullptr;       // device task records
     int32_t *d_up_row = nullptr;       // row of each upper entry
     void *rs_yb = nullptr, *rs_zb = nullptr;
-    int32_t *d_rs_err = nullptr;
+    uint32_t *d_rs_ctl = nullptr;      // error word + ticket counters of the sweep
     uint32_t rs_epoch = 0;             // tag of the last call
     int32_t rs_ready = 0;              // Lrs/Wrs/Drs hold the current factor
     int32_t rs_ok = 1;                 // 0: the pattern does not fit the stream schedule (AUTO keeps levels)
@@ -181,7 +181,8 @@ int rs_relayout(cdg_ctx *ctx, cdg_prec *M);
 int launch_fold_rs(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A);
 int launch_ilu_apply_rs(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
 int launch_ilu_apply_rs_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb,
-                            int32_t *err, uint32_t tag);
+                            uint32_t *ctl, uint32_t tag);
+int rs_ctl_words();
 int rs_slice_rows(int n, int wmax);
 void rs_ctx_count(int delta);
 // W_u = A_{up_blk[u]} Dinv_{up_col[u]} for all upper blocks (after a factor)

@@ -15,26 +15,32 @@
 // partial sums travel between tasks.
 //
 // Scheduling: tasks are listed in a topological order (forward levels, then
-// backward levels) and dealt round-robin to the G workgroups of one launch
-// (task t to workgroup t mod G), each of which takes its tasks in order.  G
-// never exceeds the number of workgroups the device holds at once, so all of
-// them are resident: the lowest unfinished task's inputs are finished and its
-// workgroup has nothing before it, so the sweep always progresses.  Each wait
-// is bounded nevertheless: past the limit a workgroup raises the error word,
-// writes NaN into the outputs of its remaining backward tasks and leaves; the
-// others leave at their next check.
+// backward levels).  Task t belongs to class t mod NC (NC <= 8, one class per
+// XCD when the grid is dealt round-robin over the XCDs); workgroup g serves
+// class g mod NC and takes that class's tasks in list order from the class's
+// ticket counter (one agent-scope atomic per task, fetched two tasks ahead).
+// Tickets go out in order to running workgroups only, so the sweep makes
+// progress whatever the number of resident workgroups: the lowest unfinished
+// task has finished parents and is held, or is next to be taken, by a running
+// workgroup of its class.  (A fixed round-robin deal instead needs every
+// workgroup resident at once, which the occupancy figure does not guarantee.)
+// Two counter sets alternate between calls: workgroup 0 clears the set the
+// next call uses.  Each wait is bounded nevertheless: past the limit a
+// workgroup raises the error word; every workgroup that sees it drains its
+// class's remaining tickets without waiting, writing NaN into the outputs of
+// backward tasks, so the sweep always ends.
 //
 // Exchange between tasks: every value of y and z sits in a 64-bit word
 // together with a 32-bit tag, the number of the apply call that wrote it
 // (FP64 values take two words, each half with the tag).  A consumer loads the
 // parent words with device-coherent (agent-scope) atomic loads and accepts
 // them once every tag is the current call's; a 64-bit aligned access is
-// single-copy atomic, so value and tag are seen together.  No counters, no
-// fences, nothing to reset between calls.
+// single-copy atomic, so value and tag are seen together.  No completion
+// counters, no fences, nothing to reset between calls but the ticket set.
 //
-// Software pipeline per workgroup: the block slices of task t+G are loaded
-// into registers while task t reduces and stores, so a task that waits for
-// its parents already holds its blocks.
+// Software pipeline per workgroup: the block slices of the next task are
+// loaded into registers while the current task reduces and stores, so a task
+// that waits for its parents already holds its blocks.
 #include "cdg_internal.h"
 
 #include <algorithm>
@@ -117,7 +123,7 @@ template <> struct Tagged<double> {
 struct RsArgs {
     const int32_t *tasks;     // kRsRec ints per task (see rs_build_tasks)
     int32_t ntasks;
-    const void *Lb, *Wb, *Db; // row-slice slots: L (forward), W (backward), Dinv (backward block 0)
+    const void *Sb;           // row-slice slots: nlo L, nup W, nrows Dinv (one allocation)
     int64_t slot;             // elements per slot
     int32_t SL;               // elements per slice
     int32_t n, w, dc;         // block size, rows per slice, columns per walk step
@@ -125,32 +131,59 @@ struct RsArgs {
     double *z;                // the result
     uint64_t *yb, *zb;        // tagged forward / backward results
     uint32_t tag;
-    int32_t *err;
+    uint32_t *ctl;            // control words: kRsCtlWords (error word, ticket counters)
+    int32_t nclass;           // ticket classes NC (<= 8)
     int32_t spin_limit;
 };
 
 }  // namespace
 
 // Task record: [0] row, [1] slice | nblocks << 8 | backward << 16,
-// [2 + k] slot of block k, [2 + kRsMaxBlocks + k] parent row of block k.
-// Backward tasks: block 0 is the row's Dinv (parent: the row's own y), blocks
-// 1.. the W of its upper entries (parents: their z).
+// [2 + k] slot of block k in the row-slice storage (L slots 0..nlo-1, W
+// slots nlo..nlo+nup-1, Dinv slots after them), [2 + kRsMaxBlocks + k] parent
+// row of block k.  Backward tasks: block 0 is the row's Dinv (parent: the
+// row's own y), blocks 1.. the W of its upper entries (parents: their z).
 constexpr int kRsMaxBlocks = 7;
 constexpr int kRsRec = 16;
 constexpr int kRsMaxThreads = 512;
 
 namespace {
 
+// Control words of a sweep (uint32): [0] error word; the ticket counter of
+// set s (0, 1) and class c at kRsLine * (1 + 8 s + c), one 128-byte line each.
+constexpr int kRsLine = 32;
+constexpr int kRsCtlWords = kRsLine * 17;
+
+__device__ __forceinline__ uint32_t *rs_counter(uint32_t *ctl, uint32_t set, int cls) {
+    return ctl + kRsLine * (1 + 8 * (set & 1u) + cls);
+}
+
 template <typename S, int NB, int R, int EPT>
-__global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
+__global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs args) {
     typedef Raw16<S> RW;
     typedef typename RW::Acc Acc;
     typedef typename RW::T RawT;
     typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type VT;
     typedef Tagged<VT> TG;
     constexpr int V = RW::V, WD = TG::Words;
+    // the arguments as scalars (no lambda refers to the argument block, which
+    // would otherwise be copied to per-lane scratch memory)
+    const int32_t *const tasks = args.tasks;
+    const int ntasks = args.ntasks;
+    const S *const Sb = static_cast<const S *>(args.Sb);
+    const int64_t slot = args.slot;
+    const int SL = args.SL, n = args.n, w = args.w, dc = args.dc;
+    const double *const vin = args.v;
+    double *const zout = args.z;
+    uint64_t *const yb = args.yb;
+    uint64_t *const zb = args.zb;
+    const uint32_t tag = args.tag;
+    uint32_t *const ctl = args.ctl;
+    const int NC = args.nclass, spin_limit = args.spin_limit;
+
     extern __shared__ unsigned char smem_raw[];
-    const int T = blockDim.x, tid = threadIdx.x, n = a.n, w = a.w;
+    __shared__ int s_tick[2];
+    const int T = blockDim.x, tid = threadIdx.x;
     Acc *xs = reinterpret_cast<Acc *>(smem_raw);                    // kRsMaxBlocks * n
     Acc *red = xs + (((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3);  // V*T
     const int lim = w * n;             // elements of one block slice
@@ -159,16 +192,31 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
     int col0[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) col0[v] = (V * tid + v) / w;
-    const S *Lb = static_cast<const S *>(a.Lb), *Wb = static_cast<const S *>(a.Wb), *Db = static_cast<const S *>(a.Db);
-    const int G = gridDim.x;
-    int t = blockIdx.x;
-    if (t >= a.ntasks) return;
 
-    auto slice_base = [&](const int32_t *rec, int k) -> const S * {
-        const int nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1, sl = rec[1] & 0xff;
+    const int cls = blockIdx.x % NC;
+    uint32_t *const cnt = rs_counter(ctl, tag, cls);
+    // workgroup 0 clears the counters of the next call (the previous call,
+    // which used them, has ended: calls on one buffer set are stream-ordered)
+    if (blockIdx.x == 0 && tid < 8)
+        __hip_atomic_store(rs_counter(ctl, tag + 1u, tid), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // ticket k of the class is task k NC + cls
+    auto ticket = [cnt, NC, cls]() -> int {
+        const uint32_t k = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return (int)min(k, (uint32_t)INT32_MAX / 16u) * NC + cls;
+    };
+    if (tid == 0) {
+        s_tick[0] = ticket();
+        s_tick[1] = ticket();
+    }
+    __syncthreads();
+    int t = s_tick[0], tn = s_tick[1];
+    __syncthreads();
+    if (t >= ntasks) return;
+
+    auto slice_base = [Sb, slot, SL](const int32_t *rec, int k) -> const S * {
+        const int nb = (rec[1] >> 8) & 0xff, sl = rec[1] & 0xff;
         const int kk = k < nb ? k : 0;
-        const S *base = bwd ? (kk == 0 ? Db : Wb) : Lb;
-        return base + (size_t)rec[2 + kk] * a.slot + (size_t)sl * a.SL;
+        return Sb + (size_t)rec[2 + kk] * slot + (size_t)sl * SL;
     };
     RawT raw[NB][R];
     // Every load is issued, none under a branch (a branch here would make the
@@ -176,7 +224,7 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
     // does not have reads the first 16 bytes of block 0, the same address in
     // every lane (one request per wave), and steps past the slice re-read its
     // last 16-byte group.
-    auto issue = [&](const int32_t *rec) {
+    auto issue = [&raw, &slice_base, tid, step, lim_ld](const int32_t *rec) {
         const int nb = (rec[1] >> 8) & 0xff;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
@@ -187,15 +235,19 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
                     reinterpret_cast<const RawT *>(b + (k < nb ? min(V * tid + m * step, lim_ld) : 0)));
         }
     };
-    issue(a.tasks + (size_t)t * kRsRec);
+    issue(tasks + (size_t)t * kRsRec);
     int aborted = 0;
+    int tnn = 0;
     for (;;) {
-        const int32_t *rec = a.tasks + (size_t)t * kRsRec;
+        const int32_t *rec = tasks + (size_t)t * kRsRec;
         const int row = rec[0];
         const int sl = rec[1] & 0xff, nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1;
-        const int tn = t + G;
+        // the ticket after next goes out now; it is read after the row sums
+        if (tid == 0) tnn = ticket();
         // ---- 1. parents into LDS: nb vectors of n values, all tags current
-        {
+        // (skipped once the sweep is abandoned: the workgroup then only
+        // drains its class's tickets, writing NaN for backward tasks)
+        if (!aborted) {
             // element j = k n + c (thread's j = tid + i T) is value c of parent
             // k: word offset off[i] in y (forward parents and the backward
             // row's own y, j < n) or z (backward upper parents, j >= n); the
@@ -227,14 +279,14 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
                 uint64_t wd[EPT][WD];
 #pragma unroll
                 for (int i = 0; i < EPT; ++i) {
-                    const uint64_t *src = (bwd && tid + i * T >= n ? a.zb : a.yb) + off[i];
+                    const uint64_t *src = (bwd && tid + i * T >= n ? zb : yb) + off[i];
 #pragma unroll
                     for (int q = 0; q < WD; ++q) wd[i][q] = ld_tag(src + q);
                 }
 #pragma unroll
                 for (int i = 0; i < EPT; ++i) {
                     bool ok;
-                    const VT x = TG::get(wd[i], a.tag, ok);
+                    const VT x = TG::get(wd[i], tag, ok);
                     if (ok && ((need >> i) & 1u)) {
                         const int j = tid + i * T;
                         xs[j] = (bwd && j < n) ? static_cast<Acc>(x) : -static_cast<Acc>(x);
@@ -243,26 +295,23 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
                 }
                 if (!__syncthreads_or(need != 0)) break;
                 // bounded wait; the error word, set to this call's tag by the
-                // first workgroup that gives up, ends every other one (a word
-                // left by an earlier call holds another tag and stops nothing)
+                // first workgroup that gives up, ends every other one's wait (a
+                // word left by an earlier call holds another tag and stops nothing)
                 ++spins;
-                int stop = spins > a.spin_limit;
-                if (stop && tid == 0)
-                    __hip_atomic_store(a.err, (int32_t)a.tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((spins & 63) == 0)
-                    stop |= __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (int32_t)a.tag;
+                int stop = spins > spin_limit;
+                if (stop && tid == 0) __hip_atomic_store(ctl, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((spins & 63) == 0) stop |= __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == tag;
                 if (__syncthreads_or(stop)) { aborted = 1; break; }
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        if (aborted) break;
         // ---- 2. products: this thread's elements of the nb block slices
         Acc acc[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[v] = 0;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            if (k < nb) {
+            if (k < nb && !aborted) {
                 const Acc *xk = xs + k * n;
 #pragma unroll
                 for (int m = 0; m < R; ++m) {
@@ -271,12 +320,12 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
                     RW::get(raw[k][m], x);
 #pragma unroll
                     for (int v = 0; v < V; ++v)
-                        if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * a.dc, n - 1)];
+                        if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
                 }
             }
         }
         // slices longer than R steps, rows with more than NB blocks: the rest from memory
-        if (R * step < lim || nb > NB) {
+        if ((R * step < lim || nb > NB) && !aborted) {
             for (int k = 0; k < nb; ++k) {
                 const S *b = slice_base(rec, k);
                 const Acc *xk = xs + k * n;
@@ -287,41 +336,38 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs a) {
                         RW::get(__builtin_nontemporal_load(reinterpret_cast<const RawT *>(b + e)), x);
 #pragma unroll
                         for (int v = 0; v < V; ++v)
-                            if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * a.dc, n - 1)];
+                            if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
                     }
                 }
             }
         }
         // ---- 3. the next task's blocks go out now (registers are free)
-        if (tn < a.ntasks) issue(a.tasks + (size_t)tn * kRsRec);
+        if (tn < ntasks && !aborted) issue(tasks + (size_t)tn * kRsRec);
         // ---- 4. row sums (fixed order) and the outputs of this task
 #pragma unroll
         for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
+        if (tid == 0) s_tick[0] = tnn;
         __syncthreads();
         for (int r = tid; r < w; r += T) {
             Acc s = 0;
-            for (int q = 0; q < a.dc; ++q) s += red[r + w * q];
+            for (int q = 0; q < dc; ++q) s += red[r + w * q];
             const size_t gi = (size_t)row * n + (size_t)sl * w + r;
-            if (!bwd) {
-                const double y = a.v[gi] + static_cast<double>(s);
-                TG::put(a.yb + gi * WD, y, a.tag);
+            if (aborted) {
+                if (bwd) zout[gi] = __builtin_nan("");
+            } else if (!bwd) {
+                const double y = vin[gi] + static_cast<double>(s);
+                TG::put(yb + gi * WD, y, tag);
             } else {
                 const VT zz = static_cast<VT>(s);
-                TG::put(a.zb + gi * WD, static_cast<double>(zz), a.tag);
-                a.z[gi] = static_cast<double>(zz);
+                TG::put(zb + gi * WD, static_cast<double>(zz), tag);
+                zout[gi] = static_cast<double>(zz);
             }
         }
+        tnn = s_tick[0];
         __syncthreads();
         t = tn;
-        if (t >= a.ntasks) break;
-    }
-    if (aborted) {   // this workgroup's backward outputs from here on never come: NaN
-        for (; t < a.ntasks; t += G) {
-            const int32_t *rec = a.tasks + (size_t)t * kRsRec;
-            if (!((rec[1] >> 16) & 1)) continue;
-            const size_t g0 = (size_t)rec[0] * n + (size_t)(rec[1] & 0xff) * w;
-            for (int r = tid; r < w; r += T) a.z[g0 + r] = __builtin_nan("");
-        }
+        tn = tnn;
+        if (t >= ntasks) break;
     }
 }
 
@@ -444,11 +490,11 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
                 rec[2 + kRsMaxBlocks + nb] = M->h_lo_col[q];
             }
         } else {
-            rec[2] = row;
+            rec[2] = M->nlo + M->nup + row;
             rec[2 + kRsMaxBlocks] = row;
             nb = 1;
             for (int u = M->h_up_ptr[row]; u < M->h_up_ptr[row + 1]; ++u, ++nb) {
-                rec[2 + nb] = u;
+                rec[2 + nb] = M->nlo + u;
                 rec[2 + kRsMaxBlocks + nb] = M->h_up_col[u];
             }
         }
@@ -474,12 +520,15 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
     };
     const int WD = prec == CDG_FP64 ? 2 : 1;
     const size_t vec_words = (size_t)M->nrows * n * WD;
-    int e = alloc(&M->Lrs, es * (size_t)M->nlo * slot);
-    if (!e) e = alloc(&M->Wrs, es * (size_t)M->nup * slot);
-    if (!e) e = alloc(&M->Drs, es * (size_t)M->nrows * slot);
+    // one allocation: L slots, then W slots, then Dinv slots
+    int e = alloc(&M->Lrs, es * (size_t)(M->nlo + M->nup + M->nrows) * slot);
+    if (!e) {
+        M->Wrs = static_cast<char *>(M->Lrs) + es * (size_t)M->nlo * slot;
+        M->Drs = static_cast<char *>(M->Wrs) + es * (size_t)M->nup * slot;
+    }
     if (!e) e = alloc(&M->rs_yb, 8 * vec_words);
     if (!e) e = alloc(&M->rs_zb, 8 * vec_words);
-    if (!e) e = alloc((void **)&M->d_rs_err, sizeof(int32_t));
+    if (!e) e = alloc((void **)&M->d_rs_ctl, sizeof(uint32_t) * kRsCtlWords);
     if (!e) {
         int32_t *dt = nullptr;
         const size_t bytes = sizeof(int32_t) * std::max<size_t>(tasks.size(), 1);
@@ -563,14 +612,13 @@ int rs_relayout(cdg_ctx *ctx, cdg_prec *M) {
 
 namespace {
 template <typename S>
-int rs_launch_t(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb, int32_t *err,
+int rs_launch_t(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb, uint32_t *ctl,
                 uint32_t tag) {
+    const int G = rs_grid(ctx, M);
     RsArgs a{};
     a.tasks = M->rs_tasks;
     a.ntasks = M->rs_ntasks;
-    a.Lb = M->Lrs;
-    a.Wb = M->Wrs;
-    a.Db = M->Drs;
+    a.Sb = M->Lrs;
     a.slot = M->rs_slot;
     a.SL = (int32_t)M->rs_SL;
     a.n = M->n;
@@ -581,34 +629,40 @@ int rs_launch_t(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t 
     a.yb = yb;
     a.zb = zb;
     a.tag = tag;
-    a.err = err;
+    a.ctl = ctl;
+    a.nclass = std::min(8, G);
     a.spin_limit = M->rs_spin;
     const void *fn = rs_kernel<S>(M->rs_R, M->rs_EPT);
     void *args[] = {&a};
-    CDG_HIP_TRY(hipLaunchKernel(fn, dim3(rs_grid(ctx, M)), dim3(M->rs_T), args, M->rs_lds, ctx->stream));
+    CDG_HIP_TRY(hipLaunchKernel(fn, dim3(G), dim3(M->rs_T), args, M->rs_lds, ctx->stream));
     return CDG_OK;
 }
 }  // namespace
 
 // One apply with the row-slice sweep on the given exchange buffers (words of
-// nrows*n tagged values each) and error word; `tag` must differ from every
-// tag the buffers hold (the caller's call counter).
+// nrows*n tagged values each) and control words (rs_ctl_words() uint32: the
+// error word and the ticket counters); `tag` must differ from every tag the
+// buffers hold (the caller's call counter), and the ticket set of parity
+// tag & 1 must be clear (zeroed memory, or cleared by the call with tag - 1).
 int launch_ilu_apply_rs_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb,
-                            int32_t *err, uint32_t tag) {
+                            uint32_t *ctl, uint32_t tag) {
     if (M->nrows == 0) return CDG_OK;
-    if (M->precision == CDG_FP64) return rs_launch_t<double>(ctx, M, v, z, yb, zb, err, tag);
-    if (M->precision == CDG_FP32) return rs_launch_t<float>(ctx, M, v, z, yb, zb, err, tag);
-    return rs_launch_t<cdg_bf16>(ctx, M, v, z, yb, zb, err, tag);
+    if (M->precision == CDG_FP64) return rs_launch_t<double>(ctx, M, v, z, yb, zb, ctl, tag);
+    if (M->precision == CDG_FP32) return rs_launch_t<float>(ctx, M, v, z, yb, zb, ctl, tag);
+    return rs_launch_t<cdg_bf16>(ctx, M, v, z, yb, zb, ctl, tag);
 }
+
+int rs_ctl_words() { return kRsCtlWords; }
 
 int launch_ilu_apply_rs(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z) {
     if (M->rs_epoch == 0xffffffffu) {   // tags wrap: clear the buffers, restart at 1
         const size_t words = (size_t)M->nrows * M->n * (M->precision == CDG_FP64 ? 2 : 1);
         CDG_HIP_TRY(hipMemsetAsync(M->rs_yb, 0, 8 * words, ctx->stream));
         CDG_HIP_TRY(hipMemsetAsync(M->rs_zb, 0, 8 * words, ctx->stream));
+        CDG_HIP_TRY(hipMemsetAsync(M->d_rs_ctl, 0, sizeof(uint32_t) * kRsCtlWords, ctx->stream));
         M->rs_epoch = 0;
     }
     M->rs_epoch += 1;
     return launch_ilu_apply_rs_buf(ctx, M, v, z, static_cast<uint64_t *>(M->rs_yb), static_cast<uint64_t *>(M->rs_zb),
-                                   M->d_rs_err, M->rs_epoch);
+                                   M->d_rs_ctl, M->rs_epoch);
 }
