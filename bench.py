@@ -13,8 +13,9 @@ Inputs are resident in HBM when timing starts.
 residual and solution in FP64); the plain FP64 form is measured beside it.
 The BF16-stored-ILU form is measured the same way.  The roofline object
 describes the ILU(0) apply (the metric's "ILU apply GB/s") in the schedule a
-long solve settles in (the stream schedule: one persistent launch over row
-slices of the factor), timed with HIP events on the library's own stream.  The CPU baseline
+long solve settles in (the row-slice schedule: one launch per wavefront level
+over row slices of the factor, W = Dinv U), timed with HIP events on the
+library's own stream.  The CPU baseline
 runs the C oracle (the reference algorithm restated) on the host cores, on the
 same system.
 """
@@ -42,10 +43,10 @@ SEED_Q = 1          # state perturbation
 SEED_B = 1          # right-hand side
 SEED_A = 20261202   # synthetic systems (P3 section, synthetic P2 section)
 HBM_PEAK_GBS = 8000.0
-# The FP32 ILU(0) apply in the stream schedule, one call = one launch of the
-# persistent row-slice sweep k_rs_sweep<float,...> (both triangular sweeps,
-# 40,960 tasks of (block row, 27-row slice)).
-APPLY_KERNEL = "ILU(0) apply FP32, stream schedule: k_rs_sweep<float,4,R,E> (one launch per call)"
+# The FP32 ILU(0) apply in the row-slice schedule: one call = 92 launches of
+# k_rs_level<float,...>, one per wavefront level (46 forward, 46 backward),
+# 40,960 tasks of (block row, 27-row slice) in all.
+APPLY_KERNEL = "ILU(0) apply FP32, row-slice schedule: k_rs_level<float,4,R,E> x 92 launches per call"
 # HBM bytes per apply call from rocprofv3 FETCH_SIZE + WRITE_SIZE on the Euler 16^3 P2
 # system, each scaled by what the same counter reads for a 512 MiB device copy; None
 # until measured for this kernel.
@@ -118,9 +119,9 @@ ITEMSIZE = {"fp64": 8, "fp32": 4, "bf16": 2}
 def measure(args, p, A, b, steps, warmup, kernels=True, precisions=("fp32", "fp64", "bf16")):
     """The FP32-ILU, FP64 and BF16-stored-ILU forms of one system: solves/s
     over ``steps`` timed solves (default schedule), and with ``kernels`` the
-    apply under the level, stream and folded schedules, the stream schedule's
-    preparation (W = Dinv U and the row-slice copies) and the matvec, timed on
-    the library stream."""
+    apply under the level, row-slice, stream and folded schedules, the
+    row-slice storage's preparation (W = Dinv U and the row-slice copies) and
+    the matvec, timed on the library stream."""
     import torch
     from chidg_amd.linsolve import chidg_mv_async, fgmres, precon_ILU0
     ctx = A.ctx
@@ -170,12 +171,15 @@ def measure(args, p, A, b, steps, warmup, kernels=True, precisions=("fp32", "fp6
             M.update(A)
             M.set_schedule("levels")
             t_lv = time_async(ctx, app, args.kreps)
-            M.set_schedule("stream")            # W = Dinv U and the row-slice copies at the first apply
+            M.set_schedule("rowslice")          # W = Dinv U and the row-slice copies at the first apply
             t_first = time_first(ctx, app)
+            t_rs = time_async(ctx, app, args.kreps)
+            M.set_schedule("stream")            # the same storage, one persistent launch
             t_st = time_async(ctx, app, args.kreps)
             r.update(apply_levels_ms=1e3 * t_lv, apply_levels_gbs=nbytes / t_lv / 1e9,
+                     apply_rowslice_ms=1e3 * t_rs, apply_rowslice_gbs=nbytes / t_rs / 1e9,
                      apply_stream_ms=1e3 * t_st, apply_stream_gbs=nbytes / t_st / 1e9,
-                     stream_prep_ms=1e3 * (t_first - t_st))
+                     rowslice_prep_ms=1e3 * (t_first - t_rs))
             if prec != "bf16" and args.folded:
                 M.update(A)
                 M.set_schedule("folded")
@@ -189,8 +193,9 @@ def measure(args, p, A, b, steps, warmup, kernels=True, precisions=("fp32", "fp6
         out[prec] = r
         log(f"{prec}: {r['solves_per_s']:.2f} solves/s, niter {st.niter}, res {r['residual_rel']:.2e}, "
             f"factor {r['t_factor_ms']:.2f} ms, precon {r['t_precon_ms']:.2f} ms, mv {r['t_mv_ms']:.2f} ms"
-            + (f"; apply levels {r['apply_levels_ms']:.3f} ms ({r['apply_levels_gbs']:.0f} GB/s), stream "
-               f"{r['apply_stream_ms']:.3f} ms ({r['apply_stream_gbs']:.0f} GB/s), prep {r['stream_prep_ms']:.2f} "
+            + (f"; apply levels {r['apply_levels_ms']:.3f} ms ({r['apply_levels_gbs']:.0f} GB/s), row-slice "
+               f"{r['apply_rowslice_ms']:.3f} ms ({r['apply_rowslice_gbs']:.0f} GB/s), stream "
+               f"{r['apply_stream_ms']:.3f} ms ({r['apply_stream_gbs']:.0f} GB/s), prep {r['rowslice_prep_ms']:.2f} "
                f"ms, mv {r['mv_ms']:.3f} ms ({r['mv_gbs']:.0f} GB/s)" if kernels else ""))
         M.tear_down()
     x64 = out["fp64"].pop("x")
@@ -409,7 +414,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kreps", type=int, default=20, help="kernel timing repetitions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--schedule", choices=["levels", "dataflow", "folded", "stream", "auto"], default=None,
+    ap.add_argument("--schedule", choices=["levels", "dataflow", "folded", "stream", "rowslice", "auto"], default=None,
                     help="ILU(0) apply schedule (default: the library's)")
     ap.add_argument("--extras", default="synthetic,p3,batch,serve",
                     help="comma list of extra sections: synthetic (seeded synthetic 16^3 P2 system), p3 "
@@ -422,8 +427,8 @@ def main():
     ap.add_argument("--batch-slots", type=int, default=6, help="request-loop slots (contexts) for the batch section")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the launch-shape autotuner (run before each timed section, outside the timed region)")
-    ap.add_argument("--folded", action="store_true",
-                    help="also time the folded level schedule (W = U Dinv) of the FP64 and FP32 forms")
+    ap.add_argument("--no-folded", dest="folded", action="store_false",
+                    help="skip timing the folded level schedule (W = U Dinv) of the FP64 and FP32 forms")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -462,8 +467,8 @@ def main():
         parity["gpu_fp64_niter"] = f64["niter"]
         log(f"parity: GPU FP64 {f64['niter']} it vs oracle {parity['oracle_niter']} it, "
             f"x rel {parity['gpu_fp64_x_rel_vs_oracle']:.2e}")
-    # the apply a long solve settles in: the stream schedule
-    t_ap = f32["apply_stream_ms"] / 1e3
+    # the apply a long solve settles in: the row-slice schedule
+    t_ap = f32["apply_rowslice_ms"] / 1e3
     gbs = ilu_apply_bytes(p, 4) / t_ap / 1e9
     bf = g.get("bf16")
     roof = {"bound": "hbm", "kernel": APPLY_KERNEL,
@@ -488,9 +493,9 @@ def main():
         "mv_shape": g.get("mv_shape"),
         "speedup_fp32_vs_fp64": round(f32["solves_per_s"] / f64["solves_per_s"], 3),
         "speedup_bf16_vs_fp64": round(bf["solves_per_s"] / f64["solves_per_s"], 3) if bf else None,
-        "ilu_apply_gbs_fp64": round(ilu_apply_bytes(p, 8) / (f64["apply_stream_ms"] / 1e3) / 1e9, 1),
+        "ilu_apply_gbs_fp64": round(ilu_apply_bytes(p, 8) / (f64["apply_rowslice_ms"] / 1e3) / 1e9, 1),
         "ilu_apply_gbs_fp32": round(gbs, 1),
-        "ilu_apply_gbs_bf16": round(ilu_apply_bytes(p, 2) / (bf["apply_stream_ms"] / 1e3) / 1e9, 1) if bf else None,
+        "ilu_apply_gbs_bf16": round(ilu_apply_bytes(p, 2) / (bf["apply_rowslice_ms"] / 1e3) / 1e9, 1) if bf else None,
         "latency_ms": {"p50": round(f32["p50_ms"], 3), "p99": round(f32["p99_ms"], 3)},
         "parity_vs_oracle": parity,
         "roofline": roof,

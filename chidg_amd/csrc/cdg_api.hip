@@ -123,10 +123,6 @@ int ctx_ensure_small(cdg_ctx *ctx, size_t n) {
     return CDG_OK;
 }
 
-namespace {
-
-std::atomic<uint64_t> g_stamp{0};
-
 bool is_device_ptr(const void *p) {
     if (!p) return false;
     hipPointerAttribute_t attr;
@@ -137,6 +133,10 @@ bool is_device_ptr(const void *p) {
     }
     return attr.type == hipMemoryTypeDevice;
 }
+
+namespace {
+
+std::atomic<uint64_t> g_stamp{0};
 
 // Device view of a vector argument: device pointers pass through, host
 // pointers go through a per-slot staging buffer.
@@ -789,7 +789,7 @@ extern "C" int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int
 extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule) {
     if (!ctx || !M) return CDG_ERR_ARG;
     if (schedule != CDG_SCHEDULE_LEVELS && schedule != CDG_SCHEDULE_DATAFLOW && schedule != CDG_SCHEDULE_FOLDED &&
-        schedule != CDG_SCHEDULE_AUTO && schedule != CDG_SCHEDULE_STREAM)
+        schedule != CDG_SCHEDULE_AUTO && schedule != CDG_SCHEDULE_STREAM && schedule != CDG_SCHEDULE_ROWSLICE)
         return cdg_fail(ctx, CDG_ERR_ARG, "unknown apply schedule %d", schedule);
     if (M->type != CDG_PREC_ILU0) return CDG_OK;
     if (schedule == CDG_SCHEDULE_DATAFLOW) {
@@ -797,7 +797,7 @@ extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule
         if (e) return e;
         CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    if (schedule == CDG_SCHEDULE_STREAM) {   // storage now; W and the copies at the next apply
+    if (schedule == CDG_SCHEDULE_STREAM || schedule == CDG_SCHEDULE_ROWSLICE) {   // storage now; W and the copies at the next apply
         int e = rs_setup(ctx, M);
         if (e) return e;
         CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -815,6 +815,7 @@ extern "C" int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule
     M->apply_mode = schedule == CDG_SCHEDULE_DATAFLOW ? CDG_APPLY_DATAFLOW
                     : schedule == CDG_SCHEDULE_FOLDED ? CDG_APPLY_FOLDED
                     : schedule == CDG_SCHEDULE_STREAM ? CDG_APPLY_STREAM
+                    : schedule == CDG_SCHEDULE_ROWSLICE ? CDG_APPLY_ROWSLICE
                     : schedule == CDG_SCHEDULE_AUTO ? CDG_APPLY_AUTO : CDG_APPLY_LEVELS;
     return CDG_OK;
 }
@@ -1046,13 +1047,16 @@ static int prec_apply_dev(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const dou
     ++M->napply;
     const bool auto_long = M->apply_mode == CDG_APPLY_AUTO && M->fold_after > 0 &&
                            (M->napply > M->fold_after || M->prev_napply > M->fold_after) && M->rs_ok;
-    if (M->apply_mode == CDG_APPLY_STREAM || auto_long) {
+    if (M->apply_mode == CDG_APPLY_ROWSLICE || M->apply_mode == CDG_APPLY_STREAM || auto_long) {
         int e = rs_prepare(ctx, M, A);
         if (e && M->apply_mode == CDG_APPLY_AUTO && e == CDG_ERR_UNSUPPORTED) {
             M->rs_ok = 0;   // this pattern or block size: the level schedule from now on
         } else {
             if (e) return e;
-            return launch_ilu_apply_rs(ctx, M, v, z);
+            // AUTO settles in the row-slice level launches (measured faster
+            // than the persistent sweep on the same storage)
+            return M->apply_mode == CDG_APPLY_STREAM ? launch_ilu_apply_rs(ctx, M, v, z)
+                                                     : launch_ilu_apply_rsl(ctx, M, v, z);
         }
     }
     const bool fold_now = !M->w_ready && M->apply_mode == CDG_APPLY_FOLDED;   // schedule chosen after the factor

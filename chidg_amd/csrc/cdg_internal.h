@@ -155,12 +155,14 @@ struct cdg_prec {
     int32_t *rs_tasks = nullptr;       // device task records
     int32_t *d_up_row = nullptr;       // row of each upper entry
     void *rs_yb = nullptr, *rs_zb = nullptr;
+    void *rs_y = nullptr;              // forward result of the row-slice level schedule
     uint32_t *d_rs_ctl = nullptr;      // error word + ticket counters of the sweep
     uint32_t rs_epoch = 0;             // tag of the last call
     int32_t rs_ready = 0;              // Lrs/Wrs/Drs hold the current factor
     int32_t rs_ok = 1;                 // 0: the pattern does not fit the stream schedule (AUTO keeps levels)
     int32_t rs_w = 0, rs_S = 0, rs_T = 0, rs_dc = 0, rs_R = 0, rs_EPT = 0, rs_ntasks = 0, rs_per_cu = 0;
     int32_t rs_spin = 1 << 20;
+    int32_t rs_classes = 64;           // ticket classes of the sweep
     int64_t rs_SL = 0, rs_slot = 0;
     size_t rs_lds = 0;
     uint64_t stamp = 0;
@@ -174,7 +176,8 @@ int launch_ilu_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *
 int launch_ilu_apply_df(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 int df_configure(cdg_ctx *ctx, cdg_prec *M);
 int df_reset(cdg_ctx *ctx, cdg_prec *M);
-enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2, CDG_APPLY_AUTO = 3, CDG_APPLY_STREAM = 4 };
+enum { CDG_APPLY_LEVELS = 0, CDG_APPLY_DATAFLOW = 1, CDG_APPLY_FOLDED = 2, CDG_APPLY_AUTO = 3, CDG_APPLY_STREAM = 4,
+       CDG_APPLY_ROWSLICE = 5 };
 // stream schedule (k_rsapply.hip; the fold in k_factor.hip)
 int rs_setup(cdg_ctx *ctx, cdg_prec *M);
 int rs_relayout(cdg_ctx *ctx, cdg_prec *M);
@@ -183,6 +186,9 @@ int launch_ilu_apply_rs(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
 int launch_ilu_apply_rs_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t *yb, uint64_t *zb,
                             uint32_t *ctl, uint32_t tag);
 int rs_ctl_words();
+int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
+int launch_ilu_apply_rsl_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y);
+bool is_device_ptr(const void *p);
 int rs_slice_rows(int n, int wmax);
 void rs_ctx_count(int delta);
 // W_u = A_{up_blk[u]} Dinv_{up_col[u]} for all upper blocks (after a factor)

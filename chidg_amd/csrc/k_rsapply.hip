@@ -120,6 +120,27 @@ template <> struct Tagged<double> {
     }
 };
 
+// Row sums of a task: red[r + w q] (q < dc) holds the dc partial products of
+// row r; K = min(T / w, dc) groups of w threads first sum every K-th partial,
+// then thread r < w sums the K group sums (fixed order: the result does not
+// depend on timing).  Returns row tid's sum for tid < w.  Ends with the
+// partials consumed (one barrier inside).
+template <typename Acc>
+__device__ __forceinline__ Acc rs_rowsum(const Acc *red, Acc *red2, int w, int dc, int T, int tid) {
+    const int K = max(1, min(T / w, dc));
+    if (tid < K * w) {
+        const int g = tid / w, r = tid - g * w;
+        Acc s = 0;
+        for (int q = g; q < dc; q += K) s += red[r + w * q];
+        red2[g * w + r] = s;
+    }
+    __syncthreads();
+    Acc s = 0;
+    if (tid < w)
+        for (int g = 0; g < K; ++g) s += red2[g * w + tid];
+    return s;
+}
+
 struct RsArgs {
     const int32_t *tasks;     // kRsRec ints per task (see rs_build_tasks)
     int32_t ntasks;
@@ -150,12 +171,14 @@ constexpr int kRsMaxThreads = 512;
 namespace {
 
 // Control words of a sweep (uint32): [0] error word; the ticket counter of
-// set s (0, 1) and class c at kRsLine * (1 + 8 s + c), one 128-byte line each.
+// set s (0, 1) and class c < kRsMaxClass at kRsLine * (1 + kRsMaxClass s + c),
+// one 128-byte line each.
 constexpr int kRsLine = 32;
-constexpr int kRsCtlWords = kRsLine * 17;
+constexpr int kRsMaxClass = 256;
+constexpr int kRsCtlWords = kRsLine * (1 + 2 * kRsMaxClass);
 
 __device__ __forceinline__ uint32_t *rs_counter(uint32_t *ctl, uint32_t set, int cls) {
-    return ctl + kRsLine * (1 + 8 * (set & 1u) + cls);
+    return ctl + kRsLine * (1 + kRsMaxClass * (set & 1u) + cls);
 }
 
 template <typename S, int NB, int R, int EPT>
@@ -186,19 +209,25 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs args) {
     const int T = blockDim.x, tid = threadIdx.x;
     Acc *xs = reinterpret_cast<Acc *>(smem_raw);                    // kRsMaxBlocks * n
     Acc *red = xs + (((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3);  // V*T
+    Acc *red2 = red + (size_t)V * blockDim.x;                       // T
     const int lim = w * n;             // elements of one block slice
     const int step = V * T;            // = dc * w
     const int lim_ld = (lim - 1) / V * V;   // last 16-byte group starting inside the slice
     int col0[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) col0[v] = (V * tid + v) / w;
+    // w >= V: a 16-byte group spans at most two columns, `split` values in the first
+    const bool two = w >= V;
+    const int split = min(V, w * (col0[0] + 1) - V * tid);
 
     const int cls = blockIdx.x % NC;
     uint32_t *const cnt = rs_counter(ctl, tag, cls);
-    // workgroup 0 clears the counters of the next call (the previous call,
-    // which used them, has ended: calls on one buffer set are stream-ordered)
-    if (blockIdx.x == 0 && tid < 8)
-        __hip_atomic_store(rs_counter(ctl, tag + 1u, tid), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // workgroup 0 clears the counters of the next call, all of them (the next
+    // call may have more classes); the previous call, which used them, has
+    // ended: calls on one buffer set are stream-ordered
+    if (blockIdx.x == 0)
+        for (int c = tid; c < kRsMaxClass; c += T)
+            __hip_atomic_store(rs_counter(ctl, tag + 1u, c), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // ticket k of the class is task k NC + cls
     auto ticket = [cnt, NC, cls]() -> int {
         const uint32_t k = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -318,9 +347,16 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs args) {
                     const int e = V * tid + m * step;
                     Acc x[V];
                     RW::get(raw[k][m], x);
+                    if (two) {   // the group's values in two columns: two LDS reads
+                        const Acc xa = xk[min(col0[0] + m * dc, n - 1)], xb = xk[min(col0[0] + 1 + m * dc, n - 1)];
 #pragma unroll
-                    for (int v = 0; v < V; ++v)
-                        if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
+                        for (int v = 0; v < V; ++v)
+                            if (e + v < lim) acc[v] += x[v] * (v < split ? xa : xb);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < V; ++v)
+                            if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
+                    }
                 }
             }
         }
@@ -348,10 +384,9 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs args) {
         for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
         if (tid == 0) s_tick[0] = tnn;
         __syncthreads();
-        for (int r = tid; r < w; r += T) {
-            Acc s = 0;
-            for (int q = 0; q < dc; ++q) s += red[r + w * q];
-            const size_t gi = (size_t)row * n + (size_t)sl * w + r;
+        const Acc s = rs_rowsum(red, red2, w, dc, T, tid);
+        if (tid < w) {
+            const size_t gi = (size_t)row * n + (size_t)sl * w + tid;
             if (aborted) {
                 if (bwd) zout[gi] = __builtin_nan("");
             } else if (!bwd) {
@@ -368,6 +403,147 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_sweep(RsArgs args) {
         t = tn;
         tn = tnn;
         if (t >= ntasks) break;
+    }
+}
+
+// ---- the same tasks, one launch per wavefront level ------------------------------
+// Level l of the forward sweep is tasks [lo, hi) of the task list, and so is
+// each backward level: every task of a launch has its parents in earlier
+// launches (stream order makes them visible), so a task loads its parents
+// with plain loads, streams its block slices and writes final values of its
+// rows -- no tags, no waits, no partial sums.  y (the forward result) is kept
+// in the storage's accumulation precision, as the persistent sweep keeps it.
+struct RslArgs {
+    const int32_t *tasks;
+    int32_t t0;               // first task of the level
+    const void *Sb;
+    int64_t slot;
+    int32_t SL, n, w, dc;
+    const double *v;
+    double *z;
+    void *y;                  // nrows * n values of the accumulation type
+};
+
+template <typename S, int NB, int R, int EPT>
+__global__ void __launch_bounds__(kRsMaxThreads) k_rs_level(RslArgs args) {
+    typedef Raw16<S> RW;
+    typedef typename RW::Acc Acc;
+    typedef typename RW::T RawT;
+    typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type VT;
+    constexpr int V = RW::V;
+    const S *const Sb = static_cast<const S *>(args.Sb);
+    const int64_t slot = args.slot;
+    const int SL = args.SL, n = args.n, w = args.w, dc = args.dc;
+    VT *const y = static_cast<VT *>(args.y);
+    double *const zout = args.z;
+    extern __shared__ unsigned char smem_raw[];
+    const int T = blockDim.x, tid = threadIdx.x;
+    Acc *xs = reinterpret_cast<Acc *>(smem_raw);
+    Acc *red = xs + (((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3);
+    Acc *red2 = red + (size_t)V * blockDim.x;
+    const int lim = w * n, step = V * T, lim_ld = (lim - 1) / V * V;
+    const int32_t *rec = args.tasks + (size_t)(args.t0 + (int)blockIdx.x) * kRsRec;
+    const int row = rec[0];
+    const int sl = rec[1] & 0xff, nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1;
+    // parents first (one batch of loads, clamped in bounds, no branch), then
+    // the block slices; the parents' wait then leaves the block loads in flight
+    const int ne = nb * n;
+    int par[kRsMaxBlocks];
+#pragma unroll
+    for (int k = 0; k < kRsMaxBlocks; ++k) par[k] = rec[2 + kRsMaxBlocks + k];
+    VT pv[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        const int j = max(0, min(tid + i * T, ne - 1));
+        const int k = j / n, c = j - k * n;
+        int pk = par[0];
+#pragma unroll
+        for (int q = 1; q < kRsMaxBlocks; ++q) pk = k == q ? par[q] : pk;
+        const size_t gi = (size_t)pk * n + c;
+        // forward parents and the backward row's own value: y; backward
+        // upper parents: z (stored as double, holding VT values)
+        pv[i] = (bwd && j >= n) ? static_cast<VT>(zout[gi]) : y[gi];
+    }
+    RawT raw[NB][R];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int kk = k < nb ? k : 0;
+        const S *b = Sb + (size_t)rec[2 + kk] * slot + (size_t)sl * SL;
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+            raw[k][m] = __builtin_nontemporal_load(
+                reinterpret_cast<const RawT *>(b + (k < nb ? min(V * tid + m * step, lim_ld) : 0)));
+    }
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        const int j = tid + i * T;
+        if (j < ne) xs[j] = (bwd && j < n) ? static_cast<Acc>(pv[i]) : -static_cast<Acc>(pv[i]);
+    }
+    __syncthreads();
+    int col0[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) col0[v] = (V * tid + v) / w;
+    // w >= V: a 16-byte group spans at most two columns, `split` values in the first
+    const bool two = w >= V;
+    const int split = min(V, w * (col0[0] + 1) - V * tid);
+    Acc acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        if (k < nb) {
+            const Acc *xk = xs + k * n;
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                const int e = V * tid + m * step;
+                Acc x[V];
+                RW::get(raw[k][m], x);
+                if (two) {   // the group's values in two columns: two LDS reads
+                    const Acc xa = xk[min(col0[0] + m * dc, n - 1)], xb = xk[min(col0[0] + 1 + m * dc, n - 1)];
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (e + v < lim) acc[v] += x[v] * (v < split ? xa : xb);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
+                }
+            }
+        }
+    }
+    if (R * step < lim || nb > NB) {
+        for (int k = 0; k < nb; ++k) {
+            const S *b = Sb + (size_t)rec[2 + k] * slot + (size_t)sl * SL;
+            const Acc *xk = xs + k * n;
+            for (int m = k < NB ? R : 0; m * step < lim; ++m) {
+                const int e = V * tid + m * step;
+                if (e < lim) {
+                    Acc x[V];
+                    RW::get(__builtin_nontemporal_load(reinterpret_cast<const RawT *>(b + e)), x);
+                    if (two) {   // the group's values in two columns: two LDS reads
+                        const Acc xa = xk[min(col0[0] + m * dc, n - 1)], xb = xk[min(col0[0] + 1 + m * dc, n - 1)];
+#pragma unroll
+                        for (int v = 0; v < V; ++v)
+                            if (e + v < lim) acc[v] += x[v] * (v < split ? xa : xb);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < V; ++v)
+                            if (e + v < lim) acc[v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) red[V * tid + v] = acc[v];
+    __syncthreads();
+    const Acc s = rs_rowsum(red, red2, w, dc, T, tid);
+    if (tid < w) {
+        const size_t gi = (size_t)row * n + (size_t)sl * w + tid;
+        if (!bwd)
+            y[gi] = static_cast<VT>(args.v[gi] + static_cast<double>(s));
+        else
+            zout[gi] = static_cast<double>(static_cast<VT>(s));
     }
 }
 
@@ -429,6 +605,22 @@ const void *rs_kernel(int R, int EPT) {
 #undef CDG_RS_K
 }
 
+template <typename S>
+const void *rsl_kernel(int R, int EPT) {
+#define CDG_RS_K(RR, EE) reinterpret_cast<const void *>(&k_rs_level<S, 4, RR, EE>)
+#define CDG_RS_E(RR)                                  \
+    if (EPT <= 2) return CDG_RS_K(RR, 2);             \
+    if (EPT <= 4) return CDG_RS_K(RR, 4);             \
+    return CDG_RS_K(RR, 8);
+    if (R <= 2) { CDG_RS_E(2) }
+    if (R <= 3) { CDG_RS_E(3) }
+    if (R <= 4) { CDG_RS_E(4) }
+    if (R <= 6) { CDG_RS_E(6) }
+    CDG_RS_E(8)
+#undef CDG_RS_E
+#undef CDG_RS_K
+}
+
 int round_pow(int x) { return x <= 2 ? 2 : (x <= 4 ? 4 : 8); }
 int round_r(int x) { return x <= 2 ? 2 : x <= 3 ? 3 : x <= 4 ? 4 : x <= 6 ? 6 : 8; }
 
@@ -452,9 +644,13 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
     const int V = prec == CDG_FP64 ? 2 : (prec == CDG_FP32 ? 4 : 8);
     const size_t es = prec == CDG_FP64 ? 8 : (prec == CDG_FP32 ? 4 : 2);
     const size_t accsz = prec == CDG_FP64 ? 8 : 4;
-    int wmax = prec == CDG_FP64 ? 16 : 32;
+    // rows per slice and threads per task: measured best on the 16^3 P2 Euler
+    // system (FP64 15 x ~256, FP32 27 x ~512, BF16 45 x ~512: the larger the
+    // slice, the fewer tasks share each parent vector's loads; FP64 slices
+    // hold twice the bytes per row)
+    int wmax = prec == CDG_FP64 ? 16 : (prec == CDG_FP32 ? 27 : 45);
     if (const char *e = std::getenv("CDG_RS_W")) wmax = std::max(1, std::atoi(e));
-    int target = 256;
+    int target = prec == CDG_FP64 ? 256 : 512;
     if (const char *e = std::getenv("CDG_RS_THREADS")) target = std::max(64, std::atoi(e));
     const int w = rs_slice_rows(n, wmax);
     // a task gathers at most (maxdep + 1) parent vectors, at most 8 values per thread
@@ -473,7 +669,8 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
     const int EPT = round_pow((ne_max + T - 1) / T);
     const void *fn = prec == CDG_FP64 ? rs_kernel<double>(R, EPT)
                    : (prec == CDG_FP32 ? rs_kernel<float>(R, EPT) : rs_kernel<cdg_bf16>(R, EPT));
-    const size_t lds = accsz * ((((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3) + (size_t)V * T);
+    if (w > T) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream schedule: %d-row slices need more than %d threads", w, T);
+    const size_t lds = accsz * ((((size_t)kRsMaxBlocks * n + 3) & ~(size_t)3) + (size_t)V * T + (size_t)T);
     int per_cu = 0;
     CDG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, T, lds));
     if (per_cu < 1) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "stream sweep does not fit a CU");
@@ -527,6 +724,7 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
         M->Drs = static_cast<char *>(M->Wrs) + es * (size_t)M->nup * slot;
     }
     if (!e) e = alloc(&M->rs_yb, 8 * vec_words);
+    if (!e) e = alloc(&M->rs_y, (prec == CDG_FP64 ? 8 : 4) * (size_t)M->nrows * n);
     if (!e) e = alloc(&M->rs_zb, 8 * vec_words);
     if (!e) e = alloc((void **)&M->d_rs_ctl, sizeof(uint32_t) * kRsCtlWords);
     if (!e) {
@@ -571,6 +769,11 @@ int rs_setup(cdg_ctx *ctx, cdg_prec *M) {
     M->rs_per_cu = per_cu;
     M->rs_ready = 0;
     M->rs_spin = 1 << 20;
+    // ticket classes: each class's counter serves G / classes workgroups; the
+    // first `classes` workgroups (one per class) must be resident together
+    M->rs_classes = 64;
+    if (const char *ev = std::getenv("CDG_RS_CLASSES")) M->rs_classes = std::atoi(ev);
+    M->rs_classes = std::max(1, std::min(M->rs_classes, kRsMaxClass));
     if (const char *ev = std::getenv("CDG_RS_SPIN")) M->rs_spin = std::max(1, std::atoi(ev));
     return CDG_OK;
 }
@@ -630,7 +833,7 @@ int rs_launch_t(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, uint64_t 
     a.zb = zb;
     a.tag = tag;
     a.ctl = ctl;
-    a.nclass = std::min(8, G);
+    a.nclass = std::max(1, std::min(M->rs_classes, G));
     a.spin_limit = M->rs_spin;
     const void *fn = rs_kernel<S>(M->rs_R, M->rs_EPT);
     void *args[] = {&a};
@@ -665,4 +868,47 @@ int launch_ilu_apply_rs(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z) {
     M->rs_epoch += 1;
     return launch_ilu_apply_rs_buf(ctx, M, v, z, static_cast<uint64_t *>(M->rs_yb), static_cast<uint64_t *>(M->rs_zb),
                                    M->d_rs_ctl, M->rs_epoch);
+}
+
+// One apply of the row-slice tasks, one launch per wavefront level (forward
+// levels, then backward levels), with the forward vector in y (nrows * n
+// values of the accumulation type).
+int launch_ilu_apply_rsl_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y) {
+    if (M->nrows == 0) return CDG_OK;
+    RslArgs a{};
+    a.tasks = M->rs_tasks;
+    a.Sb = M->Lrs;
+    a.slot = M->rs_slot;
+    a.SL = (int32_t)M->rs_SL;
+    a.n = M->n;
+    a.w = M->rs_w;
+    a.dc = M->rs_dc;
+    a.v = v;
+    a.z = z;
+    a.y = y;
+    const void *fn = M->precision == CDG_FP64 ? rsl_kernel<double>(M->rs_R, M->rs_EPT)
+                   : (M->precision == CDG_FP32 ? rsl_kernel<float>(M->rs_R, M->rs_EPT)
+                                               : rsl_kernel<cdg_bf16>(M->rs_R, M->rs_EPT));
+    const int S = M->rs_S;
+    auto run = [&](int lo, int hi) -> int {
+        if (hi <= lo) return CDG_OK;
+        a.t0 = lo;
+        void *args[] = {&a};
+        CDG_HIP_TRY(hipLaunchKernel(fn, dim3(hi - lo), dim3(M->rs_T), args, M->rs_lds, ctx->stream));
+        return CDG_OK;
+    };
+    for (int l = 0; l < M->nlev; ++l) {
+        int e = run(S * M->lev_row_ptr[l], S * M->lev_row_ptr[l + 1]);
+        if (e) return e;
+    }
+    const int nf = S * M->nrows;
+    for (int l = 0; l < M->nblev; ++l) {
+        int e = run(nf + S * M->blev_row_ptr[l], nf + S * M->blev_row_ptr[l + 1]);
+        if (e) return e;
+    }
+    return CDG_OK;
+}
+
+int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z) {
+    return launch_ilu_apply_rsl_buf(ctx, M, v, z, M->rs_y);
 }

@@ -268,10 +268,12 @@ class _Preconditioner:
         partial sums), 'folded' (levels, with Dinv folded into the upper blocks
         at the first apply after each factor; FP64/FP32), 'stream' (one
         persistent launch over row slices of the factor, W = Dinv U formed at
-        the first apply after each factor) or 'auto' (the default: 'levels'
-        for the first 12 applies after each factor, 'stream' from then on, or
-        from the first apply when the previous factor served more than 12)."""
-        code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3, "stream": 4}[schedule]
+        the first apply after each factor), 'rowslice' (the same row-slice
+        tasks, one launch per wavefront level) or 'auto' (the default:
+        'levels' for the first 12 applies after each factor, 'rowslice' from
+        then on, or from the first apply when the previous factor served more
+        than 12)."""
+        code = {"levels": 0, "dataflow": 1, "folded": 2, "auto": 3, "stream": 4, "rowslice": 5}[schedule]
         self.ctx._check(self.ctx.lib.cdg_prec_set_schedule(self.ctx.h, self.h, code), "cdg_prec_set_schedule")
         self.schedule = schedule
 

@@ -182,18 +182,23 @@ int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
  *                          nup + nrows more block slots), so every task writes
  *                          final values and the backward sweep needs no
  *                          Dinv stage.  All precisions.
+ *   CDG_SCHEDULE_ROWSLICE  the tasks of CDG_SCHEDULE_STREAM on the same
+ *                          storage, one launch per wavefront level: every task
+ *                          reads final parent values written by earlier
+ *                          launches (no waits inside a launch).  All precisions.
  *   CDG_SCHEDULE_AUTO      (the default) CDG_SCHEDULE_LEVELS for the first 12
- *                          applies after each factor, CDG_SCHEDULE_STREAM from
- *                          the 13th on, or from the first when the previous
- *                          factor served more than 12 (environment
+ *                          applies after each factor, CDG_SCHEDULE_ROWSLICE
+ *                          from the 13th on, or from the first when the
+ *                          previous factor served more than 12 (environment
  *                          CDG_FOLD_AFTER=k sets k; 0 never switches), so long
- *                          solves get the streaming sweep and short ones skip
- *                          its preparation.  */
+ *                          solves get the row-slice storage and short ones
+ *                          skip its preparation.  */
 #define CDG_SCHEDULE_LEVELS   0
 #define CDG_SCHEDULE_DATAFLOW 1
 #define CDG_SCHEDULE_FOLDED   2
 #define CDG_SCHEDULE_AUTO     3
 #define CDG_SCHEDULE_STREAM   4
+#define CDG_SCHEDULE_ROWSLICE 5
 int cdg_prec_set_schedule(cdg_ctx *ctx, cdg_prec *M, int32_t schedule);
 /* Launch-shape autotuner (no reference counterpart).  cdg_prec_tune times the
  * apply of M's current factor (level schedules; after cdg_prec_update) under

@@ -165,7 +165,8 @@ def test_folded_schedule_set_after_factor(ctx):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6), ("bf16", 1e-2)])
 def test_auto_schedule_streams_after_12_applies(ctx, precision, tol):
     """The default schedule: the first 12 applies after a factor are the level
-    schedule's bit for bit, the 13th on the stream schedule's; after a factor
+    schedule's bit for bit, the 13th on the row-slice schedule's (the stream
+    sweep's bits: the same tasks and sums); after a factor
     that served more than 12 applies the next switches at its first apply,
     after one that served fewer it waits for 12 again; choosing 'levels' gives
     the level sweep."""
@@ -257,6 +258,28 @@ def test_stream_matches_oracle_and_levels(ctx, dims, degree, precision, tol):
             assert np.array_equal(Ms.apply(A, v), zs)
 
 
+@pytest.mark.parametrize("dims,degree,precision,tol", STREAM_CASES)
+def test_rowslice_matches_oracle_and_stream(ctx, dims, degree, precision, tol):
+    """The row-slice tasks one launch per level: the oracle's apply to the
+    stated tolerance, and bit for bit the persistent sweep's result (same
+    products, same sums, same roundings), on every repeat."""
+    p, blocks = synthetic_system(*dims, degree, seed=17, periodic=(True, dims[1] > 1, dims[2] > 1))
+    A = upload(ctx, p, blocks)
+    Mr = precon_ILU0(precision, schedule="rowslice").init(A)
+    Ms = precon_ILU0(precision, schedule="stream").init(A)
+    Mr.update(A)
+    Ms.update(A)
+    L_ref, D_ref = po.ilu0_factor(p, blocks)
+    for seed in (3, 4):
+        v = po.fill_vector(p.vec_len, seed)
+        z_ref = po.ilu0_apply(p, blocks, L_ref, D_ref, v)
+        zr = Mr.apply(A, v)
+        assert rel(zr, z_ref) < tol
+        assert np.array_equal(zr, Ms.apply(A, v))
+        for _ in range(2):
+            assert np.array_equal(Mr.apply(A, v), zr)
+
+
 def test_stream_refactor_and_schedule_switch(ctx):
     """A new factor re-forms W and the row-slice copies at its first stream
     apply; switching schedules back and forth keeps the results."""
@@ -277,7 +300,13 @@ def test_stream_refactor_and_schedule_switch(ctx):
     M.set_schedule("levels")
     z_lv = M.apply(A, v)
     M.set_schedule("stream")
-    assert rel(M.apply(A, v), z_lv) < 1e-13
+    z_st = M.apply(A, v)
+    assert rel(z_st, z_lv) < 1e-13
+    M.set_schedule("rowslice")
+    assert np.array_equal(M.apply(A, v), z_st)
+    A.store(blocks)                  # a new factor while the row-slice schedule is set
+    M.update(A)
+    assert np.array_equal(M.apply(A, v), z1)
 
 
 def test_stream_fgmres_full_size_all_precisions(ctx):
@@ -301,6 +330,8 @@ def test_stream_fgmres_full_size_all_precisions(ctx):
         assert float(torch.linalg.norm(zs - zl) / torch.linalg.norm(zl)) < tol, precision
         assert torch.equal(Ms.apply(A, v), zs)
         Ml.tear_down()
+        Ms.set_schedule("rowslice")
+        assert torch.equal(Ms.apply(A, v), zs)
         x = torch.zeros_like(v)
         ls = fgmres(nkrylov=30, tol=1e-30, rtol=1e-8, inner_fgmres=False)
         ls.solve(A, x, v, Ms)
