@@ -24,7 +24,8 @@ EXPORTS = [
     "cdg_ctx_wait_stream", "cdg_ctx_signal_stream",
     "cdg_mat_create", "cdg_mat_destroy", "cdg_mat_upload", "cdg_mat_download",
     "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_set_partition", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
-    "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_destroy", "cdg_prec_download",
+    "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_apply_noalloc",
+    "cdg_prec_destroy", "cdg_prec_download",
     "cdg_prec_set_schedule", "cdg_prec_tune", "cdg_mat_tune", "cdg_prec_launch_shape", "cdg_mat_launch_shape",
     "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
     "cdg_server_last_error", "cdg_server_destroy", "cdg_server_register", "cdg_server_submit_system",
@@ -99,6 +100,7 @@ def load() -> C.CDLL:
         "cdg_ctx_signal_stream": ([vp, vp], C.c_int),
         "cdg_mv_async": ([vp, vp, dp, dp], C.c_int),
         "cdg_prec_apply_async": ([vp, vp, vp, dp, dp], C.c_int),
+        "cdg_prec_apply_noalloc": ([vp, vp, vp, dp, dp], C.c_int),
         "cdg_mat_create": ([vp, C.c_int32, C.c_int32, ip, ip, ip, pp], C.c_int),
         "cdg_mat_destroy": ([vp, vp], C.c_int),
         "cdg_mat_upload": ([vp, vp, dp], C.c_int),

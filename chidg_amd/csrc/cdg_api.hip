@@ -371,6 +371,7 @@ extern "C" int cdg_ctx_create(int32_t device, cdg_ctx **out) {
     }
     if (const char *sp = std::getenv("CDG_SPIN")) ctx->spin = std::atoi(sp) != 0;
     if (ctx_ensure_small(ctx, 4096) || ctx_ensure_part(ctx, 1 << 16)) { cdg_ctx_destroy(ctx); return CDG_ERR_OOM; }
+    if (noalloc_init(device) != CDG_OK) { cdg_ctx_destroy(ctx); return CDG_ERR_HIP; }   // scratch array address
     rs_ctx_count(+1);
     ctx->counted = true;
     *out = ctx;

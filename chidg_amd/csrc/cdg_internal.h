@@ -188,6 +188,7 @@ int launch_ilu_apply_rs_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *
 int rs_ctl_words();
 int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
 int launch_ilu_apply_rsl_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y);
+int noalloc_init(int device);
 bool is_device_ptr(const void *p);
 int rs_slice_rows(int n, int wmax);
 void rs_ctx_count(int delta);

@@ -215,6 +215,21 @@ int cdg_mat_tune(cdg_ctx *ctx, cdg_mat *A, int32_t reps, double *best_ms);
  * flight} for the apply; out[2] = {threads, loads in flight} for the matvec. */
 int cdg_prec_launch_shape(const cdg_prec *M, int32_t *out);
 int cdg_mat_launch_shape(const cdg_mat *A, int32_t *out);
+/* Allocation-free apply.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  z = M^-1 v exactly as
+ * cdg_prec_apply_async computes it under CDG_SCHEDULE_ROWSLICE (same kernels,
+ * same bits), enqueued on the context's stream; v and z are device pointers.
+ * M must be an updated ILU(0) preconditioner whose row-slice storage exists
+ * (cdg_prec_set_schedule with CDG_SCHEDULE_ROWSLICE or CDG_SCHEDULE_STREAM
+ * allocates it; CDG_ERR_STATE otherwise).  The forward sweep's vector
+ * (nrows*n*8 bytes for FP64, nrows*n*4 for FP32/BF16) lives in a library
+ * array of CDG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+ * larger systems get CDG_ERR_ARG.  All calls on one GPU share that array:
+ * calls that overlap on one GPU (streams of several contexts) race.  The call
+ * performs no hipMalloc/hipFree. */
+#define CDG_NOALLOC_SCRATCH_BYTES (32 << 20)
+int cdg_prec_apply_noalloc(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 /* Factor as FP64 for inspection: L (nnzb*n*n, lower kind-0 slots filled,
  * others zero) and Dinv (nrows*n*n), the LD matrix of precon_ILU0_t. */
 int cdg_prec_download(cdg_ctx *ctx, const cdg_prec *M, double *L, double *Dinv);
