@@ -23,14 +23,14 @@ EXPORTS = [
     "cdg_ctx_create", "cdg_ctx_destroy", "cdg_last_error", "cdg_ctx_mem_used", "cdg_ctx_stream",
     "cdg_ctx_wait_stream", "cdg_ctx_signal_stream",
     "cdg_mat_create", "cdg_mat_destroy", "cdg_mat_upload", "cdg_mat_download",
-    "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_set_partition", "cdg_mat_info", "cdg_mv", "cdg_mv_async",
-    "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_apply_noalloc",
+    "cdg_mat_fill_synthetic", "cdg_vec_fill_synthetic", "cdg_mat_stamp", "cdg_mat_set_partition", "cdg_mat_info", "cdg_mv", "cdg_mv_async", "cdg_mv_multi",
+    "cdg_prec_create", "cdg_prec_update", "cdg_prec_apply", "cdg_prec_apply_async", "cdg_prec_apply_noalloc", "cdg_prec_apply_multi",
     "cdg_prec_destroy", "cdg_prec_download",
     "cdg_prec_set_schedule", "cdg_prec_tune", "cdg_mat_tune", "cdg_prec_launch_shape", "cdg_mat_launch_shape",
     "cdg_server_create", "cdg_server_submit", "cdg_server_wait", "cdg_server_footprint", "cdg_server_info",
     "cdg_server_last_error", "cdg_server_destroy", "cdg_server_register", "cdg_server_submit_system",
     "cdg_server_unregister",
-    "cdg_fgmres_solve", "cdg_fgmres_default_params",
+    "cdg_fgmres_solve", "cdg_fgmres_solve_multi", "cdg_fgmres_default_params",
     "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian",
 ]
 
@@ -101,6 +101,8 @@ def load() -> C.CDLL:
         "cdg_mv_async": ([vp, vp, dp, dp], C.c_int),
         "cdg_prec_apply_async": ([vp, vp, vp, dp, dp], C.c_int),
         "cdg_prec_apply_noalloc": ([vp, vp, vp, dp, dp], C.c_int),
+        "cdg_prec_apply_multi": ([vp, vp, vp, C.c_int32, dp, dp], C.c_int),
+        "cdg_mv_multi": ([vp, vp, C.c_int32, dp, dp], C.c_int),
         "cdg_mat_create": ([vp, C.c_int32, C.c_int32, ip, ip, ip, pp], C.c_int),
         "cdg_mat_destroy": ([vp, vp], C.c_int),
         "cdg_mat_upload": ([vp, vp, dp], C.c_int),
@@ -117,6 +119,8 @@ def load() -> C.CDLL:
         "cdg_prec_destroy": ([vp, vp], C.c_int),
         "cdg_prec_download": ([vp, vp, dp, dp], C.c_int),
         "cdg_fgmres_solve": ([vp, vp, vp, dp, dp, C.POINTER(FgmresParams), C.POINTER(FgmresStats)], C.c_int),
+        "cdg_fgmres_solve_multi": ([vp, vp, vp, C.c_int32, dp, dp, C.POINTER(FgmresParams), C.POINTER(FgmresStats)],
+                                   C.c_int),
         "cdg_fgmres_default_params": ([C.POINTER(FgmresParams)], None),
         "cdg_prec_set_schedule": ([vp, vp, C.c_int32], C.c_int),
         "cdg_prec_tune": ([vp, vp, vp, C.c_int32, C.POINTER(C.c_double)], C.c_int),

@@ -627,6 +627,7 @@ extern "C" int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M) {
     if (ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (DevMem &m : M->owned) dev_free(ctx, m.p, m.bytes);
+        if (M->ym) dev_free(ctx, M->ym, M->ym_bytes);
     }
     delete M;
     return CDG_OK;
@@ -1420,6 +1421,287 @@ int fgmres_core(Solver &S, std::vector<Workspace> &wss, int depth, const double 
     return CDG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// k right-hand sides in lockstep: k independent runs of the outer FGMRES
+// (type_fgmres.f90:81-310, inner FGMRES off), one per right-hand side, whose
+// Arnoldi steps share launches: one k-vector ILU(0) apply and one k-vector
+// matvec per step (each block read once for all k), the per-vector
+// orthogonalization launches, and one device-to-host fetch of all vectors'
+// scalars.  Each vector keeps its own Hessenberg matrix, rotations,
+// residual estimate, restarts and stopping test, so it takes the iterations it
+// takes alone; a vector that stops waits for the others' cycle to end.
+// Vectors are laid out V[(j k + r) len], Z likewise.
+struct MultiWs {
+    int m = 0, k = 0;
+    int64_t len = 0;
+    double *V = nullptr, *Z = nullptr, *r0 = nullptr, *x0 = nullptr;
+    void *y = nullptr;      // forward vectors of the k-vector apply
+    void *base = nullptr;
+    size_t bytes = 0;
+};
+
+int mws_alloc(cdg_ctx *ctx, MultiWs &ws, int m, int k, int64_t len) {
+    ws.m = m;
+    ws.k = k;
+    ws.len = len;
+    const size_t lv = (size_t)std::max<int64_t>(len, 1) * k;
+    ws.bytes = (2 * (size_t)(m + 1) + 2) * lv * sizeof(double) + lv * sizeof(double);
+    int e = dev_alloc(ctx, &ws.base, ws.bytes);
+    if (e) return e;
+    double *p = static_cast<double *>(ws.base);
+    ws.V = p; p += (size_t)(m + 1) * lv;
+    ws.Z = p; p += (size_t)(m + 1) * lv;
+    ws.r0 = p; p += lv;
+    ws.x0 = p; p += lv;
+    ws.y = p;
+    return CDG_OK;
+}
+
+// z_r = M^-1 v_r for r < k (vectors `len` apart): the k-vector row-slice
+// apply for ILU(0), else one apply per vector.
+int prec_apply_multi(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z, int64_t len, int k,
+                     void *y) {
+    if (M->type == CDG_PREC_IDENTITY) return kv_copy(ctx, len * k, v, z);
+    if (!M->factored) return cdg_fail(ctx, CDG_ERR_STATE, "ILU0 apply before update");
+    if (k > 1 && M->type == CDG_PREC_ILU0 && M->rs_ok) {
+        int e = rs_prepare(ctx, M, A);
+        if (e == CDG_ERR_UNSUPPORTED) {
+            M->rs_ok = 0;
+        } else {
+            if (e) return e;
+            M->napply += k;
+            return launch_ilu_apply_rs_multi(ctx, M, v, z, y, len, k);
+        }
+    }
+    for (int r = 0; r < k; ++r) {
+        int e = prec_apply_dev(ctx, M, A, v + (size_t)r * len, z + (size_t)r * len);
+        if (e) return e;
+    }
+    return CDG_OK;
+}
+
+struct Lane {
+    std::vector<double> h, c, s, p, y;
+    double r0norm_initial = 0.0, res = std::numeric_limits<double>::max();
+    int niter = 0, nvecs = 0, nrestart = 0, nreorth = 0;
+    bool done = false;      // converged (or stopped by nmax)
+    bool in_cycle = false;  // still taking Arnoldi steps in this cycle
+    bool conv = false;      // this cycle ended by the stopping test
+    double &H(int m, int i, int j) { return h[(size_t)j * (m + 1) + i]; }
+};
+
+// Device ops of one vector's orthogonalization of w = V_jj against V_0..V_jj-1
+// (V_i at stride vstride), results in d_h with orthogonalize()'s layout.
+int orth_enqueue(cdg_ctx *ctx, int64_t len, int orth, int jj, double *w, const double *V, int64_t vstride,
+                 double *d_h, bool with_normb, bool gated) {
+    int e = CDG_OK;
+    if (orth == CDG_MGS) {
+        if (with_normb) e = kv_sumsq(ctx, len, w, d_h + jj);
+        for (int i = 0; i < jj && !e; ++i) {
+            e = kv_multidot(ctx, len, w, V + (size_t)i * vstride, vstride, 1, d_h + i);
+            if (!e) e = kv_axpy_dev(ctx, len, w, V + (size_t)i * vstride, d_h + i);
+        }
+        if (!e) e = kv_sumsq(ctx, len, w, d_h + jj + 1);
+        return e;
+    }
+    double *flag = gated ? d_h + jj + 2 : nullptr, *h2 = d_h + jj + 3;
+    e = kv_multidot(ctx, len, w, V, vstride, with_normb ? jj + 1 : jj, d_h);
+    if (!e) e = kv_cgs_update(ctx, len, w, V, vstride, jj, d_h, d_h + jj + 1, nullptr, flag);
+    if (gated) {
+        if (!e) e = kv_multidot(ctx, len, w, V, vstride, jj, h2, flag);
+        if (!e) e = kv_cgs_update(ctx, len, w, V, vstride, jj, h2, h2 + jj, flag);
+    }
+    return e;
+}
+
+int fgmres_multi(Solver &S, MultiWs &ws, int k, const double *B, double *X, int m, double tol, double rtol, int nmax,
+                 int orth, std::vector<Lane> &L) {
+    cdg_ctx *ctx = S.ctx;
+    const int64_t len = S.len;
+    const int64_t kl = (int64_t)k * len;   // stride between basis steps
+    if (m < 1) return cdg_fail(ctx, CDG_ERR_ARG, "nkrylov must be >= 1");
+    const size_t hstride = 2 * (size_t)m + 8;
+    int e = ctx_ensure_small(ctx, 16 + (size_t)m + hstride * k);
+    if (e) return e;
+    double *d_lane = ctx->d_small + 16 + m;   // beyond the scratch kv_combine uses
+    L.assign(k, Lane());
+    for (Lane &ln : L) {
+        ln.h.assign((size_t)(m + 1) * m, 0.0);
+        ln.c.assign(m + 1, 0.0);
+        ln.s.assign(m + 1, 0.0);
+        ln.p.assign(m + 1, 0.0);
+        ln.y.assign(m + 1, 0.0);
+    }
+    e = kv_zero(ctx, kl, ws.x0);
+    if (!e) e = kv_zero(ctx, kl, X);
+    if (e) return e;
+    auto any = [&](bool Lane::*f) { for (Lane &ln : L) if (ln.*f) return true; return false; };
+    auto none_done = [&] { for (Lane &ln : L) if (!ln.done) return true; return false; };
+    while (none_done()) {
+        // r0 = b - A x0, v1 = r0 / ||r0||, p1 = ||r0|| for every vector still running   (:160-163)
+        auto t0 = clk::now();
+        e = launch_mv_multi(ctx, S.A, ws.x0, ws.r0, len, k);
+        for (int r = 0; r < k && !e; ++r) {
+            if (L[r].done) continue;
+            e = kv_sub(ctx, len, B + (size_t)r * len, ws.r0 + (size_t)r * len, ws.r0 + (size_t)r * len);
+            if (!e) e = kv_sumsq(ctx, len, ws.r0 + (size_t)r * len, d_lane + (size_t)r * hstride);
+        }
+        if (e) return e;
+        CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_lane, sizeof(double) * hstride * k, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        e = ctx_wait(ctx);
+        if (e) return e;
+        S.t.norm += secs(t0, clk::now());
+        std::vector<double> r0n(k, 0.0);
+        for (int r = 0; r < k; ++r) {
+            Lane &ln = L[r];
+            ln.in_cycle = false;
+            ln.conv = false;
+            if (ln.done) continue;
+            r0n[r] = std::sqrt(ctx->h_small[(size_t)r * hstride]);
+            if (!std::isfinite(r0n[r])) return cdg_fail(ctx, CDG_ERR_NAN, "non-finite residual norm (vector %d)", r);
+            if (ln.niter == 0) ln.r0norm_initial = r0n[r];
+            std::fill(ln.h.begin(), ln.h.end(), 0.0);
+            std::fill(ln.p.begin(), ln.p.end(), 0.0);
+            std::fill(ln.c.begin(), ln.c.end(), 0.0);
+            std::fill(ln.s.begin(), ln.s.end(), 0.0);
+            if (r0n[r] == 0.0) {   // exact x0: x = x0 (the reference divides by zero here)
+                e = kv_copy(ctx, len, ws.x0 + (size_t)r * len, X + (size_t)r * len);
+                if (e) return e;
+                ln.res = 0.0;
+                ln.done = true;
+                continue;
+            }
+            e = kv_div(ctx, len, ws.r0 + (size_t)r * len, r0n[r], ws.V + (size_t)r * len);
+            if (e) return e;
+            ln.p[0] = r0n[r];
+            ln.nvecs = 0;
+            ln.in_cycle = true;
+        }
+        for (int j = 0; j < m && any(&Lane::in_cycle); ++j) {
+            double *Vj = ws.V + (size_t)j * kl, *Zj = ws.Z + (size_t)j * kl, *Wj = ws.V + (size_t)(j + 1) * kl;
+            auto ta = clk::now();
+            CDG_HIP_TRY(hipEventRecord(S.ev[0][0], ctx->stream));
+            e = prec_apply_multi(ctx, S.M, S.A, Vj, Zj, len, k, ws.y);
+            if (e) return e;
+            CDG_HIP_TRY(hipEventRecord(S.ev[0][1], ctx->stream));
+            e = launch_mv_multi(ctx, S.A, Zj, Wj, len, k);   // w = A z_j into the slot of v_{j+1}   (:189)
+            if (e) return e;
+            CDG_HIP_TRY(hipEventRecord(S.ev[0][2], ctx->stream));
+            // orthogonalize every running vector, then one fetch   (:191-223)
+            const bool gated = orth != CDG_MGS;
+            for (int r = 0; r < k && !e; ++r)
+                if (L[r].in_cycle)
+                    e = orth_enqueue(ctx, len, orth, j + 1, Wj + (size_t)r * len, ws.V + (size_t)r * len, kl,
+                                     d_lane + (size_t)r * hstride, true, gated);
+            if (e) return e;
+            CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_lane, sizeof(double) * hstride * k, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            e = ctx_wait(ctx);
+            if (e) return e;
+            {
+                float ms_p = 0.f, ms_m = 0.f;
+                CDG_HIP_TRY(hipEventElapsedTime(&ms_p, S.ev[0][0], S.ev[0][1]));
+                CDG_HIP_TRY(hipEventElapsedTime(&ms_m, S.ev[0][1], S.ev[0][2]));
+                S.t.precon += 1e-3 * ms_p;
+                S.t.mv += 1e-3 * ms_m;
+                S.t.dot += std::max(0.0, secs(ta, clk::now()) - 1e-3 * (ms_p + ms_m));
+            }
+            const int jj = j + 1;
+            std::vector<int> redo;   // MGS vectors whose test asks for a second pass
+            std::vector<double> nrm2(k, 0.0);
+            for (int r = 0; r < k; ++r) {
+                Lane &ln = L[r];
+                if (!ln.in_cycle) continue;
+                const double *hs = ctx->h_small + (size_t)r * hstride;
+                for (int i = 0; i < jj; ++i) ln.H(m, i, j) += hs[i];
+                const double normb2 = hs[jj];
+                nrm2[r] = hs[jj + 1];
+                if (gated) {
+                    if (hs[jj + 2] != 0.0) {
+                        for (int i = 0; i < jj; ++i) ln.H(m, i, j) += hs[jj + 3 + i];
+                        nrm2[r] = hs[2 * jj + 3];
+                        ln.nreorth++;
+                    }
+                } else {   // selective re-orthogonalization, Giraud-Langou L_crit = 1   (:212-223)
+                    double crit = 0.0;
+                    for (int i = 0; i <= j; ++i) crit += std::fabs(ln.H(m, i, j));
+                    crit /= std::sqrt(normb2);
+                    if (crit >= 1.0) redo.push_back(r);
+                }
+            }
+            if (!redo.empty()) {
+                for (int r : redo)
+                    if (!e) e = orth_enqueue(ctx, len, orth, jj, Wj + (size_t)r * len, ws.V + (size_t)r * len, kl,
+                                             d_lane + (size_t)r * hstride, false, false);
+                if (e) return e;
+                CDG_HIP_TRY(hipMemcpyAsync(ctx->h_small, d_lane, sizeof(double) * hstride * k,
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                e = ctx_wait(ctx);
+                if (e) return e;
+                for (int r : redo) {
+                    const double *hs = ctx->h_small + (size_t)r * hstride;
+                    for (int i = 0; i < jj; ++i) L[r].H(m, i, j) += hs[i];
+                    nrm2[r] = hs[jj + 1];
+                    L[r].nreorth++;
+                }
+            }
+            for (int r = 0; r < k; ++r) {
+                Lane &ln = L[r];
+                if (!ln.in_cycle) continue;
+                ln.nvecs++;
+                // h(j+1,j) = ||w|| ; v_{j+1} = w / h(j+1,j)   (:229-231)
+                ln.H(m, j + 1, j) = std::sqrt(nrm2[r]);
+                e = kv_div_inplace(ctx, len, Wj + (size_t)r * len, ln.H(m, j + 1, j));
+                if (e) return e;
+                // rotations   (:235-264)
+                for (int i = 0; i < j; ++i) {
+                    const double hij = ln.c[i] * ln.H(m, i, j) + ln.s[i] * ln.H(m, i + 1, j);
+                    const double hipj = -ln.s[i] * ln.H(m, i, j) + ln.c[i] * ln.H(m, i + 1, j);
+                    ln.H(m, i, j) = hij;
+                    ln.H(m, i + 1, j) = hipj;
+                }
+                const double gam = std::hypot(ln.H(m, j, j), ln.H(m, j + 1, j));
+                ln.c[j] = ln.H(m, j, j) / gam;
+                ln.s[j] = ln.H(m, j + 1, j) / gam;
+                ln.H(m, j, j) = gam;
+                ln.H(m, j + 1, j) = 0.0;
+                const double pj = ln.c[j] * ln.p[j], pjp = -ln.s[j] * ln.p[j];
+                ln.p[j] = pj;
+                ln.p[j + 1] = pjp;
+                ln.niter++;
+                ln.res = std::fabs(ln.p[j + 1]);
+                if (!std::isfinite(ln.res))
+                    return cdg_fail(ctx, CDG_ERR_NAN, "non-finite FGMRES residual estimate (vector %d)", r);
+                const bool ca = ln.res < tol, cr = (ln.res / ln.r0norm_initial) < rtol,
+                           cn = (ln.niter >= nmax) && (nmax > 0);
+                if (ca || cr || cn) { ln.conv = true; ln.in_cycle = false; }
+            }
+        }
+        // x = x0 + Z y per vector   (:287-294); converged vectors are done, the others restart
+        for (int r = 0; r < k; ++r) {
+            Lane &ln = L[r];
+            if (ln.done || ln.nvecs == 0) continue;
+            for (int i = ln.nvecs - 1; i >= 0; --i) {
+                double acc = ln.p[i];
+                for (int q = i + 1; q < ln.nvecs; ++q) acc -= ln.H(m, i, q) * ln.y[q];
+                ln.y[i] = acc / ln.H(m, i, i);
+            }
+            e = kv_combine(ctx, len, X + (size_t)r * len, ws.x0 + (size_t)r * len, ws.Z + (size_t)r * len, kl,
+                           ln.nvecs, ln.y.data());
+            if (e) return e;
+            if (ln.conv || !(ln.res > tol)) {
+                ln.done = true;
+            } else {
+                e = kv_copy(ctx, len, X + (size_t)r * len, ws.x0 + (size_t)r * len);
+                if (e) return e;
+                ln.nrestart++;
+            }
+        }
+    }
+    return CDG_OK;
+}
+
 }  // namespace
 
 extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, const double *b, double *x,
@@ -1483,5 +1765,119 @@ extern "C" int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, con
         const int e2 = df_check(ctx, M);
         if (e2) return e2;
     }
+    return e;
+}
+
+extern "C" int cdg_fgmres_solve_multi(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, int32_t k, const double *b, double *x,
+                                      const cdg_fgmres_params *prm, cdg_fgmres_stats *st) {
+    if (!ctx || !A || !M || !b || !x || !prm || k < 1) return CDG_ERR_ARG;
+    if (A->nrows != M->nrows || A->n != M->n) return cdg_fail(ctx, CDG_ERR_ARG, "matrix does not match preconditioner");
+    const int64_t len = (int64_t)A->nrows * A->n;
+    if (k == 1 || prm->inner_fgmres) {   // one vector, or nested FGMRES: one solve per vector
+        for (int r = 0; r < k; ++r) {
+            cdg_fgmres_params p1 = *prm;
+            if (r > 0) p1.update_preconditioner = 0;   // the first solve's factor serves the others
+            int e = cdg_fgmres_solve(ctx, A, M, b + (size_t)r * len, x + (size_t)r * len, &p1, st ? st + r : nullptr);
+            if (e) return e;
+        }
+        return CDG_OK;
+    }
+    std::vector<cdg_fgmres_stats> local(k);
+    auto t_start = clk::now();
+    int e = CDG_OK;
+    double t_factor = 0.0;
+    int factored = 0;
+    if (prm->update_preconditioner || !M->factored || M->stamp != A->stamp) {
+        auto tf = clk::now();
+        e = cdg_prec_update(ctx, M, A, nullptr);
+        if (e) return e;
+        factored = 1;
+        t_factor = secs(tf, clk::now());
+    }
+    VecArg bi, xo;
+    e = stage_in(ctx, 0, b, len * k, bi);
+    if (!e) e = stage_in(ctx, 1, x, len * k, xo);
+    if (!e) e = upload_in(ctx, bi);
+    if (e) return e;
+    Solver S{ctx, A, M, len};
+    for (auto &lv : S.ev)
+        for (hipEvent_t &ev : lv)
+            if (!e && hipEventCreate(&ev) != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "hipEventCreate failed");
+    MultiWs ws;
+    if (!e) e = mws_alloc(ctx, ws, prm->nkrylov, k, len);
+    std::vector<Lane> lanes;
+    if (!e) e = fgmres_multi(S, ws, k, bi.dev, xo.dev, prm->nkrylov, prm->tol, prm->rtol, prm->nmax,
+                             prm->orthogonalization, lanes);
+    // residual_norm = ||b - A x|| per vector   (type_fgmres.f90:305,515)
+    if (!e) {
+        const size_t hstride = 2 * (size_t)prm->nkrylov + 8;
+        double *d_lane = ctx->d_small + 16 + prm->nkrylov;
+        e = launch_mv_multi(ctx, A, xo.dev, ws.r0, len, k);
+        for (int r = 0; r < k && !e; ++r) {
+            e = kv_sub(ctx, len, bi.dev + (size_t)r * len, ws.r0 + (size_t)r * len, ws.r0 + (size_t)r * len);
+            if (!e) e = kv_sumsq(ctx, len, ws.r0 + (size_t)r * len, d_lane + (size_t)r * hstride);
+        }
+        if (!e) {
+            hipError_t he = hipMemcpyAsync(ctx->h_small, d_lane, sizeof(double) * hstride * k, hipMemcpyDeviceToHost,
+                                           ctx->stream);
+            e = he == hipSuccess ? ctx_wait(ctx) : cdg_fail(ctx, CDG_ERR_HIP, "%s", hipGetErrorString(he));
+        }
+        for (int r = 0; r < k && !e; ++r) local[r].residual_norm = std::sqrt(ctx->h_small[(size_t)r * hstride]);
+    }
+    if (!e) e = finish_out(ctx, xo);
+    dev_free(ctx, ws.base, ws.bytes);
+    for (auto &lv : S.ev)
+        for (hipEvent_t &ev : lv)
+            if (ev) (void)hipEventDestroy(ev);
+    const double t_total = secs(t_start, clk::now());
+    for (int r = 0; r < k && r < (int)lanes.size(); ++r) {
+        cdg_fgmres_stats &o = local[r];
+        o.niter = lanes[r].niter;
+        o.nrestart = lanes[r].nrestart;
+        o.nreorth = lanes[r].nreorth;
+        o.factored = factored;
+        o.t_factor = t_factor;
+        o.t_precon = S.t.precon;
+        o.t_mv = S.t.mv;
+        o.t_dot = S.t.dot;
+        o.t_norm = S.t.norm;
+        o.t_total = t_total;
+        if (!e && !std::isfinite(o.residual_norm)) e = cdg_fail(ctx, CDG_ERR_NAN, "non-finite final residual");
+    }
+    if (st) std::memcpy(st, local.data(), sizeof(cdg_fgmres_stats) * k);
+    return e;
+}
+
+extern "C" int cdg_mv_multi(cdg_ctx *ctx, const cdg_mat *A, int32_t k, const double *X, double *Y) {
+    if (!ctx || !A || !X || !Y || k < 1) return CDG_ERR_ARG;
+    const int64_t len = (int64_t)A->nrows * A->n;
+    VecArg xi, yo;
+    int e = stage_in(ctx, 0, X, len * k, xi);
+    if (!e) e = stage_in(ctx, 1, Y, len * k, yo);
+    if (!e) e = upload_in(ctx, xi);
+    if (!e) e = launch_mv_multi(ctx, A, xi.dev, yo.dev, len, k);
+    if (!e) e = finish_out(ctx, yo);
+    return e;
+}
+
+extern "C" int cdg_prec_apply_multi(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t k, const double *V, double *Z) {
+    if (!ctx || !M || !A || !V || !Z || k < 1) return CDG_ERR_ARG;
+    if (A->nrows != M->nrows || A->n != M->n) return cdg_fail(ctx, CDG_ERR_ARG, "matrix does not match preconditioner");
+    const int64_t len = (int64_t)A->nrows * A->n;
+    const size_t ybytes = (M->precision == CDG_FP64 ? 8 : 4) * (size_t)len * k;
+    if (M->type == CDG_PREC_ILU0 && M->ym_bytes < ybytes) {   // forward vectors of the k-vector apply
+        if (M->ym) dev_free(ctx, M->ym, M->ym_bytes);
+        M->ym = nullptr;
+        M->ym_bytes = 0;
+        int e = dev_alloc(ctx, &M->ym, ybytes);
+        if (e) return e;
+        M->ym_bytes = ybytes;
+    }
+    VecArg vi, zo;
+    int e = stage_in(ctx, 0, V, len * k, vi);
+    if (!e) e = stage_in(ctx, 1, Z, len * k, zo);
+    if (!e) e = upload_in(ctx, vi);
+    if (!e) e = prec_apply_multi(ctx, M, A, vi.dev, zo.dev, len, k, M->ym);
+    if (!e) e = finish_out(ctx, zo);
     return e;
 }

@@ -156,6 +156,8 @@ struct cdg_prec {
     int32_t *d_up_row = nullptr;       // row of each upper entry
     void *rs_yb = nullptr, *rs_zb = nullptr;
     void *rs_y = nullptr;              // forward result of the row-slice level schedule
+    void *ym = nullptr;                // forward results of k-vector applies (cdg_prec_apply_multi)
+    size_t ym_bytes = 0;
     uint32_t *d_rs_ctl = nullptr;      // error word + ticket counters of the sweep
     uint32_t rs_epoch = 0;             // tag of the last call
     int32_t rs_ready = 0;              // Lrs/Wrs/Drs hold the current factor
@@ -189,6 +191,8 @@ int rs_ctl_words();
 int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
 int launch_ilu_apply_rsl_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y);
 int noalloc_init(int device);
+int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y, int64_t stride, int k);
+int launch_mv_multi(cdg_ctx *ctx, const cdg_mat *A, const double *X, double *Y, int64_t stride, int k);
 bool is_device_ptr(const void *p);
 int rs_slice_rows(int n, int wmax);
 void rs_ctx_count(int delta);

@@ -547,6 +547,136 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_level(RslArgs args) {
     }
 }
 
+// ---- k right-hand sides per task ------------------------------------------------
+// The row-slice level launches of k vectors at once (v, z and y hold k
+// vectors `stride` elements apart): a task streams its block slices once and
+// applies them to all k vectors.  Each vector's products, sums and roundings
+// are the single-vector kernel's, so each gets the same bits as alone.
+struct RsmArgs {
+    const int32_t *tasks;
+    int32_t t0;
+    const void *Sb;
+    int64_t slot;
+    int32_t SL, n, w, dc;
+    const double *v;
+    double *z;
+    void *y;                  // k vectors of nrows * n values of the accumulation type
+    int64_t stride;
+    int32_t k;                // right-hand sides of this launch (<= K)
+    int32_t nbmax;            // blocks per task at most (parents per task)
+};
+
+template <typename S, int K, int R>
+__global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
+    typedef Raw16<S> RW;
+    typedef typename RW::Acc Acc;
+    typedef typename RW::T RawT;
+    typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type VT;
+    constexpr int V = RW::V, NB = 4;
+    const S *const Sb = static_cast<const S *>(args.Sb);
+    const int64_t slot = args.slot, stride = args.stride;
+    const int SL = args.SL, n = args.n, w = args.w, dc = args.dc, k = args.k, nbmax = args.nbmax;
+    VT *const y = static_cast<VT *>(args.y);
+    double *const zout = args.z;
+    extern __shared__ unsigned char smem_raw[];
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int xsz = nbmax * n;                                       // parents of one right-hand side
+    Acc *xs = reinterpret_cast<Acc *>(smem_raw);                     // K * xsz
+    Acc *red = xs + (((size_t)K * xsz + 3) & ~(size_t)3);            // V*T
+    Acc *red2 = red + (size_t)V * T;                                 // T
+    const int lim = w * n, step = V * T, lim_ld = (lim - 1) / V * V;
+    const int32_t *rec = args.tasks + (size_t)(args.t0 + (int)blockIdx.x) * kRsRec;
+    const int row = rec[0];
+    const int sl = rec[1] & 0xff, nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1;
+    // the block slices first (they do not depend on the parents), then the
+    // parents of all k right-hand sides into LDS
+    RawT raw[NB][R];
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+        const int kk = kb < nb ? kb : 0;
+        const S *b = Sb + (size_t)rec[2 + kk] * slot + (size_t)sl * SL;
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+            raw[kb][m] = __builtin_nontemporal_load(
+                reinterpret_cast<const RawT *>(b + (kb < nb ? min(V * tid + m * step, lim_ld) : 0)));
+    }
+    const int ne = nb * n;
+    for (int j = tid; j < k * ne; j += T) {
+        const int r = j / ne, jj = j - r * ne;
+        const int kb = jj / n, c = jj - kb * n;
+        const size_t gi = (size_t)r * stride + (size_t)rec[2 + kRsMaxBlocks + kb] * n + c;
+        const VT x = (bwd && jj >= n) ? static_cast<VT>(zout[gi]) : y[gi];
+        xs[r * xsz + jj] = (bwd && jj < n) ? static_cast<Acc>(x) : -static_cast<Acc>(x);
+    }
+    __syncthreads();
+    int col0[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) col0[v] = (V * tid + v) / w;
+    const bool two = w >= V;
+    const int split = min(V, w * (col0[0] + 1) - V * tid);
+    Acc acc[K][V];
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[r][v] = 0;
+    auto fma_group = [&](const Acc (&x)[V], int e, int kb, int m) {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            const Acc *xk = xs + r * xsz + kb * n;
+            if (two) {
+                const Acc xa = xk[min(col0[0] + m * dc, n - 1)], xb = xk[min(col0[0] + 1 + m * dc, n - 1)];
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    if (e + v < lim) acc[r][v] += x[v] * (v < split ? xa : xb);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    if (e + v < lim) acc[r][v] += x[v] * xk[min(col0[v] + m * dc, n - 1)];
+            }
+        }
+    };
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+        if (kb < nb) {
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                Acc x[V];
+                RW::get(raw[kb][m], x);
+                fma_group(x, V * tid + m * step, kb, m);
+            }
+        }
+    }
+    if (R * step < lim || nb > NB) {
+        for (int kb = 0; kb < nb; ++kb) {
+            const S *b = Sb + (size_t)rec[2 + kb] * slot + (size_t)sl * SL;
+            for (int m = kb < NB ? R : 0; m * step < lim; ++m) {
+                const int e = V * tid + m * step;
+                if (e < lim) {
+                    Acc x[V];
+                    RW::get(__builtin_nontemporal_load(reinterpret_cast<const RawT *>(b + e)), x);
+                    fma_group(x, e, kb, m);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        if (r >= k) break;
+        __syncthreads();   // the previous right-hand side's row sums are done with red and red2
+#pragma unroll
+        for (int v = 0; v < V; ++v) red[V * tid + v] = acc[r][v];
+        __syncthreads();
+        const Acc s = rs_rowsum(red, red2, w, dc, T, tid);
+        if (tid < w) {
+            const size_t gi = (size_t)row * n + (size_t)sl * w + tid;
+            if (!bwd)
+                y[(size_t)r * stride + gi] = static_cast<VT>(args.v[(size_t)r * stride + gi] + static_cast<double>(s));
+            else
+                zout[(size_t)r * stride + gi] = static_cast<double>(static_cast<VT>(s));
+        }
+    }
+}
+
 // ---- conversions to the storage type ----------------------------------------
 template <typename SO> __device__ __forceinline__ SO to_store(double v);
 template <> __device__ __forceinline__ double to_store<double>(double v) { return v; }
@@ -619,6 +749,22 @@ const void *rsl_kernel(int R, int EPT) {
     CDG_RS_E(8)
 #undef CDG_RS_E
 #undef CDG_RS_K
+}
+
+template <typename S, int K>
+const void *rsm_kernel_k(int R) {
+    if (R <= 2) return reinterpret_cast<const void *>(&k_rs_multi<S, K, 2>);
+    if (R <= 3) return reinterpret_cast<const void *>(&k_rs_multi<S, K, 3>);
+    if (R <= 4) return reinterpret_cast<const void *>(&k_rs_multi<S, K, 4>);
+    if (R <= 6) return reinterpret_cast<const void *>(&k_rs_multi<S, K, 6>);
+    return reinterpret_cast<const void *>(&k_rs_multi<S, K, 8>);
+}
+
+template <typename S>
+const void *rsm_kernel(int K, int R) {
+    if (K <= 2) return rsm_kernel_k<S, 2>(R);
+    if (K <= 4) return rsm_kernel_k<S, 4>(R);
+    return rsm_kernel_k<S, 8>(R);
 }
 
 int round_pow(int x) { return x <= 2 ? 2 : (x <= 4 ? 4 : 8); }
@@ -911,4 +1057,75 @@ int launch_ilu_apply_rsl_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double 
 
 int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z) {
     return launch_ilu_apply_rsl_buf(ctx, M, v, z, M->rs_y);
+}
+
+// Right-hand sides per launch of the k-vector apply: as many as keep the
+// parents of one task within 48 KB of LDS (1, 2, 4 or 8).
+int rs_multi_width(const cdg_prec *M) {
+    const size_t accsz = M->precision == CDG_FP64 ? 8 : 4;
+    const size_t per = (size_t)(M->maxdep + 1) * M->n * accsz;
+    const int fit = (int)std::max<size_t>(1, (48 * 1024) / std::max<size_t>(per, 1));
+    return fit >= 8 ? 8 : fit >= 4 ? 4 : fit >= 2 ? 2 : 1;
+}
+
+// k applies of the row-slice schedule at once: z_r = M^-1 v_r for r < k
+// (v, z: k vectors `stride` apart, device); y: the forward vectors, k
+// vectors of nrows * n values of the accumulation type `stride` apart.
+// The row-slice storage must be prepared (rs_prepare).  Groups of up to
+// rs_multi_width(M) right-hand sides share each block read; a single vector
+// runs the single-vector kernel.
+int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y, int64_t stride, int k) {
+    if (M->nrows == 0 || k <= 0) return CDG_OK;
+    const int S = M->rs_S, nf = S * M->nrows;
+    const int kw = rs_multi_width(M);
+    const size_t vt = M->precision == CDG_FP64 ? 8 : 4;
+    for (int r0 = 0; r0 < k; r0 += kw) {
+        const int kk = std::min(kw, k - r0);
+        const double *vr = v + (size_t)r0 * stride;
+        double *zr = z + (size_t)r0 * stride;
+        void *yr = static_cast<char *>(y) + vt * (size_t)r0 * stride;
+        if (kk == 1) {
+            int e = launch_ilu_apply_rsl_buf(ctx, M, vr, zr, yr);
+            if (e) return e;
+            continue;
+        }
+        const int K = kk <= 2 ? 2 : kk <= 4 ? 4 : 8;
+        RsmArgs a{};
+        a.tasks = M->rs_tasks;
+        a.Sb = M->Lrs;
+        a.slot = M->rs_slot;
+        a.SL = (int32_t)M->rs_SL;
+        a.n = M->n;
+        a.w = M->rs_w;
+        a.dc = M->rs_dc;
+        a.v = vr;
+        a.z = zr;
+        a.y = yr;
+        a.stride = stride;
+        a.k = kk;
+        a.nbmax = M->maxdep + 1;
+        const void *fn = M->precision == CDG_FP64 ? rsm_kernel<double>(K, M->rs_R)
+                       : (M->precision == CDG_FP32 ? rsm_kernel<float>(K, M->rs_R)
+                                                   : rsm_kernel<cdg_bf16>(K, M->rs_R));
+        const size_t accsz = M->precision == CDG_FP64 ? 8 : 4;
+        const int V = M->precision == CDG_FP64 ? 2 : (M->precision == CDG_FP32 ? 4 : 8);
+        const size_t lds = accsz * ((((size_t)K * a.nbmax * M->n + 3) & ~(size_t)3) + (size_t)V * M->rs_T +
+                                    (size_t)M->rs_T);
+        auto run = [&](int lo, int hi) -> int {
+            if (hi <= lo) return CDG_OK;
+            a.t0 = lo;
+            void *args[] = {&a};
+            CDG_HIP_TRY(hipLaunchKernel(fn, dim3(hi - lo), dim3(M->rs_T), args, lds, ctx->stream));
+            return CDG_OK;
+        };
+        for (int l = 0; l < M->nlev; ++l) {
+            int e = run(S * M->lev_row_ptr[l], S * M->lev_row_ptr[l + 1]);
+            if (e) return e;
+        }
+        for (int l = 0; l < M->nblev; ++l) {
+            int e = run(nf + S * M->blev_row_ptr[l], nf + S * M->blev_row_ptr[l + 1]);
+            if (e) return e;
+        }
+    }
+    return CDG_OK;
 }

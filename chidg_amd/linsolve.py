@@ -415,6 +415,28 @@ class fgmres:
             M.stamp = A.stamp
         return x
 
+    def solve_multi(self, A: chidg_matrix, X, B, M: _Preconditioner, solver_controller=None, comm=None):
+        """k right-hand sides with one preconditioner (``cdg_fgmres_solve_multi``):
+        B and X are (k, nrows*n); every row is the solve of that right-hand
+        side alone, the Arnoldi steps share k-vector applies and matvecs.
+        ``niter`` / ``residual_norm`` become lists, ``stats`` a list of k."""
+        if not M.initialized:
+            M.init(A)
+        k = int(B.shape[0])
+        assert tuple(X.shape) == tuple(B.shape) and B.shape[1] == A.vec_len
+        update = True if solver_controller is None else solver_controller.update_preconditioner(A, M, comm)
+        prm = self.params(update_preconditioner=update)
+        st = (_lib.FgmresStats * k)()
+        _after_torch(A.ctx, X, B)
+        A.ctx._check(A.ctx.lib.cdg_fgmres_solve_multi(A.ctx.h, A.h, M.h, k, _ptr(B), _ptr(X), C.byref(prm), st),
+                     "cdg_fgmres_solve_multi")
+        self.stats = list(st)
+        self.niter = [s.niter for s in st]
+        self.residual_norm = [s.residual_norm for s in st]
+        if st[0].factored:
+            M.stamp = A.stamp
+        return X
+
 
 class solver_controller:
     """``solver_controller_t`` (type_solver_controller.f90:25-124): controls

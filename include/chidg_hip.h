@@ -152,6 +152,11 @@ int cdg_mv(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y);
  * stream and the call returns without waiting. */
 int cdg_mv_async(cdg_ctx *ctx, const cdg_mat *A, const double *x, double *y);
 
+/* Y_r = A X_r for r < k: k vectors of nrows*n one after another (host or
+ * device), each block read once for all k.  Each Y_r has the bits cdg_mv
+ * gives for X_r alone.  No reference counterpart beyond chidg_mv per vector. */
+int cdg_mv_multi(cdg_ctx *ctx, const cdg_mat *A, int32_t k, const double *X, double *Y);
+
 /* ---- preconditioner: precon_ILU0_t / precon_identity -------------------------
  * create == preconditioner%init (precon_ILU0.f90:64-90)                        */
 int cdg_prec_create(cdg_ctx *ctx, const cdg_mat *A, int32_t type, int32_t precision,
@@ -162,6 +167,11 @@ int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_ro
 /* z = preconditioner%apply(A,v) (precon_ILU0.f90:286-411). */
 int cdg_prec_apply(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
 int cdg_prec_apply_async(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const double *v, double *z);
+/* Z_r = M^-1 V_r for r < k (k vectors one after another, host or device):
+ * the row-slice schedule with each block read once for all k (ILU(0));
+ * each Z_r has the bits of CDG_SCHEDULE_ROWSLICE's apply of V_r alone.
+ * precon_ILU0%apply (precon_ILU0.f90:286-411) per vector in the reference. */
+int cdg_prec_apply_multi(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t k, const double *V, double *Z);
 /* preconditioner%tear_down (precon_ILU0.f90:450-470). */
 int cdg_prec_destroy(cdg_ctx *ctx, cdg_prec *M);
 /* Schedule of the ILU(0) apply (no reference counterpart: the reference's
@@ -238,6 +248,18 @@ int cdg_prec_download(cdg_ctx *ctx, const cdg_prec *M, double *L, double *Dinv);
  * x is overwritten (zero initial guess).  Updates M per params. */
 int cdg_fgmres_solve(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, const double *b, double *x,
                      const cdg_fgmres_params *params, cdg_fgmres_stats *stats);
+
+/* k right-hand sides of one system with one preconditioner: k runs of
+ * fgmres_t%solve (type_fgmres.f90:81-310) on the same A and M, in lockstep.
+ * b and x hold k vectors of nrows*n doubles one after another (host or
+ * device); stats (may be NULL) receives k entries.  Each vector keeps its own
+ * Arnoldi process, rotations, restarts and stopping test (the iterations it
+ * takes alone); the steps share one k-vector ILU(0) apply and one k-vector
+ * matvec (each block read once for all k) and one host fetch.  M is updated
+ * once, per params.  With inner_fgmres the vectors are solved one after
+ * another.  No reference counterpart beyond the per-vector solve. */
+int cdg_fgmres_solve_multi(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, int32_t k, const double *b, double *x,
+                           const cdg_fgmres_params *params, cdg_fgmres_stats *stats);
 
 /* Reference defaults of linear_solver_t (type_linear_solver.f90:21-40). */
 void cdg_fgmres_default_params(cdg_fgmres_params *params);
