@@ -160,7 +160,8 @@ extern "C" int cdg_server_footprint(const cdg_request *req, const cdg_fgmres_par
     int64_t f = 8 * nnzb * ld;                                   // A
     f += factor_extra_bytes(nrows, nlo, n, req->precision);
     const int64_t m = std::max(1, params->nkrylov);
-    f += 8 * len * (2 * (m + 1) + 6);                            // V, Z, b, x, work vectors
+    const int64_t k = std::max(1, req->nrhs);
+    f += 8 * len * k * (2 * (m + 1) + 6);                        // V, Z, b, x, work vectors per right-hand side
     if (params->inner_fgmres) f += 8 * len * (2 * (std::max(1, params->inner_nkrylov) + 1) + 4);
     *bytes = f + f / 20 + (64ll << 20);
     return CDG_OK;
@@ -178,36 +179,34 @@ int run_job(cdg_ctx *ctx, const cdg_fgmres_params &params, Job &job, cdg_result 
     cdg_mat *A = nullptr;
     cdg_prec *M = nullptr;
     double *b = nullptr, *x = nullptr;
+    const int nrhs = std::max(1, r.nrhs);
     int e = cdg_mat_create(ctx, nrows, n, rp.data(), col.data(), kind.data(), &A);
     if (!e) e = cdg_mat_fill_synthetic(ctx, A, r.seed_A, r.s_diag, r.s_off);
-    if (!e) e = dev_alloc(ctx, (void **)&b, sizeof(double) * len);
-    if (!e) e = dev_alloc(ctx, (void **)&x, sizeof(double) * len);
-    if (!e) e = launch_vec_synthetic(ctx, len, r.seed_b, 1.0, b);
+    if (!e) e = dev_alloc(ctx, (void **)&b, sizeof(double) * len * nrhs);
+    if (!e) e = dev_alloc(ctx, (void **)&x, sizeof(double) * len * nrhs);
+    // right-hand sides k = 0.. (seeds seed_b + k) share the matrix and the
+    // factor (updated once); several of them are solved in lockstep, each
+    // block read once per step for all of them
+    for (int k = 0; k < nrhs && !e; ++k)
+        e = launch_vec_synthetic(ctx, len, r.seed_b + (uint64_t)k, 1.0, b + (size_t)k * len);
     if (!e) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, r.precision, &M);
-    // several right-hand sides share the factor: the stream schedule (one
-    // more GEMM per upper block and the row-slice copies, then one persistent
-    // launch per apply)
-    if (!e && r.nrhs > 1) e = cdg_prec_set_schedule(ctx, M, CDG_SCHEDULE_STREAM);
-    // right-hand sides k = 0.. share the matrix and the factor (updated once)
-    const int nrhs = std::max(1, r.nrhs);
     cdg_fgmres_params prm = params;
+    prm.update_preconditioner = 1;
     res.niter = 0;
     res.residual_norm = 0.0;
+    std::vector<cdg_fgmres_stats> st(nrhs);
+    if (!e) e = cdg_fgmres_solve_multi(ctx, A, M, nrhs, b, x, &prm, st.data());
     for (int k = 0; k < nrhs && !e; ++k) {
-        if (k > 0) e = launch_vec_synthetic(ctx, len, r.seed_b + (uint64_t)k, 1.0, b);
-        prm.update_preconditioner = k == 0 ? 1 : 0;
-        cdg_fgmres_stats st{};
-        if (!e) e = cdg_fgmres_solve(ctx, A, M, b, x, &prm, &st);
         double bn = 0.0;
-        if (!e) e = ctx_norm2(ctx, len, b, &bn);
-        if (!e && k == 0 && job.x_out) {
-            if (hipMemcpyAsync(job.x_out, x, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-                hipStreamSynchronize(ctx->stream) != hipSuccess)
-                e = cdg_fail(ctx, CDG_ERR_HIP, "copying the solution out failed");
-        }
-        res.niter += st.niter;
-        res.residual_norm = std::max(res.residual_norm, bn > 0 ? st.residual_norm / bn : st.residual_norm);
-        res.t_factor += st.t_factor;
+        e = ctx_norm2(ctx, len, b + (size_t)k * len, &bn);
+        res.niter += st[k].niter;
+        res.residual_norm = std::max(res.residual_norm, bn > 0 ? st[k].residual_norm / bn : st[k].residual_norm);
+    }
+    if (!e) res.t_factor = st[0].t_factor;
+    if (!e && job.x_out) {
+        if (hipMemcpyAsync(job.x_out, x, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            e = cdg_fail(ctx, CDG_ERR_HIP, "copying the solution out failed");
     }
     if (M) cdg_prec_destroy(ctx, M);
     if (A) cdg_mat_destroy(ctx, A);
@@ -252,23 +251,25 @@ int run_system_job(cdg_server *s, cdg_ctx *ctx, Job &job, cdg_result &res) {
     if (!sys->M) e = cdg_prec_create(ctx, A, CDG_PREC_ILU0, sys->precision, &sys->M);
     const int64_t len = (int64_t)A->nrows * A->n;
     double *xs = nullptr;
-    if (!e && !job.x_out) e = dev_alloc(ctx, (void **)&xs, sizeof(double) * (size_t)std::max<int64_t>(len, 1));
+    if (!e && !job.x_out)
+        e = dev_alloc(ctx, (void **)&xs, sizeof(double) * (size_t)std::max<int64_t>(len, 1) * std::max(1, job.nrhs));
     cdg_fgmres_params prm = s->params;
     prm.update_preconditioner = 0;
     res.niter = 0;
     res.residual_norm = 0.0;
     res.factored = 0;
-    for (int k = 0; k < std::max(1, job.nrhs) && !e; ++k) {
-        const double *bk = job.b + (size_t)k * len;
-        double *xk = job.x_out ? job.x_out + (size_t)k * len : xs;
-        cdg_fgmres_stats st{};
-        e = cdg_fgmres_solve(ctx, A, sys->M, bk, xk, &prm, &st);
+    const int nrhs = std::max(1, job.nrhs);
+    std::vector<cdg_fgmres_stats> st(nrhs);
+    if (!e) e = cdg_fgmres_solve_multi(ctx, A, sys->M, nrhs, job.b, job.x_out ? job.x_out : xs, &prm, st.data());
+    for (int k = 0; k < nrhs && !e; ++k) {
         double bn = 0.0;
-        if (!e) e = vec_norm(ctx, bk, len, &bn);
-        res.niter += st.niter;
-        res.residual_norm = std::max(res.residual_norm, bn > 0 ? st.residual_norm / bn : st.residual_norm);
-        res.t_factor += st.t_factor;
-        res.factored += st.factored;
+        e = vec_norm(ctx, job.b + (size_t)k * len, len, &bn);
+        res.niter += st[k].niter;
+        res.residual_norm = std::max(res.residual_norm, bn > 0 ? st[k].residual_norm / bn : st[k].residual_norm);
+    }
+    if (!e) {
+        res.t_factor = st[0].t_factor;
+        res.factored = st[0].factored;
     }
     if (xs) dev_free(ctx, xs, 0);
     return e;

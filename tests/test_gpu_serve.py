@@ -58,8 +58,8 @@ def test_poisson_burst_completes():
 
 
 def test_multi_rhs_shares_factor():
-    """nrhs right-hand sides on one factor (stream schedule: W = Dinv U and
-    the row-slice copies formed once): the first agrees with a one-RHS request
+    """nrhs right-hand sides on one factor, solved in lockstep (k-vector
+    row-slice apply and matvec): the first agrees with a one-RHS request
     (level schedule) to the distance of two solutions at rtol 1e-8, every one
     converges, and the same request gives the same bits again."""
     one = Request(6, 6, 6, 2, seed_A=301, seed_b=11)
