@@ -314,17 +314,22 @@ def batch_run(args):
     return {k: sig(float(v)) if isinstance(v, float) else v for k, v in out.items()}
 
 
+SERVE_RATES = (10.0, 50.0, 100.0, 200.0, 500.0)
+
+
 def serve_run(args):
-    """Poisson arrivals of sweep solves (meshes 4^3..8^3, P1..P3, Mach-scaled
-    coupling), FP32 ILU(0), 6 slots by default; p50/p99 latency from submit to done."""
+    """Poisson arrivals of sweep solves at the config-3/4 size mix (meshes
+    4^3..12^3, P1..P3, Mach-scaled coupling), FP32 ILU(0), 6 slots by
+    default, at each rate of SERVE_RATES (requests per second); p50/p99
+    latency from submit to done over ``--serve-requests`` requests per rate."""
     from chidg_amd.serve import Server, poisson_load, sweep_requests
     out = []
     with Server(nslots=args.serve_slots, nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False) as srv:
         warm = sweep_requests(4, np.random.default_rng(1), meshes=(4, 6))
         poisson_load(srv, warm, rate=1000.0, rng=np.random.default_rng(2))
-        for rate in (10.0, 100.0, 500.0):
+        for rate in SERVE_RATES:
             rng = np.random.default_rng(int(rate))
-            reqs = sweep_requests(args.serve_requests, rng, meshes=(4, 8))
+            reqs = sweep_requests(args.serve_requests, rng, meshes=(4, 12))
             r = poisson_load(srv, reqs, rate=rate, rng=rng)
             log(f"serve rate {rate:.0f}/s: {r['solves_per_s']:.1f} solves/s, p50 {r['p50_ms']:.1f} ms, "
                 f"p99 {r['p99_ms']:.1f} ms, peak {r['peak_active']} concurrent")
@@ -420,7 +425,7 @@ def main():
                     help="comma list of extra sections: synthetic (seeded synthetic 16^3 P2 system), p3 "
                          "(16^3 P3 mixed precision), batch (64 uneven systems), serve (Poisson request "
                          "loop); empty for none")
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests per arrival rate")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests per arrival rate")
     # 6 slots: the process gets 4 hardware queues; at 100 req/s 6 slots gave p99 63-76 ms
     # against 82-104 ms with 8 (4 slots: 123-126 ms), the same batch throughput
     ap.add_argument("--serve-slots", type=int, default=6, help="request-loop slots (contexts) for the serve section")
