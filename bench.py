@@ -47,10 +47,10 @@ HBM_PEAK_GBS = 8000.0
 # k_rs_level<float,...>, one per wavefront level (46 forward, 46 backward),
 # 40,960 tasks of (block row, 27-row slice) in all.
 APPLY_KERNEL = "ILU(0) apply FP32, row-slice schedule: k_rs_level<float,4,R,E> x 92 launches per call"
-# HBM bytes per apply call from rocprofv3 FETCH_SIZE + WRITE_SIZE on the Euler 16^3 P2
-# system, each scaled by what the same counter reads for a 512 MiB device copy; None
-# until measured for this kernel.
-APPLY_TRAFFIC_FP32 = None
+# HBM bytes per FP32 row-slice apply call: rocprofv3 FETCH_SIZE (x 2.0) + WRITE_SIZE
+# (x 1.0) on the Euler 16^3 P2 system, the factors being what the same counters read for
+# a 512 MiB device copy (1,015,833 + 7,520 KiB raw per call).
+APPLY_TRAFFIC_FP32 = 2.088e9
 
 
 def sig(v, digits=5):
