@@ -31,7 +31,8 @@ EXPORTS = [
     "cdg_server_last_error", "cdg_server_destroy", "cdg_server_register", "cdg_server_submit_system",
     "cdg_server_unregister",
     "cdg_fgmres_solve", "cdg_fgmres_solve_multi", "cdg_fgmres_default_params",
-    "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian",
+    "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian", "cdg_euler_residual_gpu",
+    "cdg_vec_axpby", "cdg_vec_dot", "cdg_vec_norm",
 ]
 
 
@@ -139,6 +140,10 @@ def load() -> C.CDLL:
         "cdg_server_unregister": ([vp, C.c_uint64], C.c_int),
         "cdg_euler_state": ([C.c_int32] * 4 + [C.c_double, C.c_uint64, C.c_double, dp], C.c_int),
         "cdg_euler_residual": ([C.c_int32] * 4 + [dp, dp, C.c_int32], C.c_int),
+        "cdg_euler_residual_gpu": ([vp] + [C.c_int32] * 4 + [dp, dp], C.c_int),
+        "cdg_vec_axpby": ([vp, C.c_int64, C.c_double, dp, C.c_double, dp], C.c_int),
+        "cdg_vec_dot": ([vp, C.c_int64, dp, dp, C.POINTER(C.c_double)], C.c_int),
+        "cdg_vec_norm": ([vp, C.c_int64, dp, C.POINTER(C.c_double)], C.c_int),
         "cdg_euler_jacobian": ([C.c_int32] * 4 + [dp, C.c_double, dp, dp, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():

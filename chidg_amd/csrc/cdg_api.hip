@@ -1881,3 +1881,56 @@ extern "C" int cdg_prec_apply_multi(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A,
     if (!e) e = finish_out(ctx, zo);
     return e;
 }
+
+// ---- device-resident vectors (chidg_vector_t operations, type_chidg_vector.f90) ----
+extern "C" int cdg_vec_axpby(cdg_ctx *ctx, int64_t len, double a, const double *x, double b, double *y) {
+    if (!ctx || !x || !y || len < 0) return CDG_ERR_ARG;
+    if (!is_device_ptr(x) || !is_device_ptr(y)) return cdg_fail(ctx, CDG_ERR_ARG, "cdg_vec_axpby needs device pointers");
+    int e = kv_axpby(ctx, len, a, x, b, y);
+    if (!e) e = ctx_wait(ctx);
+    return e;
+}
+
+extern "C" int cdg_vec_dot(cdg_ctx *ctx, int64_t len, const double *x, const double *y, double *out) {
+    if (!ctx || !x || !y || !out || len < 0) return CDG_ERR_ARG;
+    VecArg xi, yi;
+    int e = stage_in(ctx, 0, x, len, xi);
+    if (!e) e = stage_in(ctx, 1, y, len, yi);
+    if (!e) e = upload_in(ctx, xi);
+    if (!e) e = upload_in(ctx, yi);
+    if (!e) e = kv_multidot(ctx, len, xi.dev, yi.dev, len, 1, ctx->d_small);
+    if (!e) e = fetch_small(ctx, 1);
+    if (!e) *out = ctx->h_small[0];
+    return e;
+}
+
+extern "C" int cdg_vec_norm(cdg_ctx *ctx, int64_t len, const double *x, double *out) {
+    if (!ctx || !x || !out || len < 0) return CDG_ERR_ARG;
+    VecArg xi;
+    int e = stage_in(ctx, 0, x, len, xi);
+    if (!e) e = upload_in(ctx, xi);
+    if (!e) e = kv_sumsq(ctx, len, xi.dev, ctx->d_small);
+    if (!e) e = fetch_small(ctx, 1);
+    if (!e) *out = std::sqrt(ctx->h_small[0]);
+    return e;
+}
+
+// ---- DG Euler residual on the device (mod_spatial.f90:40-197) ----------------
+extern "C" int cdg_euler_residual_gpu(cdg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q,
+                                      double *R) {
+    if (!ctx || !q || !R) return CDG_ERR_ARG;
+    if (nx < 2 || nx > 256 || ny != nx || nz != nx || order < 0 || order > 3) return CDG_ERR_ARG;
+    const int64_t len = (int64_t)nx * ny * nz * 5 * (order + 1) * (order + 1) * (order + 1);
+    VecArg qi, ro;
+    int e = stage_in(ctx, 0, q, len, qi);
+    if (!e) e = stage_in(ctx, 1, R, len, ro);
+    if (!e) e = upload_in(ctx, qi);
+    if (!e) e = launch_euler_residual(ctx, nx, ny, nz, order, qi.dev, ro.dev);
+    if (!e) e = finish_out(ctx, ro);
+    if (!e) {   // a state with negative pressure gives a non-finite residual, as on the host
+        e = kv_sumsq(ctx, len, ro.dev, ctx->d_small);
+        if (!e) e = fetch_small(ctx, 1);
+        if (!e && !std::isfinite(ctx->h_small[0])) e = cdg_fail(ctx, CDG_ERR_NAN, "non-finite Euler residual");
+    }
+    return e;
+}

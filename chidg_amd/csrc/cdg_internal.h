@@ -226,6 +226,8 @@ int kv_add(cdg_ctx *ctx, int64_t len, double *z, const double *dz);             
 int kv_combine(cdg_ctx *ctx, int64_t len, double *x, const double *x0, const double *Z, int64_t zstride,
                int k, const double *h_y /* host */);                                    // x = x0 + sum y_i Z_i
 int kv_zero(cdg_ctx *ctx, int64_t len, double *x);
+int kv_axpby(cdg_ctx *ctx, int64_t len, double a, const double *x, double b, double *y);   // y = a x + b y
+int launch_euler_residual(cdg_ctx *ctx, int nx, int ny, int nz, int order, const double *q, double *R);
 int kv_copy(cdg_ctx *ctx, int64_t len, const double *src, double *dst);
 
 int ctx_ensure_small(cdg_ctx *ctx, size_t n);

@@ -250,6 +250,11 @@ __global__ void k_combine(int64_t len, double *__restrict__ x, const double *__r
     }
 }
 
+__global__ void k_axpby(int64_t len, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = a * x[i] + b * y[i];
+}
+
 __global__ void k_fill(int64_t len, double *__restrict__ x, double v) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < len; k += (int64_t)gridDim.x * blockDim.x)
         x[k] = v;
@@ -350,6 +355,13 @@ int kv_combine(cdg_ctx *ctx, int64_t len, double *x, const double *x0, const dou
     CDG_HIP_TRY(hipGetLastError());
     // the host array may be reused right away
     CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return CDG_OK;
+}
+
+int kv_axpby(cdg_ctx *ctx, int64_t len, double a, const double *x, double b, double *y) {
+    if (len == 0) return CDG_OK;
+    hipLaunchKernelGGL(k_axpby, egrid(len), dim3(256), 0, ctx->stream, len, a, x, b, y);
+    CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
 }
 

@@ -61,3 +61,14 @@ def system(N: int, order: int, mach: float = 0.3, seed: int = 1, perturb: float 
     q = state(N, order, mach, seed, perturb)
     p, blocks, R = jacobian(N, order, q, cfl, nthreads)
     return p, blocks, R, q
+
+
+def residual_gpu(ctx, N: int, order: int, q, out=None):
+    """R(q) on the context's GPU (``cdg_euler_residual_gpu``): q a host array or
+    a device tensor; R of the same kind (``out`` if given)."""
+    from .linsolve import _after_torch, _before_torch, _empty_like, _ptr as _vptr
+    R = out if out is not None else _empty_like(q, len(q))
+    _after_torch(ctx, q, R)
+    ctx._check(ctx.lib.cdg_euler_residual_gpu(ctx.h, N, N, N, order, _vptr(q), _vptr(R)), "cdg_euler_residual_gpu")
+    _before_torch(ctx)
+    return R

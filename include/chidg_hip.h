@@ -326,6 +326,16 @@ int cdg_server_submit_system(cdg_server *s, uint64_t system, const double *b, in
 int cdg_server_unregister(cdg_server *s, uint64_t system);
 int cdg_server_destroy(cdg_server *s);
 
+/* ---- device-resident vectors ---------------------------------------------------
+ * chidg_vector_t operations on vectors that stay on the device between calls
+ * (type_chidg_vector.f90: the overloaded +, -, * (:1740-2080) and norm
+ * (:1237-1300); operator_chidg_dot.f90:30-89 dot), so a Newton loop keeps q,
+ * dq and rhs in HBM.  cdg_vec_axpby: y = a x + b y (device pointers);
+ * cdg_vec_dot / cdg_vec_norm: host or device vectors, result on the host. */
+int cdg_vec_axpby(cdg_ctx *ctx, int64_t len, double a, const double *x, double b, double *y);
+int cdg_vec_dot(cdg_ctx *ctx, int64_t len, const double *x, const double *y, double *out);
+int cdg_vec_norm(cdg_ctx *ctx, int64_t len, const double *x, double *out);
+
 /* ---- DG Euler first-Newton-step system (host code) --------------------------
  * The system ChiDG assembles for the first Newton step of a steady Euler case
  * (mod_spatial.f90:40-197 residual and Jacobian, type_newton.f90:320-351
@@ -349,6 +359,11 @@ int cdg_euler_state(int32_t nx, int32_t ny, int32_t nz, int32_t order, double ma
 /* R(q): sum_faces int phi Fhat.n - int grad(phi).F over each element. */
 int cdg_euler_residual(int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double *R,
                        int32_t nthreads);
+/* R(q) on the context's GPU: the same discretization, one workgroup per
+ * element (the spatial residual assembly of mod_spatial.f90:40-197); q and R
+ * host or device; equal to cdg_euler_residual to rounding (sums in another
+ * order).  CDG_ERR_NAN for a non-finite residual. */
+int cdg_euler_residual_gpu(cdg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double *R);
 /* blocks (nnzb*n*n, column-major per block) = dR/dq + mass/dtau (cfl > 0);
  * R (may be NULL) receives R(q). */
 int cdg_euler_jacobian(int32_t nx, int32_t ny, int32_t nz, int32_t order, const double *q, double cfl,
