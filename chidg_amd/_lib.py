@@ -32,7 +32,7 @@ EXPORTS = [
     "cdg_server_unregister",
     "cdg_fgmres_solve", "cdg_fgmres_solve_multi", "cdg_fgmres_default_params",
     "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian", "cdg_euler_residual_gpu",
-    "cdg_vec_axpby", "cdg_vec_dot", "cdg_vec_norm",
+    "cdg_vec_create", "cdg_vec_destroy", "cdg_vec_copy", "cdg_vec_axpby", "cdg_vec_dot", "cdg_vec_norm",
 ]
 
 
@@ -141,6 +141,9 @@ def load() -> C.CDLL:
         "cdg_euler_state": ([C.c_int32] * 4 + [C.c_double, C.c_uint64, C.c_double, dp], C.c_int),
         "cdg_euler_residual": ([C.c_int32] * 4 + [dp, dp, C.c_int32], C.c_int),
         "cdg_euler_residual_gpu": ([vp] + [C.c_int32] * 4 + [dp, dp], C.c_int),
+        "cdg_vec_create": ([vp, C.c_int64, C.POINTER(C.c_void_p)], C.c_int),
+        "cdg_vec_destroy": ([vp, dp], C.c_int),
+        "cdg_vec_copy": ([vp, C.c_int64, dp, dp], C.c_int),
         "cdg_vec_axpby": ([vp, C.c_int64, C.c_double, dp, C.c_double, dp], C.c_int),
         "cdg_vec_dot": ([vp, C.c_int64, dp, dp, C.POINTER(C.c_double)], C.c_int),
         "cdg_vec_norm": ([vp, C.c_int64, dp, C.POINTER(C.c_double)], C.c_int),

@@ -1883,6 +1883,34 @@ extern "C" int cdg_prec_apply_multi(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A,
 }
 
 // ---- device-resident vectors (chidg_vector_t operations, type_chidg_vector.f90) ----
+extern "C" int cdg_vec_create(cdg_ctx *ctx, int64_t len, double **out) {
+    if (!ctx || !out || len < 0) return CDG_ERR_ARG;
+    *out = nullptr;
+    CDG_HIP_TRY(hipSetDevice(ctx->device));
+    void *p = nullptr;
+    int e = dev_alloc(ctx, &p, sizeof(double) * (size_t)std::max<int64_t>(len, 1));
+    if (!e) e = kv_zero(ctx, len, static_cast<double *>(p));
+    if (!e) e = ctx_wait(ctx);
+    if (e) { if (p) dev_free(ctx, p, 0); return e; }
+    *out = static_cast<double *>(p);
+    return CDG_OK;
+}
+
+extern "C" int cdg_vec_destroy(cdg_ctx *ctx, double *v) {
+    if (!ctx) return CDG_ERR_ARG;
+    if (v) {
+        CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        dev_free(ctx, v, 0);
+    }
+    return CDG_OK;
+}
+
+extern "C" int cdg_vec_copy(cdg_ctx *ctx, int64_t len, const double *src, double *dst) {
+    if (!ctx || !src || !dst || len < 0) return CDG_ERR_ARG;
+    CDG_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)len, hipMemcpyDefault, ctx->stream));
+    return ctx_wait(ctx);
+}
+
 extern "C" int cdg_vec_axpby(cdg_ctx *ctx, int64_t len, double a, const double *x, double b, double *y) {
     if (!ctx || !x || !y || len < 0) return CDG_ERR_ARG;
     if (!is_device_ptr(x) || !is_device_ptr(y)) return cdg_fail(ctx, CDG_ERR_ARG, "cdg_vec_axpby needs device pointers");

@@ -1,6 +1,7 @@
 #!/bin/bash
 # Builds the Fortran binding (module + object) with amdflang and links the
-# known-answer test program against the in-tree libchidg_hip.so.
+# known-answer and device-vector Newton test programs against the in-tree
+# libchidg_hip.so.
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 ROOT="$(cd "$HERE/../.." && pwd)"
@@ -14,3 +15,6 @@ mkdir -p "$OUT"
 "$FC" -O2 -I "$OUT" "$ROOT/tests/fortran/known_answers.f90" "$OUT/mod_chidg_hip.o" \
     -L "$ROOT/chidg_amd" -lchidg_hip -Wl,-rpath,'$ORIGIN/../../chidg_amd' \
     -o "$ROOT/tests/fortran/known_answers"
+"$FC" -O2 -I "$OUT" "$ROOT/tests/fortran/newton_device.f90" "$OUT/mod_chidg_hip.o" \
+    -L "$ROOT/chidg_amd" -lchidg_hip -Wl,-rpath,'$ORIGIN/../../chidg_amd' \
+    -o "$ROOT/tests/fortran/newton_device"

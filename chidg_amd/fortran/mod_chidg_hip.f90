@@ -7,6 +7,10 @@
 !!   hip_precon_t   <- precon_ILU0_t / precon_identity_t
 !!                     init/update/apply/tear_down (src/preconditioners/precon_ILU0.f90:64-470)
 !!   hip_fgmres_t   <- fgmres_t with solve(A,x,b,M) (src/linear_solvers/type_fgmres.f90:81-310)
+!!   hip_vector_t   <- chidg_vector_t with backend 'hip': values resident on the
+!!                     device between calls, with the vector operations a
+!!                     Newton loop needs (src/containers/type_chidg_vector.f90:
+!!                     norm :1237-1300, the overloaded +, -, * :1740-2080)
 !! Blocks are the reference's densematrix_t%mat: n x n column-major real(8);
 !! vectors are contiguous element-major real(8) arrays.  Every call aborts
 !! through hip_check on a non-zero status, the way chidg_signal(FATAL, ...)
@@ -23,7 +27,7 @@ module mod_chidg_hip
     ! ILU(0) apply schedules (cdg_prec_set_schedule); AUTO is the default
     integer(c_int32_t), parameter, public :: HIP_SCHEDULE_LEVELS = 0, HIP_SCHEDULE_DATAFLOW = 1, &
                                              HIP_SCHEDULE_FOLDED = 2, HIP_SCHEDULE_AUTO = 3, &
-                                             HIP_SCHEDULE_STREAM = 4
+                                             HIP_SCHEDULE_STREAM = 4, HIP_SCHEDULE_ROWSLICE = 5
 
     ! cdg_fgmres_params / cdg_fgmres_stats (same layout as the C structs)
     type, bind(c), public :: hip_fgmres_params_t
@@ -156,6 +160,45 @@ module mod_chidg_hip
             type(c_ptr), value :: mat
             integer(c_int64_t) :: stamp
         end function
+        integer(c_int) function cdg_vec_create(ctx, len, v) bind(c, name='cdg_vec_create')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: len
+            type(c_ptr) :: v
+        end function
+        integer(c_int) function cdg_vec_destroy(ctx, v) bind(c, name='cdg_vec_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, v
+        end function
+        integer(c_int) function cdg_vec_copy(ctx, len, src, dst) bind(c, name='cdg_vec_copy')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx, src, dst
+            integer(c_int64_t), value :: len
+        end function
+        integer(c_int) function cdg_vec_axpby(ctx, len, a, x, b, y) bind(c, name='cdg_vec_axpby')
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx, x, y
+            integer(c_int64_t), value :: len
+            real(c_double), value :: a, b
+        end function
+        integer(c_int) function cdg_vec_dot(ctx, len, x, y, out) bind(c, name='cdg_vec_dot')
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx, x, y
+            integer(c_int64_t), value :: len
+            real(c_double) :: out
+        end function
+        integer(c_int) function cdg_vec_norm(ctx, len, x, out) bind(c, name='cdg_vec_norm')
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx, x
+            integer(c_int64_t), value :: len
+            real(c_double) :: out
+        end function
+        integer(c_int) function cdg_euler_residual_gpu(ctx, nx, ny, nz, order, q, R) &
+                bind(c, name='cdg_euler_residual_gpu')
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx, q, R
+            integer(c_int32_t), value :: nx, ny, nz, order
+        end function
         integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -242,9 +285,27 @@ module mod_chidg_hip
     contains
         procedure :: init => fgmres_init
         procedure :: solve => fgmres_solve
+        procedure :: solve_dev => fgmres_solve_dev   ! x, b resident on the device
     end type
 
-    public :: hip_mv, hip_check
+    !> chidg_vector_t with backend 'hip': nrows*n values on the device, in the
+    !! element-major order of get_packed/set_packed.  set/get move values
+    !! between host and device; the operations run on the device.
+    type, public :: hip_vector_t
+        type(c_ptr) :: dev = c_null_ptr
+        type(c_ptr) :: ctx = c_null_ptr
+        integer(c_int64_t) :: len = 0
+    contains
+        procedure :: init => vec_init
+        procedure :: set => vec_set           ! host values in
+        procedure :: get => vec_get           ! device values out
+        procedure :: axpby => vec_axpby       ! self = a x + b self
+        procedure :: norm => vec_norm         ! chidg_vector_t%norm (one rank)
+        procedure :: dot => vec_dot           ! dot(self, x) (operator_chidg_dot.f90:30-89)
+        procedure :: release => vec_release
+    end type
+
+    public :: hip_mv, hip_check, hip_euler_residual
     public :: hip_pattern_from_rows, hip_pack_blocks, hip_get_packed, hip_set_packed
 
 contains
@@ -435,6 +496,90 @@ contains
         self%niter = self%stats%niter
         self%residual_norm = self%stats%residual_norm
         if (self%stats%factored /= 0) M%stamp = A%stamp()
+    end subroutine
+
+    !> solve(A,x,b,M[,solver_controller]) with x and b on the device.
+    subroutine fgmres_solve_dev(self, A, x, b, M, solver_controller)
+        class(hip_fgmres_t), intent(inout) :: self
+        type(hip_matrix_t), intent(in) :: A
+        type(hip_vector_t), intent(inout) :: x
+        type(hip_vector_t), intent(in) :: b
+        type(hip_precon_t), intent(inout) :: M
+        type(hip_solver_controller_t), intent(inout), optional :: solver_controller
+        real(c_double), pointer, contiguous :: xp(:), bp(:)
+        ! the device addresses, passed through as they are (never dereferenced here)
+        call c_f_pointer(x%dev, xp, [x%len])
+        call c_f_pointer(b%dev, bp, [b%len])
+        if (.not. M%initialized) call M%init(A)
+        self%params%update_preconditioner = 1
+        if (present(solver_controller)) then
+            self%params%update_preconditioner = 0
+            if (solver_controller%update_preconditioner(A, M)) self%params%update_preconditioner = 1
+        end if
+        call hip_check(cdg_fgmres_solve(A%ctx, A%h, M%h, bp, xp, self%params, self%stats), 'cdg_fgmres_solve', A%ctx)
+        self%niter = self%stats%niter
+        self%residual_norm = self%stats%residual_norm
+        if (self%stats%factored /= 0) M%stamp = A%stamp()
+    end subroutine
+
+    subroutine vec_init(self, ctx, len)
+        class(hip_vector_t), intent(inout) :: self
+        type(hip_context_t), intent(in) :: ctx
+        integer, intent(in) :: len
+        self%ctx = ctx%h
+        self%len = int(len, c_int64_t)
+        call hip_check(cdg_vec_create(self%ctx, self%len, self%dev), 'cdg_vec_create', self%ctx)
+    end subroutine
+
+    subroutine vec_set(self, x)
+        class(hip_vector_t), intent(inout) :: self
+        real(c_double), intent(in), target, contiguous :: x(:)
+        call hip_check(cdg_vec_copy(self%ctx, self%len, c_loc(x), self%dev), 'cdg_vec_copy', self%ctx)
+    end subroutine
+
+    subroutine vec_get(self, x)
+        class(hip_vector_t), intent(in) :: self
+        real(c_double), intent(inout), target, contiguous :: x(:)
+        call hip_check(cdg_vec_copy(self%ctx, self%len, self%dev, c_loc(x)), 'cdg_vec_copy', self%ctx)
+    end subroutine
+
+    subroutine vec_axpby(self, a, x, b)
+        class(hip_vector_t), intent(inout) :: self
+        real(c_double), intent(in) :: a, b
+        type(hip_vector_t), intent(in) :: x
+        call hip_check(cdg_vec_axpby(self%ctx, self%len, a, x%dev, b, self%dev), 'cdg_vec_axpby', self%ctx)
+    end subroutine
+
+    function vec_norm(self) result(r)
+        class(hip_vector_t), intent(in) :: self
+        real(c_double) :: r
+        call hip_check(cdg_vec_norm(self%ctx, self%len, self%dev, r), 'cdg_vec_norm', self%ctx)
+    end function
+
+    function vec_dot(self, x) result(r)
+        class(hip_vector_t), intent(in) :: self
+        type(hip_vector_t), intent(in) :: x
+        real(c_double) :: r
+        call hip_check(cdg_vec_dot(self%ctx, self%len, self%dev, x%dev, r), 'cdg_vec_dot', self%ctx)
+    end function
+
+    subroutine vec_release(self)
+        class(hip_vector_t), intent(inout) :: self
+        integer(c_int) :: st
+        if (c_associated(self%dev)) st = cdg_vec_destroy(self%ctx, self%dev)
+        self%dev = c_null_ptr
+    end subroutine
+
+    !> R(q) of the DG Euler discretization on the periodic N^3 box, degree
+    !! order, on the device (cdg_euler_residual_gpu; the reference assembles
+    !! its residual in mod_spatial.f90:40-197).
+    subroutine hip_euler_residual(ctx, N, order, q, R)
+        type(hip_context_t), intent(in) :: ctx
+        integer, intent(in) :: N, order
+        type(hip_vector_t), intent(in) :: q
+        type(hip_vector_t), intent(inout) :: R
+        call hip_check(cdg_euler_residual_gpu(ctx%h, int(N, c_int32_t), int(N, c_int32_t), int(N, c_int32_t), &
+                                              int(order, c_int32_t), q%dev, R%dev), 'cdg_euler_residual_gpu', ctx%h)
     end subroutine
 
     !> Default: always update the lhs (type_solver_controller.f90:58-74).

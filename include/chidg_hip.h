@@ -332,6 +332,12 @@ int cdg_server_destroy(cdg_server *s);
  * (:1237-1300); operator_chidg_dot.f90:30-89 dot), so a Newton loop keeps q,
  * dq and rhs in HBM.  cdg_vec_axpby: y = a x + b y (device pointers);
  * cdg_vec_dot / cdg_vec_norm: host or device vectors, result on the host. */
+/* cdg_vec_create: a zeroed device vector of len doubles from the context's
+ * memory (cdg_vec_destroy returns it); cdg_vec_copy: len doubles between any
+ * host / device pointers (upload, download, device copy), synchronous. */
+int cdg_vec_create(cdg_ctx *ctx, int64_t len, double **out);
+int cdg_vec_destroy(cdg_ctx *ctx, double *v);
+int cdg_vec_copy(cdg_ctx *ctx, int64_t len, const double *src, double *dst);
 int cdg_vec_axpby(cdg_ctx *ctx, int64_t len, double a, const double *x, double b, double *y);
 int cdg_vec_dot(cdg_ctx *ctx, int64_t len, const double *x, const double *y, double *out);
 int cdg_vec_norm(cdg_ctx *ctx, int64_t len, const double *x, double *out);

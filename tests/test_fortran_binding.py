@@ -34,3 +34,19 @@ def test_fortran_known_answers_on_gpu():
                  "mem_used", "bridge_pattern", "bridge_pack", "bridge_get_packed", "bridge_mv", "bridge_set_packed",
                  "controller_refactor_rule", "controller_solution", "fgmres_3x3", "fgmres_2x2_skew", "mv"):
         assert f"PASS {name}" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_newton_with_device_vectors():
+    """Newton steps with q, dq and rhs resident on the device through the
+    Fortran binding (hip_vector_t, hip_euler_residual, solve_dev) equal the
+    same steps with host arrays."""
+    exe = os.path.join(ROOT, "tests", "fortran", "newton_device")
+    assert os.path.exists(exe), "build the Fortran test programs first (chidg_amd/fortran/build.sh)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASS" in r.stdout
+    for name in ("newton_device_residual_norms", "newton_device_residual_falls", "newton_device_state",
+                 "device_vector_dot"):
+        assert f"PASS {name}" in r.stdout
