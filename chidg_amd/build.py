@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchidg_hip.so")
 SOURCES = ["cdg_api.hip", "k_stream.hip", "k_dfapply.hip", "k_rsapply.hip", "k_factor.hip", "k_krylov.hip",
-           "k_synth.hip", "k_multi.hip", "k_euler.hip", "cdg_serve.hip", "cdg_noalloc.hip", "euler_jac.cpp"]
+           "k_synth.hip", "k_multi.hip", "k_euler.hip", "cdg_batch.hip", "cdg_serve.hip", "cdg_noalloc.hip", "euler_jac.cpp"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
          "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1032,7 +1032,7 @@ extern "C" int cdg_prec_update(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int3
 
 // The stream schedule's storage for the current factor: W = Dinv U and the
 // row-slice copies of L and Dinv (once per factor).
-static int rs_prepare(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
+int rs_prepare(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
     if (M->rs_ready) return CDG_OK;
     int e = rs_setup(ctx, M);
     if (!e) e = launch_fold_rs(ctx, M, A);

@@ -191,8 +191,12 @@ int rs_ctl_words();
 int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z);
 int launch_ilu_apply_rsl_buf(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y);
 int noalloc_init(int device);
-int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y, int64_t stride, int k);
-int launch_mv_multi(cdg_ctx *ctx, const cdg_mat *A, const double *X, double *Y, int64_t stride, int k);
+// W = Dinv U and the row-slice copies of M's current factor (once per factor)
+int rs_prepare(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A);
+int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y, int64_t stride, int k,
+                              const int32_t *row_k = nullptr, int row_lo = 0, int row_hi = -1);
+int launch_mv_multi(cdg_ctx *ctx, const cdg_mat *A, const double *X, double *Y, int64_t stride, int k,
+                    const int32_t *row_k = nullptr, int row_lo = 0, int row_hi = -1);
 bool is_device_ptr(const void *p);
 int rs_slice_rows(int n, int wmax);
 void rs_ctx_count(int delta);

@@ -19,6 +19,7 @@
 #include "cdg_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 namespace {
@@ -35,10 +36,12 @@ template <int K, int U>
 __global__ void __launch_bounds__(kMvMaxThreads)
 k_mv_multi(const double *__restrict__ blocks, int64_t ld, const int32_t *__restrict__ row_ptr,
            const int32_t *__restrict__ col, const double *__restrict__ X, double *__restrict__ Y, int64_t stride,
-           int k, int n, int dc, int chunk) {
+           int k, int n, int dc, int chunk, const int32_t *__restrict__ row_k, int r0, int row0) {
     extern __shared__ double smem[];
     constexpr int V = 2;
-    const int T = blockDim.x, tid = threadIdx.x, row = blockIdx.x;
+    const int T = blockDim.x, tid = threadIdx.x, row = row0 + (int)blockIdx.x;
+    if (row_k) k = min(max(row_k[row] - r0, 0), K);   // the row's system has row_k[row] right-hand sides
+    if (k == 0) return;
     const int nn = n * n;
     double *xs = smem;                            // chunk * K * n: [block][rhs][c]
     double *red = smem + (size_t)chunk * K * n;   // V * T
@@ -112,12 +115,18 @@ k_mv_multi(const double *__restrict__ blocks, int64_t ld, const int32_t *__restr
 }  // namespace
 
 // Y_r = A X_r for r < k (X, Y: k vectors `stride` elements apart, device).
-int launch_mv_multi(cdg_ctx *ctx, const cdg_mat *A, const double *X, double *Y, int64_t stride, int k) {
-    if (A->nrows == 0 || k <= 0) return CDG_OK;
+// row_k (may be null): per row, the right-hand sides of the row's system
+// (<= k), for a batch of systems joined in one matrix.  row_hi >= 0: only
+// the rows row_lo <= i < row_hi.
+int launch_mv_multi(cdg_ctx *ctx, const cdg_mat *A, const double *X, double *Y, int64_t stride, int k,
+                    const int32_t *row_k, int row_lo, int row_hi) {
+    if (row_hi < 0) { row_lo = 0; row_hi = A->nrows; }
+    if (row_hi <= row_lo || k <= 0) return CDG_OK;
     const StreamShape &s = A->shape64;
     if (s.T == 0) return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "block size %d not supported", A->n);
-    for (int r0 = 0; r0 < k; r0 += 8) {
-        const int kk = std::min(8, k - r0);
+    static const int cap = std::getenv("CDG_MV_MULTI_WIDTH") ? std::max(1, std::min(8, std::atoi(std::getenv("CDG_MV_MULTI_WIDTH")))) : 8;
+    for (int r0 = 0; r0 < k; r0 += cap) {
+        const int kk = std::min(cap, k - r0);
         const int K = kk <= 1 ? 1 : kk <= 2 ? 2 : kk <= 4 ? 4 : 8;
         const int chunk = std::max(1, std::min<int>(A->max_row_blocks, (32 * 1024) / (8 * A->n * K)));
         const size_t lds = sizeof(double) * ((size_t)chunk * K * A->n + 2 * (size_t)s.T);
@@ -125,8 +134,9 @@ int launch_mv_multi(cdg_ctx *ctx, const cdg_mat *A, const double *X, double *Y, 
         double *y = Y + (size_t)r0 * stride;
         // loads in flight per thread: fewer as K grows (the products per load
         // grow with K; 16-byte loads x K accumulators must fit 128 VGPRs)
-#define CDG_MVM(KK, UU) hipLaunchKernelGGL((k_mv_multi<KK, UU>), dim3(A->nrows), dim3(s.T), lds, ctx->stream, \
-                                           A->blocks, A->ld, A->d_row_ptr, A->d_col, x, y, stride, kk, A->n, s.dc, chunk)
+#define CDG_MVM(KK, UU) hipLaunchKernelGGL((k_mv_multi<KK, UU>), dim3(row_hi - row_lo), dim3(s.T), lds, ctx->stream, \
+                                           A->blocks, A->ld, A->d_row_ptr, A->d_col, x, y, stride, kk, A->n, s.dc, chunk, \
+                                           row_k, r0, row_lo)
         if (K == 1) CDG_MVM(1, 4);
         else if (K == 2) CDG_MVM(2, 4);
         else if (K == 4) CDG_MVM(4, 2);

@@ -564,6 +564,9 @@ struct RsmArgs {
     int64_t stride;
     int32_t k;                // right-hand sides of this launch (<= K)
     int32_t nbmax;            // blocks per task at most (parents per task)
+    const int32_t *row_k;     // null, or right-hand sides of each row's system (a batch of
+                              // systems in one matrix): this launch serves r0 .. r0+K-1
+    int32_t r0;
 };
 
 template <typename S, int K, int R>
@@ -575,7 +578,7 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
     constexpr int V = RW::V, NB = 4;
     const S *const Sb = static_cast<const S *>(args.Sb);
     const int64_t slot = args.slot, stride = args.stride;
-    const int SL = args.SL, n = args.n, w = args.w, dc = args.dc, k = args.k, nbmax = args.nbmax;
+    const int SL = args.SL, n = args.n, w = args.w, dc = args.dc, nbmax = args.nbmax;
     VT *const y = static_cast<VT *>(args.y);
     double *const zout = args.z;
     extern __shared__ unsigned char smem_raw[];
@@ -587,6 +590,8 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
     const int lim = w * n, step = V * T, lim_ld = (lim - 1) / V * V;
     const int32_t *rec = args.tasks + (size_t)(args.t0 + (int)blockIdx.x) * kRsRec;
     const int row = rec[0];
+    const int k = args.row_k ? min(max(args.row_k[row] - args.r0, 0), K) : args.k;
+    if (k == 0) return;
     const int sl = rec[1] & 0xff, nb = (rec[1] >> 8) & 0xff, bwd = (rec[1] >> 16) & 1;
     // the block slices first (they do not depend on the parents), then the
     // parents of all k right-hand sides into LDS
@@ -1064,7 +1069,9 @@ int launch_ilu_apply_rsl(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z) 
 int rs_multi_width(const cdg_prec *M) {
     const size_t accsz = M->precision == CDG_FP64 ? 8 : 4;
     const size_t per = (size_t)(M->maxdep + 1) * M->n * accsz;
-    const int fit = (int)std::max<size_t>(1, (48 * 1024) / std::max<size_t>(per, 1));
+    int fit = (int)std::max<size_t>(1, (48 * 1024) / std::max<size_t>(per, 1));
+    static const int cap = std::getenv("CDG_RS_MULTI_WIDTH") ? std::max(1, std::atoi(std::getenv("CDG_RS_MULTI_WIDTH"))) : 8;
+    fit = std::min(fit, cap);
     return fit >= 8 ? 8 : fit >= 4 ? 4 : fit >= 2 ? 2 : 1;
 }
 
@@ -1073,10 +1080,22 @@ int rs_multi_width(const cdg_prec *M) {
 // vectors of nrows * n values of the accumulation type `stride` apart.
 // The row-slice storage must be prepared (rs_prepare).  Groups of up to
 // rs_multi_width(M) right-hand sides share each block read; a single vector
-// runs the single-vector kernel.
-int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y, int64_t stride, int k) {
+// runs the single-vector kernel.  row_k (may be null): per row, the number of
+// right-hand sides of the row's system (<= k), for a batch of systems joined
+// in one matrix; vectors r >= row_k[row] of that row are left alone.
+// row_hi >= 0: only the rows row_lo <= i < row_hi (every level's rows are in
+// ascending order, so a row range is one task range per level; levels
+// without such rows launch nothing).
+int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double *z, void *y, int64_t stride, int k,
+                              const int32_t *row_k, int row_lo, int row_hi) {
     if (M->nrows == 0 || k <= 0) return CDG_OK;
     const int S = M->rs_S, nf = S * M->nrows;
+    const bool ranged = row_hi >= 0;
+    auto sub = [&](const std::vector<int32_t> &rows, int a, int b, int *lo, int *hi) {
+        if (!ranged) { *lo = a; *hi = b; return; }
+        *lo = (int)(std::lower_bound(rows.begin() + a, rows.begin() + b, row_lo) - rows.begin());
+        *hi = (int)(std::lower_bound(rows.begin() + a, rows.begin() + b, row_hi) - rows.begin());
+    };
     const int kw = rs_multi_width(M);
     const size_t vt = M->precision == CDG_FP64 ? 8 : 4;
     for (int r0 = 0; r0 < k; r0 += kw) {
@@ -1084,7 +1103,7 @@ int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double
         const double *vr = v + (size_t)r0 * stride;
         double *zr = z + (size_t)r0 * stride;
         void *yr = static_cast<char *>(y) + vt * (size_t)r0 * stride;
-        if (kk == 1) {
+        if (kk == 1 && !row_k && !ranged) {
             int e = launch_ilu_apply_rsl_buf(ctx, M, vr, zr, yr);
             if (e) return e;
             continue;
@@ -1104,6 +1123,8 @@ int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double
         a.stride = stride;
         a.k = kk;
         a.nbmax = M->maxdep + 1;
+        a.row_k = row_k;
+        a.r0 = r0;
         const void *fn = M->precision == CDG_FP64 ? rsm_kernel<double>(K, M->rs_R)
                        : (M->precision == CDG_FP32 ? rsm_kernel<float>(K, M->rs_R)
                                                    : rsm_kernel<cdg_bf16>(K, M->rs_R));
@@ -1119,11 +1140,15 @@ int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double
             return CDG_OK;
         };
         for (int l = 0; l < M->nlev; ++l) {
-            int e = run(S * M->lev_row_ptr[l], S * M->lev_row_ptr[l + 1]);
+            int lo, hi;
+            sub(M->h_lev_rows, M->lev_row_ptr[l], M->lev_row_ptr[l + 1], &lo, &hi);
+            int e = run(S * lo, S * hi);
             if (e) return e;
         }
         for (int l = 0; l < M->nblev; ++l) {
-            int e = run(nf + S * M->blev_row_ptr[l], nf + S * M->blev_row_ptr[l + 1]);
+            int lo, hi;
+            sub(M->h_blev_rows, M->blev_row_ptr[l], M->blev_row_ptr[l + 1], &lo, &hi);
+            int e = run(nf + S * lo, nf + S * hi);
             if (e) return e;
         }
     }

@@ -470,3 +470,55 @@ class solver_controller:
         self.force_update_preconditioner = False
         self.preconditioner_updated = bool(update)
         return self.preconditioner_updated
+
+
+class batch:
+    """Many independent systems solved together (``cdg_batch_*``): the
+    systems of each block size joined in one block-diagonal matrix with one
+    ILU(0), every right-hand side an FGMRES(nkrylov) run in lockstep with the
+    others, each cycle of Arnoldi steps one captured HIP graph.  ``mats`` are
+    stored ``chidg_matrix`` objects of one context; ``nrhs[s]`` right-hand
+    sides (1..8) for system s."""
+
+    def __init__(self, mats, nrhs, precision: str = "fp32", nkrylov: int = 30):
+        self.ctx = mats[0].ctx
+        self.mats = list(mats)
+        self.nrhs = [int(k) for k in nrhs]
+        handles = (C.c_void_p * len(mats))(*[A.h for A in mats])
+        ks = (C.c_int32 * len(mats))(*self.nrhs)
+        h = C.c_void_p()
+        prec = {"fp64": _lib.FP64, "fp32": _lib.FP32, "bf16": _lib.BF16}[precision]
+        self.ctx._check(self.ctx.lib.cdg_batch_create(self.ctx.h, len(mats), handles, ks, prec, nkrylov, C.byref(h)),
+                        "cdg_batch_create")
+        self.h = h
+        self.nkrylov = nkrylov
+
+    def solve(self, B, X, tol=0.0, rtol=1e-8, nmax=-1):
+        """B[s], X[s]: (nrhs[s], nrows*n) arrays or device tensors; X overwritten.
+        Returns one cdg_fgmres_stats per right-hand side, systems in order."""
+        ls = fgmres(nkrylov=self.nkrylov, tol=tol, rtol=rtol, nmax=nmax, inner_fgmres=False)
+        prm = ls.params(update_preconditioner=False)
+        nl = sum(self.nrhs)
+        st = (_lib.FgmresStats * nl)()
+        _after_torch(self.ctx, *B, *X)
+        bp = (C.c_void_p * len(B))(*[_ptr(b) for b in B])
+        xp = (C.c_void_p * len(X))(*[_ptr(x) for x in X])
+        self.ctx._check(self.ctx.lib.cdg_batch_solve(self.ctx.h, self.h, bp, xp, C.byref(prm), st), "cdg_batch_solve")
+        _before_torch(self.ctx)
+        return list(st)
+
+    def info(self) -> dict:
+        nl, ng, nv = C.c_int32(), C.c_int32(), C.c_int64()
+        self.ctx._check(self.ctx.lib.cdg_batch_info(self.h, C.byref(nl), C.byref(ng), C.byref(nv)), "cdg_batch_info")
+        return dict(lanes=nl.value, groups=ng.value, values=nv.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.cdg_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

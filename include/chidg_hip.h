@@ -264,6 +264,36 @@ int cdg_fgmres_solve_multi(cdg_ctx *ctx, const cdg_mat *A, cdg_prec *M, int32_t 
 /* Reference defaults of linear_solver_t (type_linear_solver.f90:21-40). */
 void cdg_fgmres_default_params(cdg_fgmres_params *params);
 
+/* ---- batch of independent systems --------------------------------------------
+ * No reference counterpart (ChiDG solves one system per MPI rank): many
+ * systems of uneven sizes and block sizes, nrhs[s] (1..8) right-hand sides
+ * each, solved together on one GPU: per right-hand side the outer FGMRES of
+ * fgmres_t%solve (type_fgmres.f90:81-310; CGS, inner FGMRES off, nkrylov <=
+ * 31), all Arnoldi steps advancing in lockstep.  cdg_batch_create joins the
+ * systems of each block size into one block-diagonal matrix (copies of the
+ * blocks of A[s], which must be complete), computes its ILU(0) in `precision`
+ * (each system's own factor: one factor launch sequence per block size) and
+ * allocates the Krylov space of nkrylov steps.  cdg_batch_solve: b[s] and
+ * x[s] hold nrhs[s] vectors of system s one after another (host or device);
+ * stats (may be NULL) receives one entry per right-hand side, systems in
+ * order.  Each Arnoldi step is, per block size, k-vector applies and
+ * matvecs over row ranges of systems with like right-hand-side counts (1, 2,
+ * 3-4, 5-8 vectors per launch; rows of systems whose lanes have all stopped
+ * are skipped), segmented vector kernels over all right-hand sides and one
+ * rotation kernel on the device; a cycle of nkrylov steps runs as one
+ * captured HIP graph (CDG_BATCH_GRAPH=0: the same launches without the
+ * graph; CDG_BATCH_TRACE=1: phase times on stderr).  Each right-hand side
+ * gets the iterations of its own solve (the same stopping test, restarts at
+ * its own cycle ends).  A solve whose lanes never meet the stopping test runs
+ * until nmax (nmax <= 0: without bound, as the reference). */
+typedef struct cdg_batch cdg_batch;
+int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, const int32_t *nrhs, int32_t precision,
+                     int32_t nkrylov, cdg_batch **out);
+int cdg_batch_solve(cdg_ctx *ctx, cdg_batch *batch, const double *const *b, double *const *x,
+                    const cdg_fgmres_params *params, cdg_fgmres_stats *stats);
+int cdg_batch_info(const cdg_batch *batch, int32_t *nlanes, int32_t *ngroups, int64_t *values);
+int cdg_batch_destroy(cdg_ctx *ctx, cdg_batch *batch);
+
 /* ---- request loop for parameter-sweep solves ----------------------------------
  * No reference counterpart (ChiDG solves one system per MPI rank); for callers
  * that solve many independent systems on one GPU.  A server runs `nslots`
