@@ -56,9 +56,14 @@ k_mv_multi(const double *__restrict__ blocks, int64_t ld, const int32_t *__restr
     for (int cb = b0; cb < b1; cb += chunk) {
         const int nb = min(chunk, b1 - cb);
         if (cb != b0) __syncthreads();
-        for (int idx = tid; idx < nb * K * n; idx += T) {
-            const int kb = idx / (K * n), rem = idx - kb * K * n, r = rem / n, c = rem - r * n;
-            xs[idx] = r < k ? X[(size_t)r * stride + (size_t)col[cb + kb] * n + c] : 0.0;
+        for (int idx = tid; idx < nb * n; idx += T) {   // each parent value's k loads together
+            const int kb = idx / n, c = idx - kb * n;
+            const size_t gb = (size_t)col[cb + kb] * n + c;
+            double xv[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r) xv[r] = r < k ? X[(size_t)r * stride + gb] : 0.0;
+#pragma unroll
+            for (int r = 0; r < K; ++r) xs[(size_t)kb * K * n + r * n + c] = xv[r];
         }
         __syncthreads();
         for (int kb = 0; kb < nb; ++kb) {

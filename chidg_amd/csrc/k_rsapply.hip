@@ -569,8 +569,13 @@ struct RsmArgs {
     int32_t r0;
 };
 
+// FP64/FP32 with K = 8 and R <= 4 need a little over 128 VGPRs unconstrained,
+// which leaves one 512-thread workgroup per CU; held to 128 two fit, and
+// one's loads overlap the other's products and row sums (BF16 would spill).
 template <typename S, int K, int R>
-__global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
+__global__ void __launch_bounds__(kRsMaxThreads)
+__attribute__((amdgpu_waves_per_eu((K >= 8 && R <= 4 && sizeof(S) >= 4) ? 4 : 1)))
+k_rs_multi(RsmArgs args) {
     typedef Raw16<S> RW;
     typedef typename RW::Acc Acc;
     typedef typename RW::T RawT;
@@ -583,10 +588,11 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
     double *const zout = args.z;
     extern __shared__ unsigned char smem_raw[];
     const int T = blockDim.x, tid = threadIdx.x;
+    constexpr int R2 = K < 4 ? K : (V >= 8 ? 2 : 4);                 // right-hand sides per row-sum round
     const int xsz = nbmax * n;                                       // parents of one right-hand side
     Acc *xs = reinterpret_cast<Acc *>(smem_raw);                     // K * xsz
-    Acc *red = xs + (((size_t)K * xsz + 3) & ~(size_t)3);            // V*T
-    Acc *red2 = red + (size_t)V * T;                                 // T
+    Acc *red = xs;                                                   // R2 * V*T, over xs once the products are done
+    Acc *red2 = xs + max((((size_t)K * xsz + 3) & ~(size_t)3), (size_t)R2 * V * T);   // R2 * T
     const int lim = w * n, step = V * T, lim_ld = (lim - 1) / V * V;
     const int32_t *rec = args.tasks + (size_t)(args.t0 + (int)blockIdx.x) * kRsRec;
     const int row = rec[0];
@@ -605,13 +611,20 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
             raw[kb][m] = __builtin_nontemporal_load(
                 reinterpret_cast<const RawT *>(b + (kb < nb ? min(V * tid + m * step, lim_ld) : 0)));
     }
+    // each parent value's offset once, then its k loads together (no load
+    // waits on the previous one's LDS store)
     const int ne = nb * n;
-    for (int j = tid; j < k * ne; j += T) {
-        const int r = j / ne, jj = j - r * ne;
+    for (int jj = tid; jj < ne; jj += T) {
         const int kb = jj / n, c = jj - kb * n;
-        const size_t gi = (size_t)r * stride + (size_t)rec[2 + kRsMaxBlocks + kb] * n + c;
-        const VT x = (bwd && jj >= n) ? static_cast<VT>(zout[gi]) : y[gi];
-        xs[r * xsz + jj] = (bwd && jj < n) ? static_cast<Acc>(x) : -static_cast<Acc>(x);
+        const size_t gb = (size_t)rec[2 + kRsMaxBlocks + kb] * n + c;
+        const bool fromz = bwd && jj >= n, own = bwd && jj < n;
+        VT x[K];
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+            if (r < k) x[r] = fromz ? static_cast<VT>(zout[(size_t)r * stride + gb]) : y[(size_t)r * stride + gb];
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+            if (r < k) xs[r * xsz + jj] = own ? static_cast<Acc>(x[r]) : -static_cast<Acc>(x[r]);
     }
     __syncthreads();
     int col0[V];
@@ -664,20 +677,46 @@ __global__ void __launch_bounds__(kRsMaxThreads) k_rs_multi(RsmArgs args) {
             }
         }
     }
+    // row sums, R2 right-hand sides per round (each in rs_rowsum's order)
+    const int G = max(1, min(T / w, dc));
 #pragma unroll
-    for (int r = 0; r < K; ++r) {
-        if (r >= k) break;
-        __syncthreads();   // the previous right-hand side's row sums are done with red and red2
+    for (int r0 = 0; r0 < K; r0 += R2) {
+        if (r0 >= k) break;
+        __syncthreads();   // the products are done with xs; the previous round with red and red2
 #pragma unroll
-        for (int v = 0; v < V; ++v) red[V * tid + v] = acc[r][v];
+        for (int q = 0; q < R2; ++q)
+            if (r0 + q < k)
+#pragma unroll
+                for (int v = 0; v < V; ++v) red[(size_t)q * step + V * tid + v] = acc[r0 + q][v];
         __syncthreads();
-        const Acc s = rs_rowsum(red, red2, w, dc, T, tid);
+        if (tid < G * w) {
+            const int g = tid / w, rr = tid - g * w;
+#pragma unroll
+            for (int q = 0; q < R2; ++q) {
+                if (r0 + q < k) {
+                    const Acc *rq = red + (size_t)q * step;
+                    Acc s = 0;
+                    for (int p = g; p < dc; p += G) s += rq[rr + w * p];
+                    red2[q * T + g * w + rr] = s;
+                }
+            }
+        }
+        __syncthreads();
         if (tid < w) {
             const size_t gi = (size_t)row * n + (size_t)sl * w + tid;
-            if (!bwd)
-                y[(size_t)r * stride + gi] = static_cast<VT>(args.v[(size_t)r * stride + gi] + static_cast<double>(s));
-            else
-                zout[(size_t)r * stride + gi] = static_cast<double>(static_cast<VT>(s));
+#pragma unroll
+            for (int q = 0; q < R2; ++q) {
+                const int r = r0 + q;
+                if (r < k) {
+                    Acc s = 0;
+                    for (int g = 0; g < G; ++g) s += red2[q * T + g * w + tid];
+                    if (!bwd)
+                        y[(size_t)r * stride + gi] =
+                            static_cast<VT>(args.v[(size_t)r * stride + gi] + static_cast<double>(s));
+                    else
+                        zout[(size_t)r * stride + gi] = static_cast<double>(static_cast<VT>(s));
+                }
+            }
         }
     }
 }
@@ -1130,8 +1169,9 @@ int launch_ilu_apply_rs_multi(cdg_ctx *ctx, cdg_prec *M, const double *v, double
                                                    : rsm_kernel<cdg_bf16>(K, M->rs_R));
         const size_t accsz = M->precision == CDG_FP64 ? 8 : 4;
         const int V = M->precision == CDG_FP64 ? 2 : (M->precision == CDG_FP32 ? 4 : 8);
-        const size_t lds = accsz * ((((size_t)K * a.nbmax * M->n + 3) & ~(size_t)3) + (size_t)V * M->rs_T +
-                                    (size_t)M->rs_T);
+        const size_t R2 = K < 4 ? K : (V >= 8 ? 2 : 4);   // the kernel's row-sum round width
+        const size_t lds = accsz * (std::max((((size_t)K * a.nbmax * M->n + 3) & ~(size_t)3), R2 * V * M->rs_T) +
+                                    R2 * M->rs_T);
         auto run = [&](int lo, int hi) -> int {
             if (hi <= lo) return CDG_OK;
             a.t0 = lo;
