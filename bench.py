@@ -302,26 +302,35 @@ def pretune_small(args):
 
 def batch_run(args):
     """64 independent systems, meshes N^3 with N in 4..12, P1..P3 and 1..8
-    right-hand sides drawn uniformly (seeded), FP32 ILU(0), solved two ways:
-    all submitted at once to the request loop (6 slots by default), and as one
-    joined solve (the systems of each block size in one block-diagonal matrix,
-    one factor, every right-hand side a lockstep FGMRES lane, graph-captured
-    cycles).  Both time the whole job: matrix fill, factor, solve."""
+    right-hand sides drawn uniformly (seeded), FP32 ILU(0), solved two ways,
+    each timed over the whole job (matrix fill, factor, solve), cold (the
+    first batch: every allocation fresh) and warm (the same batch again on the
+    same objects):
+    * request loop: all systems submitted at once (6 slots by default); warm =
+      the same server again (its slots' device memory cached);
+    * joined: the systems of each block size in one block-diagonal matrix, one
+      factor, every right-hand side a lockstep FGMRES lane, per-step graphs;
+      warm = the matrices refilled, copied in and refactored (no allocation)."""
     from chidg_amd.serve import Server, batch, joined, sweep_requests
     rng = np.random.default_rng(20261202)
     reqs = sweep_requests(64, rng, meshes=(4, 12), orders=(1, 2, 3), nrhs=(1, 8))
-    with Server(nslots=args.batch_slots, nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False) as srv:
-        loop = batch(srv, reqs)
-    log(f"batch (request loop): {loop['systems']} systems / {loop['rhs']} rhs in {loop['wall_s']:.2f} s "
-        f"({loop['rhs_per_s']:.1f} rhs/s), peak {loop['peak_active']} concurrent")
-    joined(reqs[:4], rtol=1e-8)          # warm-up: kernels loaded, graph path exercised
-    jn = joined(reqs, rtol=1e-8)
-    log(f"batch (joined): {jn['rhs']} rhs in {jn['wall_s']:.2f} s ({jn['rhs_per_s']:.1f} rhs/s; fill "
-        f"{jn['t_fill_s']:.2f} s, join+factor {jn['t_join_factor_s']:.2f} s, solve {jn['t_solve_s']:.2f} s), "
-        f"max relative residual {jn['max_rel_residual']:.2e}")
     fmt = lambda d: {k: sig(float(v)) if isinstance(v, float) else v for k, v in d.items()}
-    out = fmt(jn)
-    out["request_loop"] = fmt(loop)
+    with Server(nslots=args.batch_slots, nkrylov=30, tol=0.0, rtol=1e-8, inner_fgmres=False) as srv:
+        loop = [batch(srv, reqs) for _ in range(2)]
+    for tag, d in zip(("cold", "warm"), loop):
+        log(f"batch (request loop, {tag}): {d['rhs']} rhs in {d['wall_s']:.2f} s ({d['rhs_per_s']:.1f} rhs/s), "
+            f"peak {d['peak_active']} concurrent")
+    joined(reqs[:4], rtol=1e-8, warm=False)          # kernels loaded, graph path exercised
+    jn = joined(reqs, rtol=1e-8)
+    jw = jn.pop("warm")
+    for tag, d in (("cold", jn), ("warm", jw)):
+        log(f"batch (joined, {tag}): {d['rhs']} rhs in {d['wall_s']:.2f} s ({d['rhs_per_s']:.1f} rhs/s; fill "
+            f"{d['t_fill_s']:.2f} s, join/update + factor {d['t_join_factor_s']:.2f} s, solve {d['t_solve_s']:.2f} s "
+            f"= {d['rhs_per_s_solve']:.0f} rhs/s), max relative residual {d['max_rel_residual']:.2e}")
+    out = {"request_loop": {"cold": fmt(loop[0]), "warm": fmt(loop[1])}, "joined": {"cold": fmt(jn), "warm": fmt(jw)}}
+    best = max((("request_loop", loop[1]["rhs_per_s"]), ("joined", jw["rhs_per_s"])), key=lambda t: t[1])
+    out["rhs_per_s_warm"] = sig(float(best[1]))
+    out["rhs_per_s_warm_path"] = best[0]
     return out
 
 

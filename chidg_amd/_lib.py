@@ -31,7 +31,7 @@ EXPORTS = [
     "cdg_server_last_error", "cdg_server_destroy", "cdg_server_register", "cdg_server_submit_system",
     "cdg_server_unregister",
     "cdg_fgmres_solve", "cdg_fgmres_solve_multi", "cdg_fgmres_default_params",
-    "cdg_batch_create", "cdg_batch_solve", "cdg_batch_info", "cdg_batch_destroy",
+    "cdg_batch_create", "cdg_batch_solve", "cdg_batch_update", "cdg_batch_info", "cdg_batch_destroy",
     "cdg_euler_state", "cdg_euler_residual", "cdg_euler_jacobian", "cdg_euler_residual_gpu",
     "cdg_vec_create", "cdg_vec_destroy", "cdg_vec_copy", "cdg_vec_axpby", "cdg_vec_dot", "cdg_vec_norm",
 ]
@@ -125,6 +125,7 @@ def load() -> C.CDLL:
                                    C.c_int),
         "cdg_batch_create": ([vp, C.c_int32, vp, vp, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
         "cdg_batch_solve": ([vp, vp, vp, vp, C.POINTER(FgmresParams), C.POINTER(FgmresStats)], C.c_int),
+        "cdg_batch_update": ([vp, vp, vp], C.c_int),
         "cdg_batch_info": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
         "cdg_batch_destroy": ([vp, vp], C.c_int),
         "cdg_fgmres_default_params": ([C.POINTER(FgmresParams)], None),

@@ -307,6 +307,8 @@ struct cdg_batch {
     int64_t Ltot = 0;
     std::vector<int64_t> off, len;     // per system: offset in a column, length
     std::vector<int> nrhs;
+    std::vector<int64_t> sys_nnzb;     // per system: blocks (for cdg_batch_update's checks)
+    std::vector<int> sys_nrows, sys_n;
     std::vector<BatchGroup> groups;
     // lanes and work items
     int nlanes = 0, nitems = 0;
@@ -321,10 +323,12 @@ struct cdg_batch {
     int32_t *live = nullptr, *nv = nullptr, *restart = nullptr;
     LaneState st{};
     int32_t *d_sys_lane0 = nullptr, *d_sys_nrhs = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    double graph_tol = -1.0, graph_rtol = -1.0;   // the stopping test the graph was captured with
+    std::vector<hipGraph_t> graphs;    // one captured graph per Arnoldi step
+    std::vector<hipGraphExec_t> execs;
+    double graph_tol = -1.0, graph_rtol = -1.0;   // the stopping test the graphs were captured with
     int graph_nmax = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int32_t *h_act = nullptr;          // pinned: three copies of the lanes' active flags (one per event)
 };
 
 namespace {
@@ -399,6 +403,15 @@ int batch_step(cdg_ctx *ctx, cdg_batch *bt, int j, double tol, double rtol, int 
 
 using bclk = std::chrono::steady_clock;
 
+void batch_drop_graphs(cdg_batch *bt) {
+    for (hipGraphExec_t x : bt->execs)
+        if (x) (void)hipGraphExecDestroy(x);
+    for (hipGraph_t g : bt->graphs)
+        if (g) (void)hipGraphDestroy(g);
+    bt->execs.clear();
+    bt->graphs.clear();
+}
+
 // CDG_BATCH_TRACE=1: phase times of create and solve on stderr
 bool btrace() {
     static const bool on = std::getenv("CDG_BATCH_TRACE") && std::atoi(std::getenv("CDG_BATCH_TRACE")) != 0;
@@ -429,6 +442,9 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
             return cdg_fail(ctx, CDG_ERR_UNSUPPORTED, "batch: system %d has a rank partition", s);
         }
         bt->K = std::max(bt->K, (int)nrhs[s]);
+        bt->sys_nnzb.push_back(A[s]->nnzb);
+        bt->sys_nrows.push_back(A[s]->nrows);
+        bt->sys_n.push_back(A[s]->n);
     }
     // groups by block size, systems of a group one after another in a column
     std::vector<int> order(nsys);
@@ -577,6 +593,10 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
     dv(&st.res, nl);
     dv(&st.r0init, nl);
     dv(&st.hn, nl);
+    for (hipEvent_t &v : bt->ev)
+        if (!e && hipEventCreateWithFlags(&v, hipEventDisableTiming) != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "event");
+    if (!e && hipHostMalloc((void **)&bt->h_act, sizeof(int32_t) * 3 * nl) != hipSuccess)
+        e = cdg_fail(ctx, CDG_ERR_OOM, "batch: pinned flags");
     if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     const double t_alloc = bsecs(t0) - t_join;
     // the joined factors (one factor launch sequence per block size) and their row-slice storage
@@ -600,11 +620,41 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
     return CDG_OK;
 }
 
+// New blocks for the same systems (same patterns, e.g. the next Newton step's
+// Jacobians): copied into the joined matrices, which are refactored; nothing
+// is allocated.
+extern "C" int cdg_batch_update(cdg_ctx *ctx, cdg_batch *bt, cdg_mat *const *A) {
+    if (!ctx || !bt || !A) return CDG_ERR_ARG;
+    for (int s = 0; s < bt->nsys; ++s)
+        if (!A[s] || A[s]->nnzb != bt->sys_nnzb[s] || A[s]->nrows != bt->sys_nrows[s] || A[s]->n != bt->sys_n[s])
+            return cdg_fail(ctx, CDG_ERR_ARG, "batch update: system %d does not match the batch's pattern", s);
+    CDG_HIP_TRY(hipDeviceSynchronize());   // the callers' matrices are complete
+    const auto t0 = bclk::now();
+    int e = CDG_OK;
+    for (BatchGroup &g : bt->groups) {
+        size_t b0 = 0;
+        for (int s : g.members) {
+            const size_t bytes = sizeof(double) * (size_t)A[s]->nnzb * A[s]->ld;
+            if (!e && hipMemcpyAsync(g.U->blocks + b0 * g.U->ld, A[s]->blocks, bytes, hipMemcpyDeviceToDevice,
+                                     ctx->stream) != hipSuccess)
+                e = cdg_fail(ctx, CDG_ERR_HIP, "batch update: copying the blocks of system %d", s);
+            b0 += A[s]->nnzb;
+        }
+        if (!e) e = cdg_prec_update(ctx, g.M, g.U, nullptr);
+        if (!e) e = rs_prepare(ctx, g.M, g.U);
+    }
+    if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (btrace()) std::fprintf(stderr, "[batch] update: copies and factors %.3f s\n", bsecs(t0));
+    return e;
+}
+
 extern "C" int cdg_batch_destroy(cdg_ctx *ctx, cdg_batch *bt) {
     if (!bt) return CDG_OK;
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
-    if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
-    if (bt->graph) (void)hipGraphDestroy(bt->graph);
+    batch_drop_graphs(bt);
+    for (hipEvent_t &v : bt->ev)
+        if (v) (void)hipEventDestroy(v);
+    if (bt->h_act) (void)hipHostFree(bt->h_act);
     for (BatchGroup &g : bt->groups) {
         if (g.M) cdg_prec_destroy(ctx, g.M);
         if (g.U) cdg_mat_destroy(ctx, g.U);
@@ -701,39 +751,61 @@ extern "C" int cdg_batch_solve(cdg_ctx *ctx, cdg_batch *bt, const double *const 
         // v1 = r0 / ||r0||
         hipLaunchKernelGGL(k_bdiv, gi, dim3(BT), 0, ctx->stream, s, bt->R0, bt->V, bt->scale0, 1, bt->st.active);
         CDG_HIP_TRY(hipGetLastError());
-        // nkrylov Arnoldi steps: a captured graph (built at the first cycle), or the same launches
-        if (use_graph) {
-            if (bt->exec && (bt->graph_tol != tol || bt->graph_rtol != rtol || bt->graph_nmax != nmax)) {
-                (void)hipGraphExecDestroy(bt->exec);   // captured with another stopping test
-                (void)hipGraphDestroy(bt->graph);
-                bt->exec = nullptr;
-                bt->graph = nullptr;
-            }
-            if (!bt->exec) {
-                const auto tc = bclk::now();
+        // nkrylov Arnoldi steps, each a captured graph (built at the first cycle) or the same
+        // launches.  After each step the lanes' active flags go to pinned memory behind an
+        // event; the host stays at most two steps ahead of the device and launches no further
+        // step once a finished one shows every lane stopped.
+        if (use_graph && !bt->execs.empty() &&
+            (bt->graph_tol != tol || bt->graph_rtol != rtol || bt->graph_nmax != nmax))
+            batch_drop_graphs(bt);   // captured with another stopping test
+        if (use_graph && bt->execs.empty()) {
+            const auto tc = bclk::now();
+            size_t nodes = 0;
+            bt->graphs.assign(m, nullptr);
+            bt->execs.assign(m, nullptr);
+            for (int j = 0; j < m && !e; ++j) {
                 CDG_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                int ce = CDG_OK;
-                for (int j = 0; j < m && !ce; ++j) ce = batch_step(ctx, bt, j, tol, rtol, nmax);
+                const int ce = batch_step(ctx, bt, j, tol, rtol, nmax);
                 hipGraph_t gph = nullptr;
                 const hipError_t he = hipStreamEndCapture(ctx->stream, &gph);
-                if (ce) { if (gph) (void)hipGraphDestroy(gph); e = ce; break; }
-                if (he != hipSuccess) { e = cdg_fail(ctx, CDG_ERR_HIP, "graph capture: %s", hipGetErrorString(he)); break; }
-                bt->graph = gph;
-                CDG_HIP_TRY(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
-                bt->graph_tol = tol;
-                bt->graph_rtol = rtol;
-                bt->graph_nmax = nmax;
-                if (btrace()) {
-                    size_t nn = 0;
-                    (void)hipGraphGetNodes(bt->graph, nullptr, &nn);
-                    std::fprintf(stderr, "[batch] graph: %zu nodes, capture + instantiate %.3f s\n", nn, bsecs(tc));
+                if (ce || he != hipSuccess) {
+                    if (gph) (void)hipGraphDestroy(gph);
+                    e = ce ? ce : cdg_fail(ctx, CDG_ERR_HIP, "graph capture: %s", hipGetErrorString(he));
+                    break;
                 }
+                bt->graphs[j] = gph;
+                if (hipGraphInstantiate(&bt->execs[j], gph, nullptr, nullptr, 0) != hipSuccess)
+                    e = cdg_fail(ctx, CDG_ERR_HIP, "graph instantiate (step %d)", j);
+                size_t nn = 0;
+                (void)hipGraphGetNodes(gph, nullptr, &nn);
+                nodes += nn;
             }
-            CDG_HIP_TRY(hipGraphLaunch(bt->exec, ctx->stream));
-        } else {
-            for (int j = 0; j < m && !e; ++j) e = batch_step(ctx, bt, j, tol, rtol, nmax);
-            if (e) break;
+            if (e) { batch_drop_graphs(bt); break; }
+            bt->graph_tol = tol;
+            bt->graph_rtol = rtol;
+            bt->graph_nmax = nmax;
+            if (btrace())
+                std::fprintf(stderr, "[batch] graphs: %d steps, %zu nodes, capture + instantiate %.3f s\n", m, nodes,
+                             bsecs(tc));
         }
+        int launched = 0;
+        for (int j = 0; j < m && !e; ++j) {
+            if (j >= 2) {
+                const int q = (j - 2) % 3;
+                CDG_HIP_TRY(hipEventSynchronize(bt->ev[q]));
+                bool any = false;
+                for (int l = 0; l < nl && !any; ++l) any = bt->h_act[(size_t)q * nl + l] != 0;
+                if (!any) break;
+            }
+            if (use_graph) CDG_HIP_TRY(hipGraphLaunch(bt->execs[j], ctx->stream));
+            else e = batch_step(ctx, bt, j, tol, rtol, nmax);
+            if (e) break;
+            CDG_HIP_TRY(hipMemcpyAsync(bt->h_act + (size_t)(j % 3) * nl, bt->st.active, sizeof(int32_t) * nl,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            CDG_HIP_TRY(hipEventRecord(bt->ev[j % 3], ctx->stream));
+            ++launched;
+        }
+        if (e) break;
         // cycle end: y = H^-1 p per lane, x = x0 + Z y; the stopped lanes are done, the others restart
         CDG_HIP_TRY(hipMemcpyAsync(h_nv.data(), bt->st.nvecs, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, ctx->stream));
         CDG_HIP_TRY(hipMemcpyAsync(h_conv.data(), bt->st.conv, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, ctx->stream));
@@ -747,8 +819,8 @@ extern "C" int cdg_batch_solve(cdg_ctx *ctx, cdg_batch *bt, const double *const 
         if (btrace()) {
             int run = 0, mx = 0;
             for (int l = 0; l < nl; ++l) { run += h_i[l]; mx = std::max(mx, h_nv[l]); }
-            std::fprintf(stderr, "[batch] cycle: %d lanes ran, at most %d steps each; %.3f s since the solve began\n", run,
-                         mx, bsecs(t_start));
+            std::fprintf(stderr, "[batch] cycle: %d lanes ran, at most %d steps each, %d steps launched; %.3f s since "
+                         "the solve began\n", run, mx, launched, bsecs(t_start));
         }
         for (int l = 0; l < nl; ++l)
             if (h_err[l]) { e = cdg_fail(ctx, CDG_ERR_NAN, "non-finite FGMRES residual estimate (lane %d)", l); break; }

@@ -507,6 +507,14 @@ class batch:
         _before_torch(self.ctx)
         return list(st)
 
+    def update(self, mats=None):
+        """New blocks for the same systems (``mats``: stored matrices with the
+        patterns the batch was created from; default the creating ones):
+        copied into the joined matrices and refactored, nothing allocated."""
+        mats = self.mats if mats is None else list(mats)
+        handles = (C.c_void_p * len(mats))(*[A.h for A in mats])
+        self.ctx._check(self.ctx.lib.cdg_batch_update(self.ctx.h, self.h, handles), "cdg_batch_update")
+
     def info(self) -> dict:
         nl, ng, nv = C.c_int32(), C.c_int32(), C.c_int64()
         self.ctx._check(self.ctx.lib.cdg_batch_info(self.h, C.byref(nl), C.byref(ng), C.byref(nv)), "cdg_batch_info")

@@ -203,14 +203,17 @@ def batch(server: Server, reqs: list[Request]) -> dict:
 
 
 def joined(reqs: list[Request], device: int = 0, nkrylov: int = 30, tol: float = 0.0, rtol: float = 1e-8,
-           nmax: int = -1) -> dict:
+           nmax: int = -1, warm: bool = True) -> dict:
     """The same batch of independent systems as one joined solve
     (``linsolve.batch``, ``cdg_batch_*``): each request's matrix filled on
     the device from seed_A (as the request loop fills it), its right-hand
     sides from seeds seed_b + k, the systems of each block size joined and
     factored once, every right-hand side an FGMRES lane.  The whole job is
-    timed (fill, join + factor, solve); the requests' own matrices are
-    released once joined."""
+    timed: fill, join + factor (allocating the joined matrices, factors and
+    Krylov space), solve.  With ``warm``, the job again on the same objects
+    (the matrices refilled, ``batch.update`` copying and refactoring them,
+    the solve), as a Newton loop over a batch or a sweep service with its
+    batch created would run it: reported under "warm"."""
     import torch
     from .linsolve import Context, batch as joined_batch, chidg_matrix
     from .pattern import hex_box
@@ -219,40 +222,62 @@ def joined(reqs: list[Request], device: int = 0, nkrylov: int = 30, tol: float =
         raise ValueError("joined: one ILU precision for the whole batch")
     ctx = Context(device)
     dev = torch.empty(0, dtype=torch.float64, device=f"cuda:{device}")
-    try:
-        torch.cuda.synchronize(device)
-        t0 = time.perf_counter()
-        mats, B = [], []
-        for r in reqs:
-            per = tuple(bool(r.periodic >> d & 1) for d in range(3))
-            A = chidg_matrix(hex_box(r.nx, r.ny, r.nz, r.n, periodic=per), ctx)
+    nrhs = sum(max(1, r.nrhs) for r in reqs)
+
+    def fill(mats):
+        for A, r in zip(mats, reqs):
             c = r.c()
             A.fill_synthetic(r.seed_A, c.s_diag, c.s_off)
-            mats.append(A)
-            k = max(1, r.nrhs)
-            B.append(torch.stack([A.synthetic_vector(r.seed_b + i, like=dev) for i in range(k)]))
-        torch.cuda.synchronize(device)
-        t1 = time.perf_counter()
-        bt = joined_batch(mats, [b.shape[0] for b in B], precision=precision, nkrylov=nkrylov)
-        for A in mats:
-            A.release()
-        t2 = time.perf_counter()
+
+    def timed_solve(bt, B):
         X = [torch.zeros_like(b) for b in B]
         st = bt.solve(B, X, tol=tol, rtol=rtol, nmax=nmax)
         torch.cuda.synchronize(device)
+        return st
+
+    def summary(st, B, t0, t1, t2, t3):
+        bnorm = [float(torch.linalg.norm(b[i])) for b in B for i in range(b.shape[0])]
+        return dict(systems=len(reqs), rhs=nrhs, wall_s=t3 - t0, rhs_per_s=nrhs / (t3 - t0),
+                    systems_per_s=len(reqs) / (t3 - t0), t_fill_s=t1 - t0, t_join_factor_s=t2 - t1,
+                    t_solve_s=t3 - t2, rhs_per_s_solve=nrhs / (t3 - t2),
+                    iterations=int(sum(s.niter for s in st)), max_iterations=int(max(s.niter for s in st)),
+                    max_residual=float(max(s.residual_norm for s in st)),
+                    max_rel_residual=float(max(s.residual_norm / nb for s, nb in zip(st, bnorm))))
+
+    try:
+        torch.cuda.synchronize(device)
+        t0 = time.perf_counter()
+        mats = [chidg_matrix(hex_box(r.nx, r.ny, r.nz, r.n, periodic=tuple(bool(r.periodic >> d & 1) for d in range(3))),
+                             ctx) for r in reqs]
+        fill(mats)
+        B = [torch.stack([A.synthetic_vector(r.seed_b + i, like=dev) for i in range(max(1, r.nrhs))])
+             for A, r in zip(mats, reqs)]
+        torch.cuda.synchronize(device)
+        t1 = time.perf_counter()
+        bt = joined_batch(mats, [b.shape[0] for b in B], precision=precision, nkrylov=nkrylov)
+        t2 = time.perf_counter()
+        st = timed_solve(bt, B)
         t3 = time.perf_counter()
-        info = bt.info()
+        out = summary(st, B, t0, t1, t2, t3)
+        out["groups"] = bt.info()["groups"]
+        if warm:
+            torch.cuda.synchronize(device)
+            t0 = time.perf_counter()
+            fill(mats)
+            torch.cuda.synchronize(device)
+            t1 = time.perf_counter()
+            bt.update(mats)
+            t2 = time.perf_counter()
+            st2 = timed_solve(bt, B)
+            t3 = time.perf_counter()
+            out["warm"] = summary(st2, B, t0, t1, t2, t3)
+            out["warm"]["same_iterations"] = [s.niter for s in st2] == [s.niter for s in st]
         bt.close()
+        for A in mats:
+            A.release()
     finally:
         ctx.close()
-    nrhs = len(st)
-    bnorm = [float(torch.linalg.norm(b[i])) for b in B for i in range(b.shape[0])]
-    return dict(systems=len(reqs), rhs=nrhs, wall_s=t3 - t0, rhs_per_s=nrhs / (t3 - t0),
-                systems_per_s=len(reqs) / (t3 - t0), t_fill_s=t1 - t0, t_join_factor_s=t2 - t1,
-                t_solve_s=t3 - t2, rhs_per_s_solve=nrhs / (t3 - t2), groups=info["groups"],
-                iterations=int(sum(s.niter for s in st)), max_iterations=int(max(s.niter for s in st)),
-                max_residual=float(max(s.residual_norm for s in st)),
-                max_rel_residual=float(max(s.residual_norm / nb for s, nb in zip(st, bnorm))))
+    return out
 
 
 def poisson_load(server: Server, reqs: list[Request], rate: float, rng: np.random.Generator) -> dict:

@@ -162,3 +162,48 @@ def test_batch_64_uneven_config4(ctx):
     bt.close()
     for A in mats:
         A.release()
+
+
+def test_batch_update_equals_fresh_batch(ctx):
+    """cdg_batch_update with new blocks (the same patterns, other seeds: the
+    next Newton step's Jacobians) gives the bits of a batch created from the
+    new matrices, and each lane the iterations of its own solve on them."""
+    mats, rhs, _ = make(ctx, SMALL[:5], seed0=700)
+    nrhs = [b.shape[0] for b in rhs]
+    bt = batch(mats, nrhs, precision="fp32", nkrylov=30)
+    X0 = [np.zeros_like(b) for b in rhs]
+    bt.solve(rhs, X0, rtol=1e-9, nmax=NMAX)
+    new, _, _ = make(ctx, SMALL[:5], seed0=900)          # same patterns, other blocks
+    for A, An in zip(mats, new):
+        A.store(An.to_host())
+    bt.update(mats)
+    X1 = [np.zeros_like(b) for b in rhs]
+    s1 = bt.solve(rhs, X1, rtol=1e-9, nmax=NMAX)
+    fresh = batch(new, nrhs, precision="fp32", nkrylov=30)
+    X2 = [np.zeros_like(b) for b in rhs]
+    s2 = fresh.solve(rhs, X2, rtol=1e-9, nmax=NMAX)
+    assert [s.niter for s in s1] == [s.niter for s in s2]
+    for a, b in zip(X1, X2):
+        assert np.array_equal(a, b)
+    assert any(not np.allclose(a, b) for a, b in zip(X0, X1))     # the update took effect
+    lane = 0
+    for A, B in zip(new, rhs):
+        Xs = np.zeros_like(B)
+        ls = fgmres(nkrylov=30, tol=0.0, rtol=1e-9, nmax=NMAX, inner_fgmres=False)
+        ls.solve_multi(A, Xs, B, precon_ILU0("fp32", schedule="rowslice").init(A))
+        for r in range(B.shape[0]):
+            assert abs(s1[lane].niter - ls.niter[r]) <= 1
+            assert rel(X1[lane_sys(nrhs, lane)][r], Xs[r]) < 1e-8
+            lane += 1
+    with pytest.raises(Exception):
+        bt.update(list(reversed(mats)))                       # another pattern: refused
+    bt.close()
+    fresh.close()
+
+
+def lane_sys(nrhs, lane):
+    s = 0
+    while lane >= nrhs[s]:
+        lane -= nrhs[s]
+        s += 1
+    return s
