@@ -895,6 +895,29 @@ __device__ __forceinline__ bool panel_ab_seq(T *G, int gp, int NP, int n, int k0
     return __builtin_amdgcn_readfirstlane(*miss_sh) != 0;
 }
 
+// Raw buffer access to one block's slot: a byte offset at or beyond the
+// descriptor's size (kOob) reads 0 and drops the store, so masked lanes need
+// no branch (descriptor word 3: raw, untyped access on CDNA).
+typedef unsigned int bu32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kOob = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(void *p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <typename T> __device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
+}
+template <> __device__ __forceinline__ void bstore<double>(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu32x2, v), r, off, 0, 0);
+}
+
 // Blocked Gauss-Jordan inverse of the n x n block M (n <= 64*NR) by one
 // workgroup, working in the global slot S (L2-resident).  The rank-16 update
 // runs on the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
@@ -904,7 +927,7 @@ __device__ __forceinline__ bool panel_ab_seq(T *G, int gp, int NP, int n, int k0
 // false returns false as soon as a step needs another pivot than the
 // diagonal (M untouched); otherwise M receives the inverse and *bad reports
 // a zero or non-finite pivot column.
-template <typename T, int NR, bool PIVOT, int NTH>
+template <typename T, int NR, bool PIVOT, int NTH, int UB = 4>
 __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh, T (*prow2_sh)[16],
                           int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh, bool *bad) {
     constexpr int GP = 17;
@@ -918,18 +941,39 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
     for (int i = tid; i < 16 * XPP; i += NTH) XP[i] = T(0);
     if (tid == 0) *flag_sh = 0;
     __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(S, nn * (int)sizeof(T));
+    constexpr int PL0 = (16 * 64 * NR + NTH - 1) / NTH;  // panel and pivot-row values per thread (NP <= 64 NR)
+    constexpr int PB = sizeof(T) == 8 && PL0 > 5 ? 5 : PL0;   // loaded per batch (FP64: registers)
+    constexpr int PL = (PL0 + PB - 1) / PB * PB;
     for (int kb = 0; kb < NT; ++kb) {
         const int k0 = 16 * kb;
         const int w = min(16, n - k0);
-        // 1. panel to LDS (zero beyond n and beyond the panel width)
-        for (int e = tid; e < NP * 16; e += NTH) {
-            const int c = e / NP, r = e - c * NP;
-            G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
+        // 1. panel to LDS (zero beyond n and beyond the panel width): a thread's
+        //    loads all go out before its LDS stores
+        if constexpr (PIVOT) {   // the partial-pivoting form (rare) keeps the registers it had
+            for (int e = tid; e < NP * 16; e += NTH) {
+                const int c = e / NP, r = e - c * NP;
+                G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
+            }
+        } else
+#pragma unroll
+        for (int i0 = 0; i0 < PL; i0 += PB) {
+            T v[PB];
+#pragma unroll
+            for (int i = 0; i < PB; ++i) {
+                const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
+                v[i] = bload<T>(rs, (e < NP * 16 && r < n && c < w) ? (uint32_t)(((k0 + c) * n + r) * (int)sizeof(T)) : kOob);
+            }
+#pragma unroll
+            for (int i = 0; i < PB; ++i) {
+                const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
+                if (e < NP * 16) G[r * GP + c] = v[i];
+            }
         }
         __syncthreads();
         // 2. panel Gauss-Jordan; flag: zero pivot (PIVOT) or another pivot needed (!PIVOT).
         //    The checked path spreads the panel over all waves, one row per lane.
-        if (!PIVOT && NP <= 64 * (NTH / 64)) {
+        if constexpr (!PIVOT && 64 * NR <= NTH) {   // NP <= 64 NR rows: one row per lane of the workgroup
             if (panel_gjw<T, NTH / 64>(G, GP, NP, n, k0, w, rowtag, pivrow, prow2_sh, flag_sh)) return false;
         } else {
             if (wv == 0) {
@@ -940,48 +984,128 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
             __syncthreads();
             if (!PIVOT && __builtin_amdgcn_readfirstlane(*flag_sh)) return false;
         }
-        // 3. pivot rows X(P, :) of the other columns
-        for (int e = tid; e < w * n; e += NTH) {
-            const int c = e / w, j = e - c * w;
-            if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
-        }
-        __syncthreads();
-        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles); the panel takes C.
-        //    Each wave loads UB tiles before multiplying, so UB * NTH/64
-        //    tile loads are in flight per block.
-        constexpr int NWV = NTH / 64, UB = 4;
-        for (int t0 = wv; t0 < NT * NT; t0 += NWV * UB) {
-            acc_t acc[UB];
+        // 3. pivot rows X(P, :) of the other columns (loads first, as in 1)
+        if constexpr (PIVOT) {
+            for (int e = tid; e < w * n; e += NTH) {
+                const int c = e / w, j = e - c * w;
+                if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
+            }
+        } else
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int t = t0 + NWV * u;
-                const int ti = t % NT, tj = t / NT;
-                const int r = 16 * ti + (lane & 15);
+        for (int i0 = 0; i0 < PL; i0 += PB) {
+            T v[PB];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int c = 16 * tj + Mfma<T>::row(lane, q);
-                    acc[u][q] = (t < NT * NT && tj != kb && r < n && c < n) ? S[(size_t)c * n + r] : T(0);
-                }
+            for (int i = 0; i < PB; ++i) {
+                const int e = tid + (i0 + i) * NTH, c = e >> 4, j = e & 15;
+                const bool ok = c < n && j < w && (c < k0 || c >= k0 + w);
+                v[i] = bload<T>(rs, ok ? (uint32_t)((c * n + pivrow[k0 + j]) * (int)sizeof(T)) : kOob);
             }
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int t = t0 + NWV * u;
-                if (t >= NT * NT) continue;
-                const int ti = t % NT, tj = t / NT;     // rows 16ti.., columns 16tj..
-                const int r = 16 * ti + (lane & 15);
-                if (tj != kb) {
+            for (int i = 0; i < PB; ++i) {
+                const int e = tid + (i0 + i) * NTH, c = e >> 4, j = e & 15;
+                if (c < n && j < w && (c < k0 || c >= k0 + w)) XP[j * XPP + c] = v[i];
+            }
+        }
+        __syncthreads();
+        // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles) for the tiles outside
+        //    the panel's tile column, then the panel's tiles take C.  The update
+        //    is branch-free (buffer loads and stores; lanes outside the block
+        //    address kOob) and each wave's tiles go in rounds of UB with the next
+        //    round's loads issued before this round's stores: a wait for loads
+        //    never waits for stores (one in-order counter covers both).
+        if constexpr (PIVOT) {
+            constexpr int NWV = NTH / 64, UBP = 4;
+            for (int t0 = wv; t0 < NT * NT; t0 += NWV * UBP) {
+                acc_t acc[UBP];
+#pragma unroll
+                for (int u = 0; u < UBP; ++u) {
+                    const int t = t0 + NWV * u;
+                    const int ti = t % NT, tj = t / NT;
+                    const int r = 16 * ti + (lane & 15);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = 16 * tj + Mfma<T>::row(lane, q);
+                        acc[u][q] = (t < NT * NT && tj != kb && r < n && c < n) ? S[(size_t)c * n + r] : T(0);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UBP; ++u) {
+                    const int t = t0 + NWV * u;
+                    if (t >= NT * NT) continue;
+                    const int ti = t % NT, tj = t / NT;
+                    const int r = 16 * ti + (lane & 15);
+                    if (tj != kb) {
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) {
+                            const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
+                            const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
+                            acc[u] = Mfma<T>::run(a, bb, acc[u]);
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int c = 16 * tj + Mfma<T>::row(lane, q);
+                            if (r < n && c < n) S[(size_t)c * n + r] = acc[u][q];
+                        }
+                    } else if (r < n) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int cc = Mfma<T>::row(lane, q);
+                            const int c = k0 + cc;
+                            if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
+                        }
+                    }
+                }
+            }
+        } else {
+            constexpr int NWV = NTH / 64;
+            const int ntile = NT * (NT - 1);
+            const int nmine = ntile > wv ? (ntile - wv + NWV - 1) / NWV : 0;
+            auto tile_rc = [&](int i, int &ti, int &tj) {     // tile i of this wave (clamped for LDS reads)
+                const int t = min(wv + NWV * i, ntile - 1), tjp = t / NT;
+                ti = t - tjp * NT;
+                tj = tjp + (tjp >= kb ? 1 : 0);
+            };
+            auto offs = [&](int i, int q) -> uint32_t {
+                int ti, tj;
+                tile_rc(i, ti, tj);
+                const int r = 16 * ti + (lane & 15), c = 16 * tj + Mfma<T>::row(lane, q);
+                return (i < nmine && r < n && c < n) ? (uint32_t)((c * n + r) * (int)sizeof(T)) : kOob;
+            };
+            auto load_round = [&](T (&v)[UB][4], int i0) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[u][q] = bload<T>(rs, offs(i0 + u, q));
+            };
+            auto update_round = [&](const T (&v)[UB][4], int i0) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    int ti, tj;
+                    tile_rc(i0 + u, ti, tj);
+                    acc_t acc;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = v[u][q];
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk) {
                         const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
                         const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
-                        acc[u] = Mfma<T>::run(a, bb, acc[u]);
+                        acc = Mfma<T>::run(a, bb, acc);
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int c = 16 * tj + Mfma<T>::row(lane, q);
-                        if (r < n && c < n) S[(size_t)c * n + r] = acc[u][q];
-                    }
-                } else if (r < n) {
+                    for (int q = 0; q < 4; ++q) bstore<T>(acc[q], rs, offs(i0 + u, q));
+                }
+            };
+            T va[UB][4], vb[UB][4];
+            load_round(va, 0);
+            for (int i0 = 0; i0 < nmine; i0 += 2 * UB) {
+                load_round(vb, i0 + UB);
+                update_round(va, i0);
+                load_round(va, i0 + 2 * UB);
+                update_round(vb, i0 + UB);
+            }
+            for (int ti = wv; ti < NT; ti += NWV) {
+                const int r = 16 * ti + (lane & 15);
+                if (r < n) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int cc = Mfma<T>::row(lane, q);
@@ -1006,7 +1130,7 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
 // block whose check fails is flagged for k_fac_pivot.  LDS is dynamic.
 constexpr int kInvThreads = 512;   // global-memory inverse: 8 waves per block
 
-template <typename T, int NR>
+template <typename T, int NR, int UB>
 __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
                                                    const int32_t *__restrict__ rows, int n,
                                                    int32_t *__restrict__ redo) {
@@ -1023,7 +1147,7 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int
     T *S = Wk + (size_t)blockIdx.x * ldw;
     bool bad = false;
     __shared__ T prow2_sh[2][16];
-    if (!ginv_body<T, NR, false, kInvThreads>(M, S, n, G, XP, prow_sh, prow2_sh, rowtag, pivrow, &flag_sh, &bad)) {
+    if (!ginv_body<T, NR, false, kInvThreads, UB>(M, S, n, G, XP, prow_sh, prow2_sh, rowtag, pivrow, &flag_sh, &bad)) {
         if (threadIdx.x == 0) redo[blk] = 1;    // left to k_fac_pivot; M untouched
     }
 }
@@ -1313,20 +1437,39 @@ int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows
     // Both kernels may take the LDS of the largest block this template serves
     // (n <= 64 NR, at most 320): set once per process, thread-safe (request-loop
     // workers launch concurrently), whatever block size comes first.
+    // tiles each wave loads per round of the rank-16 update (two rounds' tiles
+    // are in registers at once); CDG_GINV_TILES = 2, 4 or 8 overrides 4
+    static const int ub = [] {
+        const char *e = std::getenv("CDG_GINV_TILES");
+        const int v = e ? std::atoi(e) : 4;
+        return v <= 2 ? 2 : v <= 4 ? 4 : 8;
+    }();
+    const void *fn = ub == 2 ? reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 2>)
+                   : ub == 4 ? reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 4>)
+                             : reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 8>);
     static std::once_flag once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(once, [] {
         const int most = (int)ginv_lds<T, NR>(std::min(64 * NR, 320));
-        attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_ginv<T, NR>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, most);
-        if (attr_err == hipSuccess)
-            attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fac_pivot<T, NR>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, most);
+        const void *fns[] = {reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 2>),
+                             reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 4>),
+                             reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 8>),
+                             reinterpret_cast<const void *>(&k_fac_pivot<T, NR>)};
+        for (const void *f : fns)
+            if (attr_err == hipSuccess)
+                attr_err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, most);
     });
     if (attr_err != hipSuccess)
         return cdg_fail(ctx, CDG_ERR_HIP, "hipFuncSetAttribute (block inverse LDS): %s", hipGetErrorString(attr_err));
-    hipLaunchKernelGGL((k_fac_ginv<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,
-                       static_cast<T *>(M->inv_w), M->ld, rows, n, M->d_redo);
+    {
+        T *Dp = D, *Wp = static_cast<T *>(M->inv_w);
+        int64_t ldp = ld, ldw = M->ld;
+        const int32_t *rp = rows;
+        int np = n;
+        int32_t *redo = M->d_redo;
+        void *args[] = {&Dp, &ldp, &Wp, &ldw, &rp, &np, &redo};
+        CDG_HIP_TRY(hipLaunchKernel(fn, dim3(nrows_lvl), dim3(kInvThreads), args, lds, ctx->stream));
+    }
     CDG_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL((k_fac_pivot<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,
                        static_cast<T *>(M->inv_w), M->ld, rows, n, err, M->d_redo);
