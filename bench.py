@@ -432,6 +432,69 @@ def cpu_baseline(p, blocks, b, x_gpu):
                                     f"iterations"}), parity
 
 
+def profile_workload(name):
+    """Short fixed workloads for counter and trace passes (no JSON line):
+    * apply: the FP32 row-slice ILU(0) apply on the Euler 16^3 P2 system, 5
+      times after the one that forms the row-slice storage, then a 512 MiB
+      device copy 3 times (read 512 MiB + write 512 MiB each: the reference
+      for scaling HBM counters);
+    * factor: the ILU(0) factor of the Euler 16^3 P2 system, twice in FP64 and
+      twice in FP32, then the seeded synthetic 16^3 P3 system once in FP32 and
+      once in FP64;
+    * batch: the 64-system uneven batch as one joined solve, cold then warm."""
+    import torch
+    from chidg_amd import pattern as pt
+    from chidg_amd.linsolve import Context, chidg_matrix, precon_ILU0
+    if name == "batch":
+        from chidg_amd.serve import joined, sweep_requests
+        reqs = sweep_requests(64, np.random.default_rng(20261202), meshes=(4, 12), orders=(1, 2, 3), nrhs=(1, 8))
+        out = joined(reqs, rtol=1e-8)
+        log(f"batch workload: cold {out['wall_s']:.2f} s, warm {out['warm']['wall_s']:.2f} s")
+        return
+    ctx = Context(0)
+    p, blocks, b_host = euler_system()
+    A = chidg_matrix(p, ctx)
+    A.store(blocks)
+    if name == "apply":
+        b = torch.from_numpy(b_host).to("cuda:0")
+        v = b / torch.linalg.norm(b)
+        z = torch.empty_like(b)
+        M = precon_ILU0("fp32", schedule="rowslice").init(A)
+        M.update(A)
+        M.apply_async(A, v, z)               # forms W and the row-slice copies
+        torch.cuda.synchronize()
+        for _ in range(5):
+            M.apply_async(A, v, z)
+        torch.cuda.synchronize()
+        log("apply workload: 5 row-slice applies done")
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("apply workload: 3 x 512 MiB copies done")
+    elif name == "factor":
+        for prec in ("fp64", "fp32"):
+            M = precon_ILU0(prec).init(A)
+            for _ in range(2):
+                M.update(A)
+            torch.cuda.synchronize()
+            log(f"factor workload: Euler 16^3 P2 {prec} x2 done")
+            M.tear_down()
+        A.release()
+        n = pt.block_size(3, NFIELDS)
+        A = chidg_matrix(pt.hex_box(N_ELEM, N_ELEM, N_ELEM, n), ctx)
+        A.fill_synthetic(SEED_A, 1.0 / np.sqrt(n), 0.1 / np.sqrt(n))
+        for prec in ("fp32", "fp64"):
+            M = precon_ILU0(prec).init(A)
+            M.update(A)
+            torch.cuda.synchronize()
+            log(f"factor workload: synthetic 16^3 P3 {prec} done")
+            M.tear_down()
+    A.release()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -456,7 +519,12 @@ def main():
                     help="skip timing the folded level schedule (W = U Dinv) of the FP64 and FP32 forms")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference for HBM counter scaling)")
+    ap.add_argument("--workload", choices=["apply", "factor", "batch"], default=None,
+                    help="run one short fixed workload for a profiler pass and exit (no JSON line)")
     args = ap.parse_args()
+    if args.workload:
+        profile_workload(args.workload)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (each system is solved on "
               "one GPU)", file=sys.stderr)
