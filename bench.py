@@ -233,6 +233,7 @@ def synthetic_run(args):
     b = A.synthetic_vector(SEED_B, 1.0, like=torch.empty(0, dtype=torch.float64, device="cuda:0"))
     out, _ = measure(args, p, A, b, 5, 1, kernels=False, precisions=("fp32", "fp64"))
     A.release()
+    ctx.close()            # its cached device memory back to the device
     return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in out.items()}
 
 
@@ -273,6 +274,7 @@ def p3_run(args):
     for prec in ("fp32", "bf16"):
         out[prec]["x_rel_vs_fp64_ilu"] = float(torch.linalg.norm(xs[prec] - xs["fp64"]) / torch.linalg.norm(xs["fp64"]))
     A.release()
+    ctx.close()
     return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in out.items()}
 
 
@@ -310,8 +312,11 @@ def batch_run(args):
       the same server again (its slots' device memory cached);
     * joined: the systems of each block size in one block-diagonal matrix, one
       factor, every right-hand side a lockstep FGMRES lane, per-step graphs;
-      warm = the matrices refilled, copied in and refactored (no allocation)."""
+      warm = the joined matrices refactored in place (no allocation, no
+      fill) and solved."""
+    import torch
     from chidg_amd.serve import Server, batch, joined, sweep_requests
+    torch.cuda.empty_cache()   # the joined batch takes about 200 GB of device memory
     rng = np.random.default_rng(20261202)
     reqs = sweep_requests(64, rng, meshes=(4, 12), orders=(1, 2, 3), nrhs=(1, 8))
     fmt = lambda d: {k: sig(float(v)) if isinstance(v, float) else v for k, v in d.items()}
@@ -541,6 +546,7 @@ def main():
     p, blocks, b_host = euler_system()
     g, x64, A = gpu_run(args, p, blocks, b_host)
     A.release()
+    A.ctx.close()          # the headline section's cached device memory back to the device
     f32, f64 = g["fp32"], g["fp64"]
     extras = [e for e in args.extras.split(",") if e]
     sections = {}

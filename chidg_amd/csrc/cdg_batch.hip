@@ -622,10 +622,10 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
 
 // New blocks for the same systems (same patterns, e.g. the next Newton step's
 // Jacobians): copied into the joined matrices, which are refactored; nothing
-// is allocated.
+// is allocated.  A = NULL refactors the joined matrices as they are.
 extern "C" int cdg_batch_update(cdg_ctx *ctx, cdg_batch *bt, cdg_mat *const *A) {
-    if (!ctx || !bt || !A) return CDG_ERR_ARG;
-    for (int s = 0; s < bt->nsys; ++s)
+    if (!ctx || !bt) return CDG_ERR_ARG;
+    for (int s = 0; A && s < bt->nsys; ++s)
         if (!A[s] || A[s]->nnzb != bt->sys_nnzb[s] || A[s]->nrows != bt->sys_nrows[s] || A[s]->n != bt->sys_n[s])
             return cdg_fail(ctx, CDG_ERR_ARG, "batch update: system %d does not match the batch's pattern", s);
     CDG_HIP_TRY(hipDeviceSynchronize());   // the callers' matrices are complete
@@ -634,6 +634,7 @@ extern "C" int cdg_batch_update(cdg_ctx *ctx, cdg_batch *bt, cdg_mat *const *A) 
     for (BatchGroup &g : bt->groups) {
         size_t b0 = 0;
         for (int s : g.members) {
+            if (!A) break;
             const size_t bytes = sizeof(double) * (size_t)A[s]->nnzb * A[s]->ld;
             if (!e && hipMemcpyAsync(g.U->blocks + b0 * g.U->ld, A[s]->blocks, bytes, hipMemcpyDeviceToDevice,
                                      ctx->stream) != hipSuccess)

@@ -509,10 +509,12 @@ class batch:
 
     def update(self, mats=None):
         """New blocks for the same systems (``mats``: stored matrices with the
-        patterns the batch was created from; default the creating ones):
-        copied into the joined matrices and refactored, nothing allocated."""
-        mats = self.mats if mats is None else list(mats)
-        handles = (C.c_void_p * len(mats))(*[A.h for A in mats])
+        patterns the batch was created from): copied into the joined matrices
+        and refactored, nothing allocated.  ``mats`` None: the joined matrices
+        refactored as they are."""
+        handles = None
+        if mats is not None:
+            handles = (C.c_void_p * len(mats))(*[A.h for A in mats])
         self.ctx._check(self.ctx.lib.cdg_batch_update(self.ctx.h, self.h, handles), "cdg_batch_update")
 
     def info(self) -> dict:

@@ -210,10 +210,11 @@ def joined(reqs: list[Request], device: int = 0, nkrylov: int = 30, tol: float =
     sides from seeds seed_b + k, the systems of each block size joined and
     factored once, every right-hand side an FGMRES lane.  The whole job is
     timed: fill, join + factor (allocating the joined matrices, factors and
-    Krylov space), solve.  With ``warm``, the job again on the same objects
-    (the matrices refilled, ``batch.update`` copying and refactoring them,
-    the solve), as a Newton loop over a batch or a sweep service with its
-    batch created would run it: reported under "warm"."""
+    Krylov space), solve; the requests' matrices are released once joined (the
+    batch holds its copy: both at once would fill the device at this size).
+    With ``warm``, the joined matrices refactored in place
+    (``batch.update()``, nothing allocated) and solved again: reported under
+    "warm", with no fill."""
     import torch
     from .linsolve import Context, batch as joined_batch, chidg_matrix
     from .pattern import hex_box
@@ -255,6 +256,8 @@ def joined(reqs: list[Request], device: int = 0, nkrylov: int = 30, tol: float =
         torch.cuda.synchronize(device)
         t1 = time.perf_counter()
         bt = joined_batch(mats, [b.shape[0] for b in B], precision=precision, nkrylov=nkrylov)
+        for A in mats:
+            A.release()
         t2 = time.perf_counter()
         st = timed_solve(bt, B)
         t3 = time.perf_counter()
@@ -262,19 +265,14 @@ def joined(reqs: list[Request], device: int = 0, nkrylov: int = 30, tol: float =
         out["groups"] = bt.info()["groups"]
         if warm:
             torch.cuda.synchronize(device)
-            t0 = time.perf_counter()
-            fill(mats)
-            torch.cuda.synchronize(device)
-            t1 = time.perf_counter()
-            bt.update(mats)
+            t0 = t1 = time.perf_counter()
+            bt.update()
             t2 = time.perf_counter()
             st2 = timed_solve(bt, B)
             t3 = time.perf_counter()
             out["warm"] = summary(st2, B, t0, t1, t2, t3)
             out["warm"]["same_iterations"] = [s.niter for s in st2] == [s.niter for s in st]
         bt.close()
-        for A in mats:
-            A.release()
     finally:
         ctx.close()
     return out

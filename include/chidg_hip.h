@@ -293,7 +293,8 @@ int cdg_batch_solve(cdg_ctx *ctx, cdg_batch *batch, const double *const *b, doub
                     const cdg_fgmres_params *params, cdg_fgmres_stats *stats);
 /* New blocks for the batch's systems (A[s] with the patterns it was created
  * from, e.g. the next Newton step's Jacobians): copied into the joined
- * matrices and refactored, with no allocation. */
+ * matrices and refactored, with no allocation.  A = NULL refactors the
+ * joined matrices as they are. */
 int cdg_batch_update(cdg_ctx *ctx, cdg_batch *batch, cdg_mat *const *A);
 int cdg_batch_info(const cdg_batch *batch, int32_t *nlanes, int32_t *ngroups, int64_t *values);
 int cdg_batch_destroy(cdg_ctx *ctx, cdg_batch *batch);

@@ -165,14 +165,20 @@ def test_batch_64_uneven_config4(ctx):
 
 
 def test_batch_update_equals_fresh_batch(ctx):
-    """cdg_batch_update with new blocks (the same patterns, other seeds: the
-    next Newton step's Jacobians) gives the bits of a batch created from the
-    new matrices, and each lane the iterations of its own solve on them."""
+    """cdg_batch_update without matrices refactors in place (the same bits);
+    with new blocks (the same patterns, other seeds: the next Newton step's
+    Jacobians) it gives the bits of a batch created from the new matrices,
+    and each lane the iterations of its own solve on them."""
     mats, rhs, _ = make(ctx, SMALL[:5], seed0=700)
     nrhs = [b.shape[0] for b in rhs]
     bt = batch(mats, nrhs, precision="fp32", nkrylov=30)
     X0 = [np.zeros_like(b) for b in rhs]
-    bt.solve(rhs, X0, rtol=1e-9, nmax=NMAX)
+    s0 = bt.solve(rhs, X0, rtol=1e-9, nmax=NMAX)
+    bt.update()                                           # refactored in place: the same bits
+    Xr = [np.zeros_like(b) for b in rhs]
+    sr = bt.solve(rhs, Xr, rtol=1e-9, nmax=NMAX)
+    assert [s.niter for s in sr] == [s.niter for s in s0]
+    assert all(np.array_equal(a, b) for a, b in zip(Xr, X0))
     new, _, _ = make(ctx, SMALL[:5], seed0=900)          # same patterns, other blocks
     for A, An in zip(mats, new):
         A.store(An.to_host())
