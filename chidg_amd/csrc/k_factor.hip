@@ -1126,6 +1126,169 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
     return true;
 }
 
+// ginv_body for FP32 with diagonal pivots, in 32-column panels: every panel
+// rewrites the whole work slot, so halving their number halves the slot
+// traffic that sets the inverse's time.  A panel is two 16-column halves:
+// half 0's Gauss-Jordan steps (panel_gjw), their transform applied to half 1
+// in LDS (half 1 += (C0 - E_P0) half1(P0, :)), half 1's steps, their
+// transform applied to half 0; then one rank-32 update of every tile outside
+// the panel, and the panel's tiles take C.  The same blocked Gauss-Jordan as
+// ginv_body (the result agrees to rounding).  LDS (ginv_lds): G two halves
+// (NP x 17 each), XP 32 x (NP + 1), PR 16 x 16, rowtag, pivrow.  Returns
+// false (M untouched) when a step needs another pivot than the diagonal.
+template <int NR, int NTH, int UB>
+__device__ bool ginv_body_wide(float *__restrict__ M, float *__restrict__ S, int n, float *G, float *XP, float *PR,
+                               float (*prow2_sh)[16], int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh) {
+    typedef float T;
+    typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int GP = 17, NWV = NTH / 64;
+    const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nn = n * n;
+    T *const G0 = G, *const G1 = G + (size_t)NP * GP;
+    for (int e = tid; e < nn; e += NTH) S[e] = M[e];
+    for (int i = tid; i < NP; i += NTH) { rowtag[i] = 0; pivrow[i] = 0; }
+    if (tid == 0) *flag_sh = 0;
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(S, nn * 4);
+    constexpr int PL = (32 * 64 * NR + NTH - 1) / NTH;   // panel (and pivot-row) values per thread
+    // half t += (half s - E_Ps) * half t(P_s, :), P_s = rows ks .. ks + ws - 1
+    auto half_update = [&](T *Gt, const T *Gs, int ks, int ws) {
+        if (tid < 256) {
+            const int j = tid >> 4, c = tid & 15;
+            PR[tid] = j < ws ? Gt[pivrow[ks + j] * GP + c] : T(0);
+        }
+        __syncthreads();
+        for (int e = tid; e < NP * 16; e += NTH) {
+            const int r = e >> 4, c = e & 15;
+            T acc = Gt[r * GP + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc = fma(Gs[r * GP + j], PR[j * 16 + c], acc);
+            Gt[r * GP + c] = acc;
+        }
+        __syncthreads();
+    };
+    for (int kb = 0; kb < NT; kb += 2) {
+        const int k0 = 16 * kb;
+        const int np = min(2, NT - kb);                          // tile columns of this panel
+        const int w0 = min(16, n - k0), w1 = np > 1 ? min(16, n - k0 - 16) : 0;
+        // 1. both halves to LDS (zero beyond n and beyond each half's width)
+        {
+            T v[PL];
+#pragma unroll
+            for (int i = 0; i < PL; ++i) {
+                const int e = tid + i * NTH, c = e / NP, r = e - c * NP;
+                const bool ok = e < NP * 32 && r < n && (c < 16 ? c < w0 : c - 16 < w1);
+                v[i] = bload<T>(rs, ok ? (uint32_t)(((k0 + c) * n + r) * 4) : kOob);
+            }
+#pragma unroll
+            for (int i = 0; i < PL; ++i) {
+                const int e = tid + i * NTH, c = e / NP, r = e - c * NP;
+                if (e < NP * 32) (c < 16 ? G0 : G1)[r * GP + (c & 15)] = v[i];
+            }
+        }
+        __syncthreads();
+        // 2. half 0's steps, then their transform on half 1; 3. half 1's steps, their transform on half 0
+        if (panel_gjw<T, NWV>(G0, GP, NP, n, k0, w0, rowtag, pivrow, prow2_sh, flag_sh)) return false;
+        if (np > 1) {
+            half_update(G1, G0, k0, w0);
+            if (panel_gjw<T, NWV>(G1, GP, NP, n, k0 + 16, w1, rowtag, pivrow, prow2_sh, flag_sh)) return false;
+            half_update(G0, G1, k0 + 16, w1);
+        }
+        // 4. pivot rows X(P, :) of the columns outside the panel (32 rows: half 0's, then half 1's)
+        {
+            T v[PL];
+#pragma unroll
+            for (int i = 0; i < PL; ++i) {
+                const int e = tid + i * NTH, c = e >> 5, j = e & 31;
+                const bool ok = c < n && (j < 16 ? j < w0 : j - 16 < w1) && (c < k0 || c >= k0 + 16 * np);
+                v[i] = bload<T>(rs, ok ? (uint32_t)((c * n + pivrow[min(k0 + j, NP - 1)]) * 4) : kOob);
+            }
+#pragma unroll
+            for (int i = 0; i < PL; ++i) {
+                const int e = tid + i * NTH, c = e >> 5, j = e & 31;
+                if (c < n) XP[j * XPP + c] = v[i];   // zero where the row or column is outside
+            }
+        }
+        __syncthreads();
+        // 5. X += (C - E_P) X(P, :) for every tile outside the panel (transposed tiles,
+        //    branch-free, the next round's loads before this round's stores), then the
+        //    panel's tiles take C
+        {
+            const int ntile = NT * (NT - np);
+            const int nmine = ntile > wv ? (ntile - wv + NWV - 1) / NWV : 0;
+            auto tile_rc = [&](int i, int &ti, int &tj) {
+                const int t = min(wv + NWV * i, max(ntile - 1, 0)), tjp = t / NT;
+                ti = t - tjp * NT;
+                tj = tjp + (tjp >= kb ? np : 0);
+            };
+            auto offs = [&](int i, int q) -> uint32_t {
+                int ti, tj;
+                tile_rc(i, ti, tj);
+                const int r = 16 * ti + (lane & 15), c = 16 * tj + Mfma<T>::row(lane, q);
+                return (i < nmine && r < n && c < n) ? (uint32_t)((c * n + r) * 4) : kOob;
+            };
+            auto load_round = [&](T (&v)[UB][4], int i0) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[u][q] = bload<T>(rs, offs(i0 + u, q));
+            };
+            auto update_round = [&](const T (&v)[UB][4], int i0) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    int ti, tj;
+                    tile_rc(i0 + u, ti, tj);
+                    acc_t acc;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = v[u][q];
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        if (kk < 4 * np) {
+                            const int h = kk >> 2, k4 = kk & 3;
+                            const T a = XP[(16 * h + 4 * k4 + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
+                            const T bb = (h ? G1 : G0)[(16 * ti + (lane & 15)) * GP + 4 * k4 + (lane >> 4)];
+                            acc = Mfma<T>::run(a, bb, acc);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) bstore<T>(acc[q], rs, offs(i0 + u, q));
+                }
+            };
+            T va[UB][4], vb[UB][4];
+            load_round(va, 0);
+            for (int i0 = 0; i0 < nmine; i0 += 2 * UB) {
+                load_round(vb, i0 + UB);
+                update_round(va, i0);
+                load_round(va, i0 + 2 * UB);
+                update_round(vb, i0 + UB);
+            }
+            for (int h = 0; h < np; ++h) {
+                const T *Gh = h ? G1 : G0;
+                const int kh = k0 + 16 * h, wh = h ? w1 : w0;
+                for (int ti = wv; ti < NT; ti += NWV) {
+                    const int r = 16 * ti + (lane & 15);
+                    if (r < n) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int cc = Mfma<T>::row(lane, q);
+                            if (cc < wh) S[(size_t)(kh + cc) * n + r] = Gh[r * GP + cc] + ((rowtag[r] - 1 - kh == cc) ? T(1) : T(0));
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // inv(rowstep(r), pivrow(c)) = S(r, c)
+    for (int e = tid; e < nn; e += NTH) {
+        const int c = e / n, r = e - c * n;
+        M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
+    }
+    return true;
+}
+
 // Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body with diagonal pivots; a
 // block whose check fails is flagged for k_fac_pivot.  LDS is dynamic.
 constexpr int kInvThreads = 512;   // global-memory inverse: 8 waves per block
@@ -1136,20 +1299,29 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int
                                                    int32_t *__restrict__ redo) {
     extern __shared__ unsigned char smem_raw[];
     const int NP = 16 * ((n + 15) / 16), XPP = NP + 1;
-    T *G = reinterpret_cast<T *>(smem_raw);                 // NP * 17
-    T *XP = G + (size_t)NP * 17;                            // 16 * XPP
-    T *prow_sh = XP + (size_t)16 * XPP;                     // 16
-    int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);   // NP
-    int32_t *pivrow = rowtag + NP;                          // NP
     __shared__ int32_t flag_sh;
+    __shared__ T prow2_sh[2][16];
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
     T *S = Wk + (size_t)blockIdx.x * ldw;
-    bool bad = false;
-    __shared__ T prow2_sh[2][16];
-    if (!ginv_body<T, NR, false, kInvThreads, UB>(M, S, n, G, XP, prow_sh, prow2_sh, rowtag, pivrow, &flag_sh, &bad)) {
-        if (threadIdx.x == 0) redo[blk] = 1;    // left to k_fac_pivot; M untouched
+    bool ok;
+    if constexpr (sizeof(T) == 4) {   // 32-column panels (LDS: ginv_lds)
+        T *G = reinterpret_cast<T *>(smem_raw);             // 2 * NP * 17
+        T *XP = G + (size_t)2 * NP * 17;                    // 32 * XPP
+        T *PR = XP + (size_t)32 * XPP;                      // 256
+        int32_t *rowtag = reinterpret_cast<int32_t *>(PR + 256);   // NP
+        int32_t *pivrow = rowtag + NP;                      // NP
+        ok = ginv_body_wide<NR, kInvThreads, UB>(M, S, n, G, XP, PR, prow2_sh, rowtag, pivrow, &flag_sh);
+    } else {
+        T *G = reinterpret_cast<T *>(smem_raw);             // NP * 17
+        T *XP = G + (size_t)NP * 17;                        // 16 * XPP
+        T *prow_sh = XP + (size_t)16 * XPP;                 // 16
+        int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);   // NP
+        int32_t *pivrow = rowtag + NP;                      // NP
+        bool bad = false;
+        ok = ginv_body<T, NR, false, kInvThreads, UB>(M, S, n, G, XP, prow_sh, prow2_sh, rowtag, pivrow, &flag_sh, &bad);
     }
+    if (!ok && threadIdx.x == 0) redo[blk] = 1;    // left to k_fac_pivot; M untouched
 }
 
 // Dinv_i = inv(D_i) for blocks of n <= 16*NT, one workgroup of 4 waves per
@@ -1380,9 +1552,12 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_pivot(T *__restrict__ D, in
     }
 }
 
+// Dynamic LDS of k_fac_ginv (FP32: wide panels) and k_fac_pivot (16-column: less)
 template <typename T, int NR>
 size_t ginv_lds(int n) {
     const int NP = 16 * ((n + 15) / 16);
+    if (sizeof(T) == 4)
+        return sizeof(T) * ((size_t)2 * NP * 17 + (size_t)32 * (NP + 1) + 256) + sizeof(int32_t) * 2 * (size_t)NP;
     return sizeof(T) * ((size_t)NP * 17 + (size_t)16 * (NP + 1) + 16) + sizeof(int32_t) * 2 * (size_t)NP;
 }
 
