@@ -307,8 +307,9 @@ struct cdg_batch {
     int64_t Ltot = 0;
     std::vector<int64_t> off, len;     // per system: offset in a column, length
     std::vector<int> nrhs;
-    std::vector<int64_t> sys_nnzb;     // per system: blocks (for cdg_batch_update's checks)
+    std::vector<int64_t> sys_nnzb;     // per system: blocks and pattern (for cdg_batch_update's checks)
     std::vector<int> sys_nrows, sys_n;
+    std::vector<std::vector<int32_t>> sys_rp, sys_col, sys_kind;
     std::vector<BatchGroup> groups;
     // lanes and work items
     int nlanes = 0, nitems = 0;
@@ -403,6 +404,11 @@ int batch_step(cdg_ctx *ctx, cdg_batch *bt, int j, double tol, double rtol, int 
 
 using bclk = std::chrono::steady_clock;
 
+// a HIP wait inside cdg_batch_create: the error goes to e (the batch is then destroyed)
+int bwait(cdg_ctx *ctx, hipError_t he, const char *what) {
+    return he == hipSuccess ? CDG_OK : cdg_fail(ctx, CDG_ERR_HIP, "batch: %s: %s", what, hipGetErrorString(he));
+}
+
 void batch_drop_graphs(cdg_batch *bt) {
     for (hipGraphExec_t x : bt->execs)
         if (x) (void)hipGraphExecDestroy(x);
@@ -445,6 +451,9 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
         bt->sys_nnzb.push_back(A[s]->nnzb);
         bt->sys_nrows.push_back(A[s]->nrows);
         bt->sys_n.push_back(A[s]->n);
+        bt->sys_rp.push_back(A[s]->row_ptr);
+        bt->sys_col.push_back(A[s]->col);
+        bt->sys_kind.push_back(A[s]->kind);
     }
     // groups by block size, systems of a group one after another in a column
     std::vector<int> order(nsys);
@@ -469,8 +478,7 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
         pos += bt->len[s];
     }
     bt->Ltot = pos;
-    int e = CDG_OK;
-    CDG_HIP_TRY(hipDeviceSynchronize());   // the callers' matrices are complete
+    int e = bwait(ctx, hipDeviceSynchronize(), "synchronize");   // the callers' matrices are complete
     // joined matrices: concatenated patterns, blocks copied device to device
     for (BatchGroup &g : bt->groups) {
         std::vector<int32_t> rp(1, 0), col, kind, rowk, rowsys;
@@ -516,7 +524,7 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
         if (!e) e = bupload(ctx, bt, &g.d_row_k, rowk);
         if (!e) e = bupload(ctx, bt, &g.d_row_kl, rowk);
         if (!e) e = bupload(ctx, bt, &g.d_row_sys, rowsys);
-        if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (!e) e = bwait(ctx, hipStreamSynchronize(ctx->stream), "block copies");
         const double t_copy = bsecs(tm) - t_mat;
         if (!e) e = cdg_prec_create(ctx, g.U, CDG_PREC_ILU0, precision, &g.M);
         if (!e) e = cdg_prec_set_schedule(ctx, g.M, CDG_SCHEDULE_ROWSLICE);
@@ -524,7 +532,7 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
             std::fprintf(stderr, "[batch] join n=%d: matrix %.3f s, block copies %.3f s, preconditioner %.3f s\n", g.n,
                          t_mat, t_copy, bsecs(tm) - t_mat - t_copy);
     }
-    if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (!e) e = bwait(ctx, hipStreamSynchronize(ctx->stream), "join");
     const double t_join = bsecs(t0);
     // lanes and work items
     std::vector<int64_t> lane_off;
@@ -597,7 +605,7 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
         if (!e && hipEventCreateWithFlags(&v, hipEventDisableTiming) != hipSuccess) e = cdg_fail(ctx, CDG_ERR_HIP, "event");
     if (!e && hipHostMalloc((void **)&bt->h_act, sizeof(int32_t) * 3 * nl) != hipSuccess)
         e = cdg_fail(ctx, CDG_ERR_OOM, "batch: pinned flags");
-    if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (!e) e = bwait(ctx, hipStreamSynchronize(ctx->stream), "allocation");
     const double t_alloc = bsecs(t0) - t_join;
     // the joined factors (one factor launch sequence per block size) and their row-slice storage
     for (BatchGroup &g : bt->groups) {
@@ -606,12 +614,12 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
         if (!e) e = hipStreamSynchronize(ctx->stream) == hipSuccess ? CDG_OK : cdg_fail(ctx, CDG_ERR_HIP, "factor");
         const double tf = bsecs(tg);
         if (!e) e = rs_prepare(ctx, g.M, g.U);
-        if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (!e) e = bwait(ctx, hipStreamSynchronize(ctx->stream), "row-slice storage");
         if (btrace())
             std::fprintf(stderr, "[batch] group n=%d: %d systems, %d rows, factor %.3f s, row-slice storage %.3f s\n", g.n,
                          (int)g.members.size(), g.U ? g.U->nrows : 0, tf, bsecs(tg) - tf);
     }
-    if (!e) CDG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (!e) e = bwait(ctx, hipStreamSynchronize(ctx->stream), "factor");
     if (btrace())
         std::fprintf(stderr, "[batch] create: join %.3f s, allocate %.3f s, total %.3f s; %d lanes, %d work items\n",
                      t_join, t_alloc, bsecs(t0), bt->nlanes, bt->nitems);
@@ -626,7 +634,9 @@ extern "C" int cdg_batch_create(cdg_ctx *ctx, int32_t nsys, cdg_mat *const *A, c
 extern "C" int cdg_batch_update(cdg_ctx *ctx, cdg_batch *bt, cdg_mat *const *A) {
     if (!ctx || !bt) return CDG_ERR_ARG;
     for (int s = 0; A && s < bt->nsys; ++s)
-        if (!A[s] || A[s]->nnzb != bt->sys_nnzb[s] || A[s]->nrows != bt->sys_nrows[s] || A[s]->n != bt->sys_n[s])
+        if (!A[s] || A[s]->nnzb != bt->sys_nnzb[s] || A[s]->nrows != bt->sys_nrows[s] || A[s]->n != bt->sys_n[s] ||
+            !A[s]->part.empty() || A[s]->row_ptr != bt->sys_rp[s] || A[s]->col != bt->sys_col[s] ||
+            A[s]->kind != bt->sys_kind[s])
             return cdg_fail(ctx, CDG_ERR_ARG, "batch update: system %d does not match the batch's pattern", s);
     CDG_HIP_TRY(hipDeviceSynchronize());   // the callers' matrices are complete
     const auto t0 = bclk::now();
