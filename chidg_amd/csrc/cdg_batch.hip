@@ -18,11 +18,14 @@
 // * The Arnoldi step's vector work runs segmented: one launch covers every
 //   lane's segment (work items of 2048 values), with per-lane reductions in a
 //   fixed order; the rotations, the residual estimate and the stopping test
-//   of every lane run on the device (one thread per lane).  A whole cycle of
-//   nkrylov steps has no host synchronization: it is captured once in a HIP
-//   graph and replayed per cycle; lanes that stop mid-cycle are skipped by
-//   the remaining steps.  The host runs the cycle ends (back substitution of
-//   the small triangular systems, restarts), as fgmres_t%solve does.
+//   of every lane run on the device (one thread per lane).  Each Arnoldi
+//   step is captured once as a HIP graph and replayed; the host never waits
+//   for the step it launched, only for the one two steps back (its lanes'
+//   active flags copied to pinned memory behind an event), and launches no
+//   further step once every lane has stopped.  Lanes that stop mid-cycle are
+//   skipped by the remaining steps (rows of fully stopped systems too).  The
+//   host runs the cycle ends (back substitution of the small triangular
+//   systems, restarts), as fgmres_t%solve does.
 #include "cdg_internal.h"
 
 #include <algorithm>
