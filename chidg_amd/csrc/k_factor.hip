@@ -600,58 +600,57 @@ __device__ __forceinline__ bool panel_gj(T *G, int gp, int NP, int n, int k0, in
     return bad;
 }
 
-// The checked-diagonal panel of panel_gj<T, NR, false>, spread over the NW
-// waves of the workgroup: row r sits in wave r % NW, lane r / NW (NP <=
-// 64 * NW), so every lane holds one row.  The pivot row of step j (row k0+j)
-// is published by its owner through a double-buffered LDS row, one barrier
-// per step (a wave can only overwrite buffer j % 2 after every wave has
-// passed step j + 1's barrier, i.e. has read it).  Same arithmetic, same
-// check, same outputs (G = C - E_P, rowtag, pivrow); returns whether some
-// step needed another pivot.  Called by all 64 * NW threads.
+// The checked-diagonal panel of panel_gj<T, NR, false>, spread over the
+// workgroup: row r sits in lane r of the block (NP <= 64 * NW), and waves past
+// the last row only keep the barriers.  Step j: the owner of pivot row
+// p = k0 + j checks its pivot, scales its row by 1 / a_pp (column j takes
+// 1 / a_pp itself) and publishes it with |a_pp| through a double-buffered LDS
+// row, one barrier per step (a wave can only overwrite buffer j % 2 after
+// every wave has passed step j + 1's barrier, i.e. has read it); every other
+// row r then takes a_rj times the scaled row (column j takes -a_rj / a_pp).
+// The steps are unrolled, so each value keeps its register.  Outputs
+// G = C - E_P, rowtag, pivrow; returns whether some step needed another pivot.
+// Called by all 64 * NW threads.
 template <typename T, int NW>
 __device__ __forceinline__ bool panel_gjw(T *G, int gp, int NP, int n, int k0, int w, int32_t *rowtag,
-                                          int32_t *pivrow, T (*prow_sh)[16], int32_t *miss_sh) {
-    const int tid = threadIdx.x;
-    const int wv = tid >> 6, lane = tid & 63;
-    const int row = lane * NW + wv;                        // this lane's panel row
-    T pr[16];
+                                          int32_t *pivrow, T (*prow_sh)[17], int32_t *miss_sh) {
+    const int row = threadIdx.x;
     const bool live = row < NP;
+    const bool wave_live = __builtin_amdgcn_readfirstlane(row & ~63) < NP;
+    T pr[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) pr[c] = live ? G[row * gp + c] : T(0);
     bool used = row >= n || (live && rowtag[row] != 0);
     bool viol = false;
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < 16; ++j) {
         if (j < w) {
-            const int p = k0 + j;
             const int buf = j & 1;
-            if (row == p) {
+            const bool isp = row == k0 + j;
+            if (isp) {
+                const T ap = fabs(pr[j]);
+                if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
+                const T rinv = T(1) / pr[j];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) pr[c] = c == j ? rinv : pr[c] * rinv;
 #pragma unroll
                 for (int c = 0; c < 16; ++c) prow_sh[buf][c] = pr[c];
+                prow_sh[buf][16] = ap;
             }
             __syncthreads();
-            T prow[16];
+            if (wave_live && !isp) {
+                T prow[17];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) prow[c] = prow_sh[buf][c];
-            const T ap = fabs(prow[0]);
-            if (!(ap > T(0)) || !(ap < T(INFINITY))) viol = true;
-            if (!used && row != p && T(kPivTau) * fabs(pr[0]) > ap) viol = true;
-            const T rinv = T(1) / prow[0];
+                for (int c = 0; c < 17; ++c) prow[c] = prow_sh[buf][c];
+                if (!used && T(kPivTau) * fabs(pr[j]) > prow[16]) viol = true;
+                const T cr = pr[j];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) prow[c] *= rinv;
-            const bool isp = row == p;
-            const T cr = isp ? T(0) : pr[0];
-#pragma unroll
-            for (int c = 1; c < 16; ++c) pr[c] = isp ? prow[c] : pr[c] - cr * prow[c];
-            pr[0] = isp ? rinv : -cr * rinv;
+                for (int c = 0; c < 16; ++c) pr[c] = c == j ? -cr * prow[j] : fma(-cr, prow[c], pr[c]);
+            }
             used = used || isp;
         }
-        const T first = pr[0];
-#pragma unroll
-        for (int c = 0; c < 15; ++c) pr[c] = pr[c + 1];
-        pr[15] = first;
     }
-    if (tid < 16 && tid < w) { rowtag[k0 + tid] = k0 + tid + 1; pivrow[k0 + tid] = k0 + tid; }
+    if (threadIdx.x < 16 && (int)threadIdx.x < w) { rowtag[k0 + threadIdx.x] = k0 + threadIdx.x + 1; pivrow[k0 + threadIdx.x] = k0 + threadIdx.x; }
     if (live) {
         const int tg = (row >= k0 && row < k0 + w) ? row - k0 : -1;   // this panel's pivot rows are k0..k0+w-1
 #pragma unroll
@@ -918,19 +917,18 @@ template <> __device__ __forceinline__ void bstore<double>(double v, __amdgpu_bu
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu32x2, v), r, off, 0, 0);
 }
 
-// Blocked Gauss-Jordan inverse of the n x n block M (n <= 64*NR) by one
-// workgroup, working in the global slot S (L2-resident).  The rank-16 update
-// runs on the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row is then a
-// matrix column, so its 16 lanes touch 16 consecutive elements of the
-// column-major block.  LDS: the panel G (NP x 17), the pivot rows XP
-// (16 x (NP + 1)), prow_sh, rowtag, pivrow (NP = 16 * ceil(n/16)).  PIVOT =
-// false returns false as soon as a step needs another pivot than the
-// diagonal (M untouched); otherwise M receives the inverse and *bad reports
-// a zero or non-finite pivot column.
-template <typename T, int NR, bool PIVOT, int NTH, int UB = 4>
-__device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh, T (*prow2_sh)[16],
-                          int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh, bool *bad) {
-    constexpr int GP = 17;
+// Blocked Gauss-Jordan inverse with partial pivoting of the n x n block M
+// (n <= 64*NR) by one workgroup, working in the global slot S (L2-resident):
+// the rare blocks that the diagonal-pivot inverse hands over.  The rank-16
+// update runs on the transposed tiles, D'(c, r) = S(r, c): an MFMA tile row
+// is then a matrix column, so its 16 lanes touch 16 consecutive elements of
+// the column-major block.  LDS: the panel G (NP x 17), the pivot rows XP
+// (16 x (NP + 1)), prow_sh, rowtag, pivrow (NP = 16 * ceil(n/16)).  M
+// receives the inverse and *bad reports a zero or non-finite pivot column.
+template <typename T, int NR, int NTH>
+__device__ void ginv_pivot_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *XP, T *prow_sh,
+                                int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh, bool *bad) {
+    constexpr int GP = 17, NWV = NTH / 64, UBP = 4;
     typedef typename Mfma<T>::acc_t acc_t;
     const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -941,171 +939,62 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
     for (int i = tid; i < 16 * XPP; i += NTH) XP[i] = T(0);
     if (tid == 0) *flag_sh = 0;
     __syncthreads();
-    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(S, nn * (int)sizeof(T));
-    constexpr int PL0 = (16 * 64 * NR + NTH - 1) / NTH;  // panel and pivot-row values per thread (NP <= 64 NR)
-    constexpr int PB = sizeof(T) == 8 && PL0 > 5 ? 5 : PL0;   // loaded per batch (FP64: registers)
-    constexpr int PL = (PL0 + PB - 1) / PB * PB;
     for (int kb = 0; kb < NT; ++kb) {
         const int k0 = 16 * kb;
         const int w = min(16, n - k0);
-        // 1. panel to LDS (zero beyond n and beyond the panel width): a thread's
-        //    loads all go out before its LDS stores
-        if constexpr (PIVOT) {   // the partial-pivoting form (rare) keeps the registers it had
-            for (int e = tid; e < NP * 16; e += NTH) {
-                const int c = e / NP, r = e - c * NP;
-                G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
-            }
-        } else
-#pragma unroll
-        for (int i0 = 0; i0 < PL; i0 += PB) {
-            T v[PB];
-#pragma unroll
-            for (int i = 0; i < PB; ++i) {
-                const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
-                v[i] = bload<T>(rs, (e < NP * 16 && r < n && c < w) ? (uint32_t)(((k0 + c) * n + r) * (int)sizeof(T)) : kOob);
-            }
-#pragma unroll
-            for (int i = 0; i < PB; ++i) {
-                const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
-                if (e < NP * 16) G[r * GP + c] = v[i];
-            }
+        // 1. panel to LDS (zero beyond n and beyond the panel width)
+        for (int e = tid; e < NP * 16; e += NTH) {
+            const int c = e / NP, r = e - c * NP;
+            G[r * GP + c] = (r < n && c < w) ? S[(size_t)(k0 + c) * n + r] : T(0);
         }
         __syncthreads();
-        // 2. panel Gauss-Jordan; flag: zero pivot (PIVOT) or another pivot needed (!PIVOT).
-        //    The checked path spreads the panel over all waves, one row per lane.
-        if constexpr (!PIVOT && 64 * NR <= NTH) {   // NP <= 64 NR rows: one row per lane of the workgroup
-            if (panel_gjw<T, NTH / 64>(G, GP, NP, n, k0, w, rowtag, pivrow, prow2_sh, flag_sh)) return false;
-        } else {
-            if (wv == 0) {
-                bool miss = false;
-                const bool b = panel_gj<T, NR, PIVOT>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
-                if (lane == 0 && (PIVOT ? b : miss)) *flag_sh = 1;
-            }
-            __syncthreads();
-            if (!PIVOT && __builtin_amdgcn_readfirstlane(*flag_sh)) return false;
+        // 2. panel Gauss-Jordan with partial pivoting (wave 0); flag: zero pivot
+        if (wv == 0) {
+            bool miss = false;
+            const bool b = panel_gj<T, NR, true>(G, GP, NP, n, k0, w, rowtag, pivrow, prow_sh, &miss);
+            if (lane == 0 && b) *flag_sh = 1;
         }
-        // 3. pivot rows X(P, :) of the other columns (loads first, as in 1)
-        if constexpr (PIVOT) {
-            for (int e = tid; e < w * n; e += NTH) {
-                const int c = e / w, j = e - c * w;
-                if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
-            }
-        } else
-#pragma unroll
-        for (int i0 = 0; i0 < PL; i0 += PB) {
-            T v[PB];
-#pragma unroll
-            for (int i = 0; i < PB; ++i) {
-                const int e = tid + (i0 + i) * NTH, c = e >> 4, j = e & 15;
-                const bool ok = c < n && j < w && (c < k0 || c >= k0 + w);
-                v[i] = bload<T>(rs, ok ? (uint32_t)((c * n + pivrow[k0 + j]) * (int)sizeof(T)) : kOob);
-            }
-#pragma unroll
-            for (int i = 0; i < PB; ++i) {
-                const int e = tid + (i0 + i) * NTH, c = e >> 4, j = e & 15;
-                if (c < n && j < w && (c < k0 || c >= k0 + w)) XP[j * XPP + c] = v[i];
-            }
+        __syncthreads();
+        // 3. pivot rows X(P, :) of the other columns
+        for (int e = tid; e < w * n; e += NTH) {
+            const int c = e / w, j = e - c * w;
+            if (c < k0 || c >= k0 + w) XP[j * XPP + c] = S[(size_t)c * n + pivrow[k0 + j]];
         }
         __syncthreads();
         // 4. X += (C - E_P) X(P,:) on MFMA (transposed tiles) for the tiles outside
-        //    the panel's tile column, then the panel's tiles take C.  The update
-        //    is branch-free (buffer loads and stores; lanes outside the block
-        //    address kOob) and each wave's tiles go in rounds of UB with the next
-        //    round's loads issued before this round's stores: a wait for loads
-        //    never waits for stores (one in-order counter covers both).
-        if constexpr (PIVOT) {
-            constexpr int NWV = NTH / 64, UBP = 4;
-            for (int t0 = wv; t0 < NT * NT; t0 += NWV * UBP) {
-                acc_t acc[UBP];
+        //    the panel's tile column, then the panel's tiles take C
+        for (int t0 = wv; t0 < NT * NT; t0 += NWV * UBP) {
+            acc_t acc[UBP];
 #pragma unroll
-                for (int u = 0; u < UBP; ++u) {
-                    const int t = t0 + NWV * u;
-                    const int ti = t % NT, tj = t / NT;
-                    const int r = 16 * ti + (lane & 15);
+            for (int u = 0; u < UBP; ++u) {
+                const int t = t0 + NWV * u;
+                const int ti = t % NT, tj = t / NT;
+                const int r = 16 * ti + (lane & 15);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int c = 16 * tj + Mfma<T>::row(lane, q);
-                        acc[u][q] = (t < NT * NT && tj != kb && r < n && c < n) ? S[(size_t)c * n + r] : T(0);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < UBP; ++u) {
-                    const int t = t0 + NWV * u;
-                    if (t >= NT * NT) continue;
-                    const int ti = t % NT, tj = t / NT;
-                    const int r = 16 * ti + (lane & 15);
-                    if (tj != kb) {
-#pragma unroll
-                        for (int kk = 0; kk < 4; ++kk) {
-                            const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
-                            const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
-                            acc[u] = Mfma<T>::run(a, bb, acc[u]);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int c = 16 * tj + Mfma<T>::row(lane, q);
-                            if (r < n && c < n) S[(size_t)c * n + r] = acc[u][q];
-                        }
-                    } else if (r < n) {
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int cc = Mfma<T>::row(lane, q);
-                            const int c = k0 + cc;
-                            if (cc < w) S[(size_t)c * n + r] = G[r * GP + cc] + ((rowtag[r] - 1 - k0 == cc) ? T(1) : T(0));
-                        }
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    const int c = 16 * tj + Mfma<T>::row(lane, q);
+                    acc[u][q] = (t < NT * NT && tj != kb && r < n && c < n) ? S[(size_t)c * n + r] : T(0);
                 }
             }
-        } else {
-            constexpr int NWV = NTH / 64;
-            const int ntile = NT * (NT - 1);
-            const int nmine = ntile > wv ? (ntile - wv + NWV - 1) / NWV : 0;
-            auto tile_rc = [&](int i, int &ti, int &tj) {     // tile i of this wave (clamped for LDS reads)
-                const int t = min(wv + NWV * i, ntile - 1), tjp = t / NT;
-                ti = t - tjp * NT;
-                tj = tjp + (tjp >= kb ? 1 : 0);
-            };
-            auto offs = [&](int i, int q) -> uint32_t {
-                int ti, tj;
-                tile_rc(i, ti, tj);
-                const int r = 16 * ti + (lane & 15), c = 16 * tj + Mfma<T>::row(lane, q);
-                return (i < nmine && r < n && c < n) ? (uint32_t)((c * n + r) * (int)sizeof(T)) : kOob;
-            };
-            auto load_round = [&](T (&v)[UB][4], int i0) {
 #pragma unroll
-                for (int u = 0; u < UB; ++u)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) v[u][q] = bload<T>(rs, offs(i0 + u, q));
-            };
-            auto update_round = [&](const T (&v)[UB][4], int i0) {
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    int ti, tj;
-                    tile_rc(i0 + u, ti, tj);
-                    acc_t acc;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[q] = v[u][q];
+            for (int u = 0; u < UBP; ++u) {
+                const int t = t0 + NWV * u;
+                if (t >= NT * NT) continue;
+                const int ti = t % NT, tj = t / NT;
+                const int r = 16 * ti + (lane & 15);
+                if (tj != kb) {
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk) {
                         const T a = XP[(4 * kk + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
                         const T bb = G[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)];
-                        acc = Mfma<T>::run(a, bb, acc);
+                        acc[u] = Mfma<T>::run(a, bb, acc[u]);
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) bstore<T>(acc[q], rs, offs(i0 + u, q));
-                }
-            };
-            T va[UB][4], vb[UB][4];
-            load_round(va, 0);
-            for (int i0 = 0; i0 < nmine; i0 += 2 * UB) {
-                load_round(vb, i0 + UB);
-                update_round(va, i0);
-                load_round(va, i0 + 2 * UB);
-                update_round(vb, i0 + UB);
-            }
-            for (int ti = wv; ti < NT; ti += NWV) {
-                const int r = 16 * ti + (lane & 15);
-                if (r < n) {
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = 16 * tj + Mfma<T>::row(lane, q);
+                        if (r < n && c < n) S[(size_t)c * n + r] = acc[u][q];
+                    }
+                } else if (r < n) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int cc = Mfma<T>::row(lane, q);
@@ -1117,117 +1006,118 @@ __device__ bool ginv_body(T *__restrict__ M, T *__restrict__ S, int n, T *G, T *
         }
         __syncthreads();
     }
-    if (PIVOT) *bad = __builtin_amdgcn_readfirstlane(*flag_sh) != 0;
+    *bad = __builtin_amdgcn_readfirstlane(*flag_sh) != 0;
     // inv(rowstep(r), pivrow(c)) = S(r, c)
     for (int e = tid; e < nn; e += NTH) {
         const int c = e / n, r = e - c * n;
         M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
     }
-    return true;
 }
 
-// ginv_body for FP32 with diagonal pivots, in 32-column panels: every panel
-// rewrites the whole work slot, so halving their number halves the slot
-// traffic that sets the inverse's time.  A panel is two 16-column halves:
-// half 0's Gauss-Jordan steps (panel_gjw), their transform applied to half 1
-// in LDS (half 1 += (C0 - E_P0) half1(P0, :)), half 1's steps, their
-// transform applied to half 0; then one rank-32 update of every tile outside
-// the panel, and the panel's tiles take C.  The same blocked Gauss-Jordan as
-// ginv_body (the result agrees to rounding).  LDS (ginv_lds): G two halves
-// (NP x 17 each), XP 32 x (NP + 1), PR 16 x 16, rowtag, pivrow.  Returns
-// false (M untouched) when a step needs another pivot than the diagonal.
-template <int NR, int NTH, int UB>
-__device__ bool ginv_body_wide(float *__restrict__ M, float *__restrict__ S, int n, float *G, float *XP, float *PR,
-                               float (*prow2_sh)[16], int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh) {
-    typedef float T;
+// Wide-panel inverse with the pivot rows in registers: panels of H 16-column
+// halves (each half's Gauss-Jordan steps, then their transform on the other
+// halves in LDS), then every tile outside the panel takes the rank-16H update.
+// Each wave owns whole tile columns; before updating any tile of a column it
+// loads that column's pivot-row operands (16H rows, 4H values per lane) into
+// registers, so no other wave can have changed them (the column's tiles, its
+// pivot-row tiles among them, are only this wave's).  Fewer, wider panels
+// mean fewer passes over the slot, whose traffic sets the inverse's time;
+// with the pivot rows out of LDS, the halves alone take it.  LDS (ginv_lds):
+// H halves of NP x 17, rowtag, pivrow.
+template <typename T> struct GinvHalves { static constexpr int H = sizeof(T) == 4 ? 4 : 3; };
+template <typename T, int NR, int NTH, int UB, int H>
+__device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T *G, T (*prow2_sh)[17],
+                                  int32_t *rowtag, int32_t *pivrow, int32_t *flag_sh) {
     typedef typename Mfma<T>::acc_t acc_t;
-    constexpr int GP = 17, NWV = NTH / 64;
-    const int NT = (n + 15) / 16, NP = 16 * NT, XPP = NP + 1;
+    constexpr int GP = 17, NWV = NTH / 64, PW = 16 * H;
+    const int NT = (n + 15) / 16, NP = 16 * NT;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nn = n * n;
-    T *const G0 = G, *const G1 = G + (size_t)NP * GP;
+    auto half = [&](int h) -> T * { return G + (size_t)h * NP * GP; };
     for (int e = tid; e < nn; e += NTH) S[e] = M[e];
     for (int i = tid; i < NP; i += NTH) { rowtag[i] = 0; pivrow[i] = 0; }
     if (tid == 0) *flag_sh = 0;
     __syncthreads();
-    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(S, nn * 4);
-    constexpr int PL = (32 * 64 * NR + NTH - 1) / NTH;   // panel (and pivot-row) values per thread
-    // half t += (half s - E_Ps) * half t(P_s, :), P_s = rows ks .. ks + ws - 1
-    auto half_update = [&](T *Gt, const T *Gs, int ks, int ws) {
-        if (tid < 256) {
-            const int j = tid >> 4, c = tid & 15;
-            PR[tid] = j < ws ? Gt[pivrow[ks + j] * GP + c] : T(0);
-        }
-        __syncthreads();
-        for (int e = tid; e < NP * 16; e += NTH) {
-            const int r = e >> 4, c = e & 15;
-            T acc = Gt[r * GP + c];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) acc = fma(Gs[r * GP + j], PR[j * 16 + c], acc);
-            Gt[r * GP + c] = acc;
-        }
-        __syncthreads();
-    };
-    for (int kb = 0; kb < NT; kb += 2) {
+    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(S, nn * (int)sizeof(T));
+    constexpr int PL0 = (PW * 64 * NR + NTH - 1) / NTH;   // panel values per thread
+    constexpr int PB = PL0 < 10 ? PL0 : 10;   // loaded per batch (registers)
+    constexpr int PL = (PL0 + PB - 1) / PB * PB;
+    for (int kb = 0; kb < NT; kb += H) {
         const int k0 = 16 * kb;
-        const int np = min(2, NT - kb);                          // tile columns of this panel
-        const int w0 = min(16, n - k0), w1 = np > 1 ? min(16, n - k0 - 16) : 0;
-        // 1. both halves to LDS (zero beyond n and beyond each half's width)
-        {
-            T v[PL];
+        const int np = min(H, NT - kb);                         // halves (tile columns) of this panel
+        auto width = [&](int h) { return h < np ? min(16, n - k0 - 16 * h) : 0; };
+        // 1. the halves to LDS (zero beyond n and beyond each half's width)
+#pragma unroll 1
+        for (int i0 = 0; i0 < PL; i0 += PB) {
+            T v[PB];
 #pragma unroll
-            for (int i = 0; i < PL; ++i) {
-                const int e = tid + i * NTH, c = e / NP, r = e - c * NP;
-                const bool ok = e < NP * 32 && r < n && (c < 16 ? c < w0 : c - 16 < w1);
-                v[i] = bload<T>(rs, ok ? (uint32_t)(((k0 + c) * n + r) * 4) : kOob);
+            for (int i = 0; i < PB; ++i) {
+                const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
+                const bool ok = e < NP * PW && r < n && (c & 15) < width(c >> 4);
+                v[i] = bload<T>(rs, ok ? (uint32_t)(((k0 + c) * n + r) * (int)sizeof(T)) : kOob);
             }
 #pragma unroll
-            for (int i = 0; i < PL; ++i) {
-                const int e = tid + i * NTH, c = e / NP, r = e - c * NP;
-                if (e < NP * 32) (c < 16 ? G0 : G1)[r * GP + (c & 15)] = v[i];
-            }
-        }
-        __syncthreads();
-        // 2. half 0's steps, then their transform on half 1; 3. half 1's steps, their transform on half 0
-        if (panel_gjw<T, NWV>(G0, GP, NP, n, k0, w0, rowtag, pivrow, prow2_sh, flag_sh)) return false;
-        if (np > 1) {
-            half_update(G1, G0, k0, w0);
-            if (panel_gjw<T, NWV>(G1, GP, NP, n, k0 + 16, w1, rowtag, pivrow, prow2_sh, flag_sh)) return false;
-            half_update(G0, G1, k0 + 16, w1);
-        }
-        // 4. pivot rows X(P, :) of the columns outside the panel (32 rows: half 0's, then half 1's)
-        {
-            T v[PL];
-#pragma unroll
-            for (int i = 0; i < PL; ++i) {
-                const int e = tid + i * NTH, c = e >> 5, j = e & 31;
-                const bool ok = c < n && (j < 16 ? j < w0 : j - 16 < w1) && (c < k0 || c >= k0 + 16 * np);
-                v[i] = bload<T>(rs, ok ? (uint32_t)((c * n + pivrow[min(k0 + j, NP - 1)]) * 4) : kOob);
-            }
-#pragma unroll
-            for (int i = 0; i < PL; ++i) {
-                const int e = tid + i * NTH, c = e >> 5, j = e & 31;
-                if (c < n) XP[j * XPP + c] = v[i];   // zero where the row or column is outside
+            for (int i = 0; i < PB; ++i) {
+                const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
+                if (e < NP * PW) half(c >> 4)[r * GP + (c & 15)] = v[i];
             }
         }
         __syncthreads();
-        // 5. X += (C - E_P) X(P, :) for every tile outside the panel (transposed tiles,
-        //    branch-free, the next round's loads before this round's stores), then the
-        //    panel's tiles take C
-        {
-            const int ntile = NT * (NT - np);
-            const int nmine = ntile > wv ? (ntile - wv + NWV - 1) / NWV : 0;
-            auto tile_rc = [&](int i, int &ti, int &tj) {
-                const int t = min(wv + NWV * i, max(ntile - 1, 0)), tjp = t / NT;
-                ti = t - tjp * NT;
-                tj = tjp + (tjp >= kb ? np : 0);
-            };
-            auto offs = [&](int i, int q) -> uint32_t {
-                int ti, tj;
-                tile_rc(i, ti, tj);
-                const int r = 16 * ti + (lane & 15), c = 16 * tj + Mfma<T>::row(lane, q);
-                return (i < nmine && r < n && c < n) ? (uint32_t)((c * n + r) * 4) : kOob;
+        // 2. each half's steps, then their transform on the panel's other halves t:
+        //    G_t += G_h G_t(P_h, :) on MFMA tiles (P_h: half h's pivot rows), whose
+        //    pivot-row operands every wave reads before any tile changes
+        for (int h = 0; h < np; ++h) {
+            const int kh = k0 + 16 * h, wh = width(h);
+            if (panel_gjw<T, NWV>(half(h), GP, NP, n, kh, wh, rowtag, pivrow, prow2_sh, flag_sh)) return false;
+            if (np == 1) break;
+            T b[H][4];
+#pragma unroll
+            for (int t = 0; t < H; ++t)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int j = 4 * kk + (lane >> 4);
+                    b[t][kk] = (t < np && t != h && j < wh) ? half(t)[pivrow[kh + j] * GP + (lane & 15)] : T(0);
+                }
+            __syncthreads();
+            const T *Gs = half(h);
+            for (int idx = wv; idx < NT * (np - 1); idx += NWV) {
+                const int s = idx / NT, ti = idx - s * NT, t = s + (s >= h);
+                T bs[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    bs[kk] = b[0][kk];
+#pragma unroll
+                    for (int u = 1; u < H; ++u) bs[kk] = t == u ? b[u][kk] : bs[kk];
+                }
+                T *Gt = half(t);
+                acc_t acc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = Gt[(16 * ti + Mfma<T>::row(lane, q)) * GP + (lane & 15)];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    acc = Mfma<T>::run(Gs[(16 * ti + (lane & 15)) * GP + 4 * kk + (lane >> 4)], bs[kk], acc);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Gt[(16 * ti + Mfma<T>::row(lane, q)) * GP + (lane & 15)] = acc[q];
+            }
+            __syncthreads();
+        }
+        // 3. X += (C - E_P) X(P, :), tile column by tile column (transposed tiles, branch-free,
+        //    the next round's loads before this round's stores), then the panel's tiles take C
+        const int ncol = NT - np;
+        for (int tjp = wv; tjp < ncol; tjp += NWV) {
+            const int tj = tjp + (tjp >= kb ? np : 0);
+            const int c = 16 * tj + (lane & 15);
+            T a[4 * H];   // X(P, c): pivot row 4 kk + lane / 16 of the panel, this lane's column
+#pragma unroll
+            for (int kk = 0; kk < 4 * H; ++kk) {
+                const int h = kk >> 2, j = 4 * (kk & 3) + (lane >> 4);
+                const bool ok = h < np && j < width(h) && c < n;
+                a[kk] = bload<T>(rs, ok ? (uint32_t)((c * n + pivrow[min(k0 + 16 * h + j, NP - 1)]) * (int)sizeof(T)) : kOob);
+            }
+            auto offs = [&](int ti, int q) -> uint32_t {
+                const int r = 16 * ti + (lane & 15), cq = 16 * tj + Mfma<T>::row(lane, q);
+                return (ti < NT && r < n && cq < n) ? (uint32_t)((cq * n + r) * (int)sizeof(T)) : kOob;
             };
             auto load_round = [&](T (&v)[UB][4], int i0) {
 #pragma unroll
@@ -1238,18 +1128,16 @@ __device__ bool ginv_body_wide(float *__restrict__ M, float *__restrict__ S, int
             auto update_round = [&](const T (&v)[UB][4], int i0) {
 #pragma unroll
                 for (int u = 0; u < UB; ++u) {
-                    int ti, tj;
-                    tile_rc(i0 + u, ti, tj);
+                    const int ti = min(i0 + u, NT - 1);
+                    const T *grow = G + (16 * ti + (lane & 15)) * GP + (lane >> 4);
                     acc_t acc;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) acc[q] = v[u][q];
 #pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) {
+                    for (int kk = 0; kk < 4 * H; ++kk) {
                         if (kk < 4 * np) {
-                            const int h = kk >> 2, k4 = kk & 3;
-                            const T a = XP[(16 * h + 4 * k4 + (lane >> 4)) * XPP + 16 * tj + (lane & 15)];
-                            const T bb = (h ? G1 : G0)[(16 * ti + (lane & 15)) * GP + 4 * k4 + (lane >> 4)];
-                            acc = Mfma<T>::run(a, bb, acc);
+                            const T bb = grow[(kk >> 2) * NP * GP + 4 * (kk & 3)];
+                            acc = Mfma<T>::run(a[kk], bb, acc);
                         }
                     }
 #pragma unroll
@@ -1258,30 +1146,29 @@ __device__ bool ginv_body_wide(float *__restrict__ M, float *__restrict__ S, int
             };
             T va[UB][4], vb[UB][4];
             load_round(va, 0);
-            for (int i0 = 0; i0 < nmine; i0 += 2 * UB) {
+            for (int i0 = 0; i0 < NT; i0 += 2 * UB) {
                 load_round(vb, i0 + UB);
                 update_round(va, i0);
                 load_round(va, i0 + 2 * UB);
                 update_round(vb, i0 + UB);
             }
-            for (int h = 0; h < np; ++h) {
-                const T *Gh = h ? G1 : G0;
-                const int kh = k0 + 16 * h, wh = h ? w1 : w0;
-                for (int ti = wv; ti < NT; ti += NWV) {
-                    const int r = 16 * ti + (lane & 15);
-                    if (r < n) {
+        }
+        for (int h = 0; h < np; ++h) {
+            const T *Gh = half(h);
+            const int kh = k0 + 16 * h, wh = width(h);
+            for (int ti = wv; ti < NT; ti += NWV) {
+                const int r = 16 * ti + (lane & 15);
+                if (r < n) {
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int cc = Mfma<T>::row(lane, q);
-                            if (cc < wh) S[(size_t)(kh + cc) * n + r] = Gh[r * GP + cc] + ((rowtag[r] - 1 - kh == cc) ? T(1) : T(0));
-                        }
+                    for (int q = 0; q < 4; ++q) {
+                        const int cc = Mfma<T>::row(lane, q);
+                        if (cc < wh) S[(size_t)(kh + cc) * n + r] = Gh[r * GP + cc] + ((rowtag[r] - 1 - kh == cc) ? T(1) : T(0));
                     }
                 }
             }
         }
         __syncthreads();
     }
-    // inv(rowstep(r), pivrow(c)) = S(r, c)
     for (int e = tid; e < nn; e += NTH) {
         const int c = e / n, r = e - c * n;
         M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
@@ -1289,38 +1176,28 @@ __device__ bool ginv_body_wide(float *__restrict__ M, float *__restrict__ S, int
     return true;
 }
 
-// Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body with diagonal pivots; a
-// block whose check fails is flagged for k_fac_pivot.  LDS is dynamic.
+// Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body_wide_rx with diagonal
+// pivots; a block whose check fails is flagged for k_fac_pivot.  LDS is dynamic.
 constexpr int kInvThreads = 512;   // global-memory inverse: 8 waves per block
+constexpr int kWideThreads = 1024;   // wide-panel inverse: 16 waves per block
 
 template <typename T, int NR, int UB>
-__global__ void __launch_bounds__(kInvThreads) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
+__global__ void __launch_bounds__(kWideThreads) k_fac_ginv(T *__restrict__ D, int64_t ld, T *__restrict__ Wk, int64_t ldw,
                                                    const int32_t *__restrict__ rows, int n,
                                                    int32_t *__restrict__ redo) {
     extern __shared__ unsigned char smem_raw[];
-    const int NP = 16 * ((n + 15) / 16), XPP = NP + 1;
+    const int NP = 16 * ((n + 15) / 16);
     __shared__ int32_t flag_sh;
-    __shared__ T prow2_sh[2][16];
+    __shared__ T prow2_sh[2][17];
     const int blk = rows[blockIdx.x];
     T *M = D + (size_t)blk * ld;
     T *S = Wk + (size_t)blockIdx.x * ldw;
-    bool ok;
-    if constexpr (sizeof(T) == 4) {   // 32-column panels (LDS: ginv_lds)
-        T *G = reinterpret_cast<T *>(smem_raw);             // 2 * NP * 17
-        T *XP = G + (size_t)2 * NP * 17;                    // 32 * XPP
-        T *PR = XP + (size_t)32 * XPP;                      // 256
-        int32_t *rowtag = reinterpret_cast<int32_t *>(PR + 256);   // NP
-        int32_t *pivrow = rowtag + NP;                      // NP
-        ok = ginv_body_wide<NR, kInvThreads, UB>(M, S, n, G, XP, PR, prow2_sh, rowtag, pivrow, &flag_sh);
-    } else {
-        T *G = reinterpret_cast<T *>(smem_raw);             // NP * 17
-        T *XP = G + (size_t)NP * 17;                        // 16 * XPP
-        T *prow_sh = XP + (size_t)16 * XPP;                 // 16
-        int32_t *rowtag = reinterpret_cast<int32_t *>(prow_sh + 16);   // NP
-        int32_t *pivrow = rowtag + NP;                      // NP
-        bool bad = false;
-        ok = ginv_body<T, NR, false, kInvThreads, UB>(M, S, n, G, XP, prow_sh, prow2_sh, rowtag, pivrow, &flag_sh, &bad);
-    }
+    // panels of GinvHalves<T>::H 16-column halves, pivot rows in registers (LDS: ginv_lds)
+    constexpr int H = GinvHalves<T>::H;
+    T *G = reinterpret_cast<T *>(smem_raw);                 // H * NP * 17
+    int32_t *rowtag = reinterpret_cast<int32_t *>(G + (size_t)H * NP * 17);   // NP
+    int32_t *pivrow = rowtag + NP;                          // NP
+    const bool ok = ginv_body_wide_rx<T, NR, kWideThreads, UB, H>(M, S, n, G, prow2_sh, rowtag, pivrow, &flag_sh);
     if (!ok && threadIdx.x == 0) redo[blk] = 1;    // left to k_fac_pivot; M untouched
 }
 
@@ -1544,8 +1421,8 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_pivot(T *__restrict__ D, in
     int32_t *pivrow = rowtag + NP;
     __shared__ int32_t flag_sh;
     bool bad = false;
-    ginv_body<T, NR, true, kInvThreads>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh,
-                                        nullptr, rowtag, pivrow, &flag_sh, &bad);
+    ginv_pivot_body<T, NR, kInvThreads>(D + (size_t)blk * ld, Wk + (size_t)blockIdx.x * ldw, n, G, XP, prow_sh,
+                                        rowtag, pivrow, &flag_sh, &bad);
     if (threadIdx.x == 0) {
         redo[blk] = 0;
         if (bad) atomicMin(err, blk);
@@ -1556,9 +1433,9 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_pivot(T *__restrict__ D, in
 template <typename T, int NR>
 size_t ginv_lds(int n) {
     const int NP = 16 * ((n + 15) / 16);
-    if (sizeof(T) == 4)
-        return sizeof(T) * ((size_t)2 * NP * 17 + (size_t)32 * (NP + 1) + 256) + sizeof(int32_t) * 2 * (size_t)NP;
-    return sizeof(T) * ((size_t)NP * 17 + (size_t)16 * (NP + 1) + 16) + sizeof(int32_t) * 2 * (size_t)NP;
+    const size_t pivot_form = sizeof(T) * ((size_t)NP * 17 + (size_t)16 * (NP + 1) + 16) + sizeof(int32_t) * 2 * (size_t)NP;
+    const size_t wide = sizeof(T) * (size_t)GinvHalves<T>::H * NP * 17 + sizeof(int32_t) * 2 * (size_t)NP;
+    return std::max(wide, pivot_form);
 }
 
 __device__ __forceinline__ cdg_bf16 to_bf16(float f) {
@@ -1612,11 +1489,11 @@ int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows
     // Both kernels may take the LDS of the largest block this template serves
     // (n <= 64 NR, at most 320): set once per process, thread-safe (request-loop
     // workers launch concurrently), whatever block size comes first.
-    // tiles each wave loads per round of the rank-16 update (two rounds' tiles
-    // are in registers at once); CDG_GINV_TILES = 2, 4 or 8 overrides 4
+    // tiles each wave loads per round of the rank-16H update (two rounds' tiles
+    // are in registers at once); CDG_GINV_TILES = 2, 4 or 8 overrides 2
     static const int ub = [] {
         const char *e = std::getenv("CDG_GINV_TILES");
-        const int v = e ? std::atoi(e) : 4;
+        const int v = e ? std::atoi(e) : 2;
         return v <= 2 ? 2 : v <= 4 ? 4 : 8;
     }();
     const void *fn = ub == 2 ? reinterpret_cast<const void *>(&k_fac_ginv<T, NR, 2>)
@@ -1643,7 +1520,7 @@ int launch_ginv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows
         int np = n;
         int32_t *redo = M->d_redo;
         void *args[] = {&Dp, &ldp, &Wp, &ldw, &rp, &np, &redo};
-        CDG_HIP_TRY(hipLaunchKernel(fn, dim3(nrows_lvl), dim3(kInvThreads), args, lds, ctx->stream));
+        CDG_HIP_TRY(hipLaunchKernel(fn, dim3(nrows_lvl), dim3(kWideThreads), args, lds, ctx->stream));
     }
     CDG_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL((k_fac_pivot<T, NR>), dim3(nrows_lvl), dim3(kInvThreads), lds, ctx->stream, D, ld,

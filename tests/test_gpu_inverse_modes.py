@@ -61,7 +61,7 @@ print("OK", flush=True)
 '''.replace("ROOT", repr(ROOT))
 
 
-@pytest.mark.parametrize("env", [{"CDG_INV": "2"}, {"CDG_INV": "2", "CDG_GINV_TILES": "2"},
+@pytest.mark.parametrize("env", [{"CDG_INV": "2"}, {"CDG_INV": "2", "CDG_GINV_TILES": "4"},
                                  {"CDG_INV": "2", "CDG_GINV_TILES": "8"}, {"CDG_INV": "0"}])
 def test_inverse_kernels_vs_oracle(env):
     e = dict(os.environ, **env)
