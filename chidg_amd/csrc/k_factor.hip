@@ -1022,8 +1022,11 @@ __device__ void ginv_pivot_body(T *__restrict__ M, T *__restrict__ S, int n, T *
 // registers, so no other wave can have changed them (the column's tiles, its
 // pivot-row tiles among them, are only this wave's).  Fewer, wider panels
 // mean fewer passes over the slot, whose traffic sets the inverse's time;
-// with the pivot rows out of LDS, the halves alone take it.  LDS (ginv_lds):
-// H halves of NP x 17, rowtag, pivrow.
+// with the pivot rows out of LDS, the halves alone take it.  The first panel
+// reads M and the last writes M (the slot S holds the passes between), so
+// neither a copy of M nor a copy back is needed; the result is inv(M) with
+// no permutation, as every pivot is diagonal.  LDS (ginv_lds): H halves of
+// NP x 17, rowtag, pivrow.
 template <typename T> struct GinvHalves { static constexpr int H = sizeof(T) == 4 ? 4 : 3; };
 template <typename T, int NR, int NTH, int UB, int H>
 __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T *G, T (*prow2_sh)[17],
@@ -1035,11 +1038,12 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nn = n * n;
     auto half = [&](int h) -> T * { return G + (size_t)h * NP * GP; };
-    for (int e = tid; e < nn; e += NTH) S[e] = M[e];
     for (int i = tid; i < NP; i += NTH) { rowtag[i] = 0; pivrow[i] = 0; }
     if (tid == 0) *flag_sh = 0;
     __syncthreads();
-    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(S, nn * (int)sizeof(T));
+    // the first panel reads the block, the last one writes it, the others work in the
+    // slot (M stays as it was until the last panel's pivots have passed their check)
+    const __amdgpu_buffer_rsrc_t rsM = slot_rsrc(M, nn * (int)sizeof(T)), rsS = slot_rsrc(S, nn * (int)sizeof(T));
     constexpr int PL0 = (PW * 64 * NR + NTH - 1) / NTH;   // panel values per thread
     constexpr int PB = PL0 < 10 ? PL0 : 10;   // loaded per batch (registers)
     constexpr int PL = (PL0 + PB - 1) / PB * PB;
@@ -1047,6 +1051,7 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
         const int k0 = 16 * kb;
         const int np = min(H, NT - kb);                         // halves (tile columns) of this panel
         auto width = [&](int h) { return h < np ? min(16, n - k0 - 16 * h) : 0; };
+        const __amdgpu_buffer_rsrc_t src = kb == 0 ? rsM : rsS, dst = kb + H >= NT ? rsM : rsS;
         // 1. the halves to LDS (zero beyond n and beyond each half's width)
 #pragma unroll 1
         for (int i0 = 0; i0 < PL; i0 += PB) {
@@ -1055,7 +1060,7 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
             for (int i = 0; i < PB; ++i) {
                 const int e = tid + (i0 + i) * NTH, c = e / NP, r = e - c * NP;
                 const bool ok = e < NP * PW && r < n && (c & 15) < width(c >> 4);
-                v[i] = bload<T>(rs, ok ? (uint32_t)(((k0 + c) * n + r) * (int)sizeof(T)) : kOob);
+                v[i] = bload<T>(src, ok ? (uint32_t)(((k0 + c) * n + r) * (int)sizeof(T)) : kOob);
             }
 #pragma unroll
             for (int i = 0; i < PB; ++i) {
@@ -1113,7 +1118,7 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
             for (int kk = 0; kk < 4 * H; ++kk) {
                 const int h = kk >> 2, j = 4 * (kk & 3) + (lane >> 4);
                 const bool ok = h < np && j < width(h) && c < n;
-                a[kk] = bload<T>(rs, ok ? (uint32_t)((c * n + pivrow[min(k0 + 16 * h + j, NP - 1)]) * (int)sizeof(T)) : kOob);
+                a[kk] = bload<T>(src, ok ? (uint32_t)((c * n + pivrow[min(k0 + 16 * h + j, NP - 1)]) * (int)sizeof(T)) : kOob);
             }
             auto offs = [&](int ti, int q) -> uint32_t {
                 const int r = 16 * ti + (lane & 15), cq = 16 * tj + Mfma<T>::row(lane, q);
@@ -1123,7 +1128,7 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
 #pragma unroll
                 for (int u = 0; u < UB; ++u)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) v[u][q] = bload<T>(rs, offs(i0 + u, q));
+                    for (int q = 0; q < 4; ++q) v[u][q] = bload<T>(src, offs(i0 + u, q));
             };
             auto update_round = [&](const T (&v)[UB][4], int i0) {
 #pragma unroll
@@ -1141,7 +1146,7 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
                         }
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) bstore<T>(acc[q], rs, offs(i0 + u, q));
+                    for (int q = 0; q < 4; ++q) bstore<T>(acc[q], dst, offs(i0 + u, q));
                 }
             };
             T va[UB][4], vb[UB][4];
@@ -1162,18 +1167,16 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int cc = Mfma<T>::row(lane, q);
-                        if (cc < wh) S[(size_t)(kh + cc) * n + r] = Gh[r * GP + cc] + ((rowtag[r] - 1 - kh == cc) ? T(1) : T(0));
+                        if (cc < wh)
+                            bstore<T>(Gh[r * GP + cc] + ((rowtag[r] - 1 - kh == cc) ? T(1) : T(0)), dst,
+                                      (uint32_t)(((kh + cc) * n + r) * (int)sizeof(T)));
                     }
                 }
             }
         }
         __syncthreads();
     }
-    for (int e = tid; e < nn; e += NTH) {
-        const int c = e / n, r = e - c * n;
-        M[(size_t)pivrow[c] * n + (rowtag[r] - 1)] = S[e];
-    }
-    return true;
+    return true;   // every pivot was diagonal: the last panel left inv(M) in place
 }
 
 // Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body_wide_rx with diagonal
