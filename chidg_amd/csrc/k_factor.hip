@@ -1107,22 +1107,35 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
             }
             __syncthreads();
         }
-        // 3. X += (C - E_P) X(P, :), tile column by tile column (transposed tiles, branch-free,
-        //    the next round's loads before this round's stores), then the panel's tiles take C
+        // 3. X += (C - E_P) X(P, :) on the tiles outside the panel (transposed tiles,
+        //    branch-free, the next round's loads before this round's stores), then the
+        //    panel's tiles take C.  A wave's work item is a tile column, or half of one
+        //    when the columns do not divide evenly among the waves (17 columns on 16
+        //    waves: three rounds of halves instead of two of columns).  Both waves of a
+        //    split column read its pivot rows before either changes them (one barrier per
+        //    round; a column's two halves are items 2j, 2j + 1 of one round).
         const int ncol = NT - np;
-        for (int tjp = wv; tjp < ncol; tjp += NWV) {
+        const int split = ncol % NWV != 0 ? 2 : 1;
+        const int nh = split == 2 ? (NT + 1) / 2 : NT;     // tiles in a column's first half
+        for (int it0 = 0; it0 < split * ncol; it0 += NWV) {
+            const int item = it0 + wv;
+            const bool act = item < split * ncol;           // uniform per wave
+            const int tjp = min(item, split * ncol - 1) / split, part = item - tjp * split;
             const int tj = tjp + (tjp >= kb ? np : 0);
+            const int lo = part == 1 ? nh : 0, hi = part == 1 || split == 1 ? NT : nh;
             const int c = 16 * tj + (lane & 15);
             T a[4 * H];   // X(P, c): pivot row 4 kk + lane / 16 of the panel, this lane's column
 #pragma unroll
             for (int kk = 0; kk < 4 * H; ++kk) {
                 const int h = kk >> 2, j = 4 * (kk & 3) + (lane >> 4);
-                const bool ok = h < np && j < width(h) && c < n;
+                const bool ok = act && h < np && j < width(h) && c < n;
                 a[kk] = bload<T>(src, ok ? (uint32_t)((c * n + pivrow[min(k0 + 16 * h + j, NP - 1)]) * (int)sizeof(T)) : kOob);
             }
+            if (split == 2) __syncthreads();
+            if (!act) continue;
             auto offs = [&](int ti, int q) -> uint32_t {
                 const int r = 16 * ti + (lane & 15), cq = 16 * tj + Mfma<T>::row(lane, q);
-                return (ti < NT && r < n && cq < n) ? (uint32_t)((cq * n + r) * (int)sizeof(T)) : kOob;
+                return (ti < hi && r < n && cq < n) ? (uint32_t)((cq * n + r) * (int)sizeof(T)) : kOob;
             };
             auto load_round = [&](T (&v)[UB][4], int i0) {
 #pragma unroll
@@ -1150,12 +1163,12 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
                 }
             };
             T va[UB][4], vb[UB][4];
-            load_round(va, 0);
-            for (int i0 = 0; i0 < NT; i0 += 2 * UB) {
+            load_round(va, lo);
+            for (int i0 = lo; i0 < hi; i0 += 2 * UB) {
                 load_round(vb, i0 + UB);
                 update_round(va, i0);
                 load_round(va, i0 + 2 * UB);
-                update_round(vb, i0 + UB);
+                if (i0 + UB < hi) update_round(vb, i0 + UB);
             }
         }
         for (int h = 0; h < np; ++h) {
