@@ -42,27 +42,30 @@ template <> struct Mfma<float> {
     __device__ static __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int TM = 48;       // output tile edge
 constexpr int KC = 16;       // K chunk staged in LDS
-constexpr int GW = 3;        // waves per workgroup
-constexpr int GT = 64 * GW;  // threads per workgroup
 
-// Batched block GEMM.  A workgroup (3 waves) computes a 48 x 48 tile of one
-// output block, acc = sum_p A_p B_p over the concatenated K of its operand
-// pairs (A_p, B_p column-major n x n); wave w holds rows 16w..16w+15 (three
-// 16 x 16 MFMA tiles).  K advances in chunks of KC through two LDS buffers;
-// the global fetch of chunk c + 2 goes out before chunk c's products, so a
-// fetch has two chunks of products to arrive.
-constexpr int LA = KC * TM / GT;    // A elements per thread and chunk (4)
-constexpr int LB = KC * TM / GT;    // B elements per thread and chunk (4)
+// Batched block GEMM.  A workgroup of TM / 16 waves computes a TM x TM tile
+// (TM = 48 or 64) of one output block, acc = sum_p A_p B_p over the
+// concatenated K of its operand pairs (A_p, B_p column-major n x n); wave w
+// holds rows 16w..16w+15 (TM / 16 MFMA tiles of 16 x 16).  K advances in
+// chunks of KC through two LDS buffers; the global fetch of chunk c + 2 goes
+// out before chunk c's products, so a fetch has two chunks of products to
+// arrive.  The 64-edge tile serves n > 192 (320 = 5 x 64: no padding, and 16
+// products per wave between barriers instead of 12).
+template <int TM> struct Geo {
+    static constexpr int GW = TM / 16;   // waves per workgroup
+    static constexpr int GT = 64 * GW;   // threads per workgroup
+};
+constexpr int LA = KC / 4;    // A elements per thread and chunk (KC * TM / GT)
+constexpr int LB = KC / 4;    // B elements per thread and chunk
 // LDS tiles are k-contiguous: As[r][k] = A(r0 + r, kc + k), Bs[c][k] =
 // B(kc + k, c0 + c), pitch KP (16-byte rows; 20 floats spread the transposing
 // A stores over the banks).  An MFMA sums over k, so lane l takes the
 // contiguous k range (KC/4)(l/16) .. +KC/4 for both operands: one 16-byte LDS
 // read per operand and step pair instead of one read per product.
 template <typename T> struct KPitch { static constexpr int P = KC + 16 / (int)sizeof(T); };
-template <typename T> using ATile = T[TM][KPitch<T>::P];
-template <typename T> using BTile = T[TM][KPitch<T>::P];
+template <typename T, int TM> using ATile = T[TM][KPitch<T>::P];
+template <typename T, int TM> using BTile = T[TM][KPitch<T>::P];
 
 // a 32-bit element offset from a uniform base (one VGPR per address)
 template <typename S> __device__ __forceinline__ S ld_off(const S *base, int off) {
@@ -73,9 +76,10 @@ template <typename S> __device__ __forceinline__ S ld_off(const S *base, int off
 // addresses with no conditions, so nothing waits for them here; conversion to
 // T and the zeros outside n happen when the chunk is stored to LDS.  (A
 // conversion at fetch time makes every fetch wait for its own loads.)
-template <typename SA, typename SB>
+template <typename SA, typename SB, int TM>
 __device__ __forceinline__ void gemm_fetch(const SA *__restrict__ A, const SB *__restrict__ B, int n, int r0, int c0,
                                            int kc, SA (&ra)[LA], SB (&rb)[LB]) {
+    constexpr int GT = Geo<TM>::GT;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
@@ -91,9 +95,10 @@ __device__ __forceinline__ void gemm_fetch(const SA *__restrict__ A, const SB *_
     }
 }
 
-template <typename T, typename SA, typename SB>
-__device__ __forceinline__ void gemm_store(ATile<T> &As, BTile<T> &Bs, const SA (&ra)[LA], const SB (&rb)[LB], int n,
-                                           int r0, int c0, int kc) {
+template <typename T, typename SA, typename SB, int TM>
+__device__ __forceinline__ void gemm_store(ATile<T, TM> &As, BTile<T, TM> &Bs, const SA (&ra)[LA], const SB (&rb)[LB],
+                                           int n, int r0, int c0, int kc) {
+    constexpr int GT = Geo<TM>::GT;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
@@ -109,26 +114,28 @@ __device__ __forceinline__ void gemm_store(ATile<T> &As, BTile<T> &Bs, const SA 
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void gemm_chunk(const ATile<T> &As, const BTile<T> &Bs, typename Mfma<T>::acc_t (&acc)[3]) {
+template <typename T, int TM>
+__device__ __forceinline__ void gemm_chunk(const ATile<T, TM> &As, const BTile<T, TM> &Bs,
+                                           typename Mfma<T>::acc_t (&acc)[TM / 16]) {
+    constexpr int NJ = TM / 16;
     constexpr int KS = KC / 4;                   // k values per lane and chunk
     typedef T vec_t __attribute__((ext_vector_type(KS)));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k0 = KS * (lane >> 4);
     const vec_t a = *reinterpret_cast<const vec_t *>(&As[16 * wv + (lane & 15)][k0]);
-    vec_t b[3];
+    vec_t b[NJ];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) b[j] = *reinterpret_cast<const vec_t *>(&Bs[16 * j + (lane & 15)][k0]);
+    for (int j = 0; j < NJ; ++j) b[j] = *reinterpret_cast<const vec_t *>(&Bs[16 * j + (lane & 15)][k0]);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) acc[j] = Mfma<T>::run(a[s], b[j][s], acc[j]);
+        for (int j = 0; j < NJ; ++j) acc[j] = Mfma<T>::run(a[s], b[j][s], acc[j]);
 }
 
 // acc = sum over the npairs operand pairs, pair(p, A, B) naming pair p.
-template <typename T, typename SA, typename SB, typename Pair>
-__device__ __forceinline__ void gemm_pairs(Pair pair, int npairs, int n, int r0, int c0, ATile<T> (&As)[2],
-                                           BTile<T> (&Bs)[2], typename Mfma<T>::acc_t (&acc)[3]) {
+template <typename T, typename SA, typename SB, int TM, typename Pair>
+__device__ __forceinline__ void gemm_pairs(Pair pair, int npairs, int n, int r0, int c0, ATile<T, TM> (&As)[2],
+                                           BTile<T, TM> (&Bs)[2], typename Mfma<T>::acc_t (&acc)[TM / 16]) {
     // Branch-free pipeline over an even number of chunks: fetches past the
     // last chunk reload it (in bounds) and their stores write zeros, so the
     // compiler's wait counts see loads in issue order (a conditional fetch
@@ -142,12 +149,12 @@ __device__ __forceinline__ void gemm_pairs(Pair pair, int npairs, int n, int r0,
         const SA *A;
         const SB *B;
         pair(p, A, B);
-        gemm_fetch<SA, SB>(A, B, n, r0, c0, (c - p * cpb) * KC, ra, rb);
+        gemm_fetch<SA, SB, TM>(A, B, n, r0, c0, (c - p * cpb) * KC, ra, rb);
     };
     auto store = [&](int c, int buf, const SA (&ra)[LA], const SB (&rb)[LB]) {
         // chunk c >= nch: kc = n stores zeros
         const int kc = c < nch ? (c - (c / cpb) * cpb) * KC : n;
-        gemm_store<T, SA, SB>(As[buf], Bs[buf], ra, rb, n, r0, c0, kc);
+        gemm_store<T, SA, SB, TM>(As[buf], Bs[buf], ra, rb, n, r0, c0, kc);
     };
     fetch(0, ra0, rb0);
     fetch(1, ra1, rb1);
@@ -155,11 +162,11 @@ __device__ __forceinline__ void gemm_pairs(Pair pair, int npairs, int n, int r0,
     __syncthreads();
     for (int c = 0; c < nch2; c += 2) {
         fetch(c + 2, ra0, rb0);
-        gemm_chunk<T>(As[0], Bs[0], acc);
+        gemm_chunk<T, TM>(As[0], Bs[0], acc);
         store(c + 1, 1, ra1, rb1);
         __syncthreads();
         fetch(c + 3, ra1, rb1);
-        gemm_chunk<T>(As[1], Bs[1], acc);
+        gemm_chunk<T, TM>(As[1], Bs[1], acc);
         store(c + 2, 0, ra0, rb0);
         __syncthreads();
     }
@@ -178,11 +185,11 @@ __device__ __forceinline__ bool xcd_job(int njobs, int tiles2, int &job, int &ti
 }
 inline int xcd_grid(int njobs, int tiles2) { return 8 * tiles2 * ((njobs + 7) / 8); }
 
-template <typename T, typename F>
-__device__ __forceinline__ void gemm_out(int tm, int tn, int n, typename Mfma<T>::acc_t (&acc)[3], F put) {
+template <typename T, int TM, typename F>
+__device__ __forceinline__ void gemm_out(int tm, int tn, int n, typename Mfma<T>::acc_t (&acc)[TM / 16], F put) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < TM / 16; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = tm * TM + 16 * wv + Mfma<T>::row(lane, r);
@@ -194,47 +201,47 @@ __device__ __forceinline__ void gemm_out(int tm, int tn, int n, typename Mfma<T>
 // phase 1: L_q = A_{lo_blk[q]} * Dinv_{lo_col[q]} (with the upper lists in
 // place of the lower ones: W_u = A_{up_blk[u]} * Dinv_{up_col[u]}); the A
 // operand is A's own FP64 blocks or, for the FP32 fold, their FP32 copies
-template <typename T, typename SA = double>
-__global__ void __launch_bounds__(GT)
+template <typename T, typename SA, int TM>
+__global__ void __launch_bounds__(Geo<TM>::GT)
 k_fac_p1(const SA *__restrict__ Ab, int64_t lda, T *__restrict__ L, const T *__restrict__ D, int64_t ld,
          const int32_t *__restrict__ jobs, int njobs, const int32_t *__restrict__ lo_blk,
          const int32_t *__restrict__ lo_col, int n, int tiles) {
-    __shared__ __attribute__((aligned(16))) ATile<T> As[2];
-    __shared__ __attribute__((aligned(16))) BTile<T> Bs[2];
+    __shared__ __attribute__((aligned(16))) ATile<T, TM> As[2];
+    __shared__ __attribute__((aligned(16))) BTile<T, TM> Bs[2];
     int job, tile;
     if (!xcd_job(njobs, tiles * tiles, job, tile)) return;
     const int q = jobs[job];
     const int tm = tile % tiles, tn = tile / tiles;
-    typename Mfma<T>::acc_t acc[3];
+    typename Mfma<T>::acc_t acc[TM / 16];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TM / 16; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     const SA *A = Ab + (size_t)lo_blk[q] * lda;
     const T *B = D + (size_t)lo_col[q] * ld;
-    gemm_pairs<T, SA, T>([&](int, const SA *&a, const T *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As, Bs,
-                         acc);
+    gemm_pairs<T, SA, T, TM>([&](int, const SA *&a, const T *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As, Bs,
+                             acc);
     T *out = L + (size_t)q * ld;
-    gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) { out[(size_t)col * n + row] = v; });
+    gemm_out<T, TM>(tm, tn, n, acc, [&](int row, int col, T v) { out[(size_t)col * n + row] = v; });
 }
 
 // phase 2: D_i = A_{diag[i]} - sum_q L_q * U_q, U_q = A_{lo_tr[q]} read from
 // Ub (stride ldu): A's own blocks (FP64), or their FP32 copies.
-template <typename T, typename SB>
-__global__ void __launch_bounds__(GT)
+template <typename T, typename SB, int TM>
+__global__ void __launch_bounds__(Geo<TM>::GT)
 k_fac_p2(const double *__restrict__ Ab, int64_t lda, const SB *__restrict__ Ub, int64_t ldu, const T *__restrict__ L,
          T *__restrict__ D, int64_t ld, const int32_t *__restrict__ rows, int njobs, const int32_t *__restrict__ lo_ptr,
          const int32_t *__restrict__ lo_tr, const int32_t *__restrict__ diag, int n, int tiles) {
-    __shared__ __attribute__((aligned(16))) ATile<T> As[2];
-    __shared__ __attribute__((aligned(16))) BTile<T> Bs[2];
+    __shared__ __attribute__((aligned(16))) ATile<T, TM> As[2];
+    __shared__ __attribute__((aligned(16))) BTile<T, TM> Bs[2];
     int job, tile;
     if (!xcd_job(njobs, tiles * tiles, job, tile)) return;
     const int i = rows[job];
     const int tm = tile % tiles, tn = tile / tiles;
-    typename Mfma<T>::acc_t acc[3];
+    typename Mfma<T>::acc_t acc[TM / 16];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TM / 16; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     const int q0 = lo_ptr[i], nq = lo_ptr[i + 1] - q0;
     if (nq > 0)
-        gemm_pairs<T, T, SB>(
+        gemm_pairs<T, T, SB, TM>(
             [&](int p, const T *&a, const SB *&b) {
                 a = L + (size_t)(q0 + p) * ld;
                 b = Ub + (size_t)lo_tr[q0 + p] * ldu;
@@ -242,7 +249,7 @@ k_fac_p2(const double *__restrict__ Ab, int64_t lda, const SB *__restrict__ Ub, 
             nq, n, tm * TM, tn * TM, As, Bs, acc);
     const double *a = Ab + (size_t)diag[i] * lda;
     T *out = D + (size_t)i * ld;
-    gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) {
+    gemm_out<T, TM>(tm, tn, n, acc, [&](int row, int col, T v) {
         out[(size_t)col * n + row] = static_cast<T>(a[(size_t)col * n + row]) - v;
     });
 }
@@ -262,25 +269,26 @@ template <> __device__ __forceinline__ cdg_bf16 fold_store<cdg_bf16>(double v) {
 // the row-slice layout of the stream schedule: row r of the block goes to
 // slice q = r / w at element q*SL + col*w + (r - q*w) of the slot.  U_u is
 // read from Ub slot up_src[u] (A's FP64 blocks, or their FP32 copies).
-template <typename T, typename SB, typename SO>
-__global__ void __launch_bounds__(GT)
+template <typename T, typename SB, typename SO, int TM>
+__global__ void __launch_bounds__(Geo<TM>::GT)
 k_fold_rs(const T *__restrict__ D, int64_t ld, const SB *__restrict__ Ub, int64_t ldu, const int32_t *__restrict__ up_row,
           const int32_t *__restrict__ up_src, int nup, SO *__restrict__ W, int64_t slot, int w, int SL, int n,
           int tiles) {
-    __shared__ __attribute__((aligned(16))) ATile<T> As[2];
-    __shared__ __attribute__((aligned(16))) BTile<T> Bs[2];
+    __shared__ __attribute__((aligned(16))) ATile<T, TM> As[2];
+    __shared__ __attribute__((aligned(16))) BTile<T, TM> Bs[2];
     int job, tile;
     if (!xcd_job(nup, tiles * tiles, job, tile)) return;
     const int u = job;
     const int tm = tile % tiles, tn = tile / tiles;
-    typename Mfma<T>::acc_t acc[3];
+    typename Mfma<T>::acc_t acc[TM / 16];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TM / 16; ++j) acc[j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     const T *A = D + (size_t)up_row[u] * ld;
     const SB *B = Ub + (size_t)up_src[u] * ldu;
-    gemm_pairs<T, T, SB>([&](int, const T *&a, const SB *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As, Bs, acc);
+    gemm_pairs<T, T, SB, TM>([&](int, const T *&a, const SB *&b) { a = A; b = B; }, 1, n, tm * TM, tn * TM, As, Bs,
+                             acc);
     SO *out = W + (size_t)u * slot;
-    gemm_out<T>(tm, tn, n, acc, [&](int row, int col, T v) {
+    gemm_out<T, TM>(tm, tn, n, acc, [&](int row, int col, T v) {
         const int q = row / w;
         out[(size_t)q * SL + (size_t)col * w + (row - q * w)] = fold_store<SO>(static_cast<double>(v));
     });
@@ -1593,21 +1601,57 @@ int launch_inv(cdg_ctx *ctx, cdg_prec *M, T *D, int64_t ld, const int32_t *rows,
     return CDG_OK;
 }
 
-template <typename T>
-int fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const T *D) {
+// Tile edge of the factor GEMMs: 64 above n = 192 (320 = 5 x 64), else 48
+// (n = 135: 3 x 48 = 144, where 64-edge tiles would pad to 192).
+inline bool gemm_wide(int n) { return n > 192; }
+
+template <typename T, int TM>
+int fold_all_tm(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const T *D) {
     const int n = M->n, tiles = (n + TM - 1) / TM;
     const int u1 = M->nup;
-    if (u1 == 0) return CDG_OK;
+    const dim3 blk(Geo<TM>::GT);
     if (std::is_same<T, float>::value && M->precision == CDG_FP32 && M->U)
         // the upper blocks' FP32 copies (slot u; the values the FP64 reads convert to): half the bytes
-        hipLaunchKernelGGL((k_fac_p1<T, float>), dim3(xcd_grid(u1, tiles * tiles)), dim3(GT), 0, ctx->stream,
+        hipLaunchKernelGGL((k_fac_p1<T, float, TM>), dim3(xcd_grid(u1, tiles * tiles)), blk, 0, ctx->stream,
                            static_cast<const float *>(M->U), M->ld, static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1,
                            M->d_up_all, M->d_up_col, n, tiles);
     else
-        hipLaunchKernelGGL((k_fac_p1<T>), dim3(xcd_grid(u1, tiles * tiles)), dim3(GT), 0, ctx->stream, A->blocks,
-                           A->ld, static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1, M->d_up_blk, M->d_up_col, n,
-                           tiles);
+        hipLaunchKernelGGL((k_fac_p1<T, double, TM>), dim3(xcd_grid(u1, tiles * tiles)), blk, 0, ctx->stream,
+                           A->blocks, A->ld, static_cast<T *>(M->W), D, M->ld, M->d_up_all, u1, M->d_up_blk,
+                           M->d_up_col, n, tiles);
     CDG_HIP_TRY(hipGetLastError());
+    return CDG_OK;
+}
+
+template <typename T>
+int fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, const T *D) {
+    if (M->nup == 0) return CDG_OK;
+    return gemm_wide(M->n) ? fold_all_tm<T, 64>(ctx, M, A, D) : fold_all_tm<T, 48>(ctx, M, A, D);
+}
+
+// One level's GEMMs: L for the level's lower entries, then the rows' D.
+template <typename T, int TM>
+int gemm_level(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D, const float *Ub32, int q0, int q1, int r0,
+               int r1) {
+    const int n = M->n, tiles = (n + TM - 1) / TM;
+    const dim3 blk(Geo<TM>::GT);
+    if (q1 > q0) {
+        hipLaunchKernelGGL((k_fac_p1<T, double, TM>), dim3(xcd_grid(q1 - q0, tiles * tiles)), blk, 0, ctx->stream,
+                           A->blocks, A->ld, L, D, M->ld, M->d_lev_lo + q0, q1 - q0, M->d_lo_blk, M->d_lo_col, n,
+                           tiles);
+        CDG_HIP_TRY(hipGetLastError());
+    }
+    if (r1 > r0) {
+        if (Ub32)   // FP32 factor: the upper blocks' FP32 copies (the same values the FP64 reads convert to)
+            hipLaunchKernelGGL((k_fac_p2<T, float, TM>), dim3(xcd_grid(r1 - r0, tiles * tiles)), blk, 0,
+                               ctx->stream, A->blocks, A->ld, Ub32, M->ld, L, D, M->ld, M->d_lev_rows + r0, r1 - r0,
+                               M->d_lo_ptr, M->d_lo_tru, M->d_diag, n, tiles);
+        else
+            hipLaunchKernelGGL((k_fac_p2<T, double, TM>), dim3(xcd_grid(r1 - r0, tiles * tiles)), blk, 0,
+                               ctx->stream, A->blocks, A->ld, A->blocks, A->ld, L, D, M->ld, M->d_lev_rows + r0,
+                               r1 - r0, M->d_lo_ptr, M->d_lo_tr, M->d_diag, n, tiles);
+        CDG_HIP_TRY(hipGetLastError());
+    }
     return CDG_OK;
 }
 
@@ -1615,27 +1659,14 @@ template <typename T>
 int factor_levels(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, T *L, T *D, const float *Ub32) {
     const int n = M->n;
     const bool fold = M->apply_mode == CDG_APPLY_FOLDED && M->W && M->precision != CDG_BF16;
-    const int tiles = (n + TM - 1) / TM;
     for (int l = 0; l < M->nlev; ++l) {
         const int q0 = M->lev_lo_ptr[l], q1 = M->lev_lo_ptr[l + 1];
         const int r0 = M->lev_row_ptr[l], r1 = M->lev_row_ptr[l + 1];
-        if (q1 > q0) {
-            hipLaunchKernelGGL((k_fac_p1<T>), dim3(xcd_grid(q1 - q0, tiles * tiles)), dim3(GT), 0, ctx->stream,
-                               A->blocks, A->ld, L, D, M->ld, M->d_lev_lo + q0, q1 - q0, M->d_lo_blk, M->d_lo_col, n,
-                               tiles);
-            CDG_HIP_TRY(hipGetLastError());
-        }
+        int e = gemm_wide(n) ? gemm_level<T, 64>(ctx, M, A, L, D, Ub32, q0, q1, r0, r1)
+                             : gemm_level<T, 48>(ctx, M, A, L, D, Ub32, q0, q1, r0, r1);
+        if (e) return e;
         if (r1 > r0) {
-            if (Ub32)   // FP32 factor: the upper blocks' FP32 copies (the same values the FP64 reads convert to)
-                hipLaunchKernelGGL((k_fac_p2<T, float>), dim3(xcd_grid(r1 - r0, tiles * tiles)), dim3(GT), 0,
-                                   ctx->stream, A->blocks, A->ld, Ub32, M->ld, L, D, M->ld, M->d_lev_rows + r0, r1 - r0,
-                                   M->d_lo_ptr, M->d_lo_tru, M->d_diag, n, tiles);
-            else
-                hipLaunchKernelGGL((k_fac_p2<T, double>), dim3(xcd_grid(r1 - r0, tiles * tiles)), dim3(GT), 0,
-                                   ctx->stream, A->blocks, A->ld, A->blocks, A->ld, L, D, M->ld, M->d_lev_rows + r0,
-                                   r1 - r0, M->d_lo_ptr, M->d_lo_tr, M->d_diag, n, tiles);
-            CDG_HIP_TRY(hipGetLastError());
-            int e = launch_inv<T>(ctx, M, D, M->ld, M->d_lev_rows + r0, r1 - r0, n, M->d_err);
+            e = launch_inv<T>(ctx, M, D, M->ld, M->d_lev_rows + r0, r1 - r0, n, M->d_err);
             if (e) return e;
         }
     }
@@ -1715,25 +1746,30 @@ int launch_ilu_factor(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A, int32_t *bad_
 // FP32: the upper blocks' FP32 copies and Dinv; BF16: A's FP64 blocks (their
 // values rounded to FP32 at the LDS store) and the FP32 working Dinv, the
 // product stored BF16.
-int launch_fold_rs(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
+template <int TM>
+int launch_fold_rs_tm(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
     const int n = M->n, tiles = (n + TM - 1) / TM, nup = M->nup;
-    if (nup == 0) return CDG_OK;
-    const dim3 grid(xcd_grid(nup, tiles * tiles)), blk(GT);
+    const dim3 grid(xcd_grid(nup, tiles * tiles)), blk(Geo<TM>::GT);
     const int w = M->rs_w, SL = (int)M->rs_SL;
     if (M->precision == CDG_FP64)
-        hipLaunchKernelGGL((k_fold_rs<double, double, double>), grid, blk, 0, ctx->stream,
+        hipLaunchKernelGGL((k_fold_rs<double, double, double, TM>), grid, blk, 0, ctx->stream,
                            static_cast<const double *>(M->Dinv), M->ld, A->blocks, A->ld, M->d_up_row, M->d_up_blk, nup,
                            static_cast<double *>(M->Wrs), M->rs_slot, w, SL, n, tiles);
     else if (M->precision == CDG_FP32)
-        hipLaunchKernelGGL((k_fold_rs<float, float, float>), grid, blk, 0, ctx->stream,
+        hipLaunchKernelGGL((k_fold_rs<float, float, float, TM>), grid, blk, 0, ctx->stream,
                            static_cast<const float *>(M->Dinv), M->ld, static_cast<const float *>(M->U), M->ld,
                            M->d_up_row, M->d_up_all, nup, static_cast<float *>(M->Wrs), M->rs_slot, w, SL, n, tiles);
     else
-        hipLaunchKernelGGL((k_fold_rs<float, double, cdg_bf16>), grid, blk, 0, ctx->stream, M->D32, M->ld, A->blocks,
-                           A->ld, M->d_up_row, M->d_up_blk, nup, static_cast<cdg_bf16 *>(M->Wrs), M->rs_slot, w, SL, n,
-                           tiles);
+        hipLaunchKernelGGL((k_fold_rs<float, double, cdg_bf16, TM>), grid, blk, 0, ctx->stream, M->D32, M->ld,
+                           A->blocks, A->ld, M->d_up_row, M->d_up_blk, nup, static_cast<cdg_bf16 *>(M->Wrs), M->rs_slot,
+                           w, SL, n, tiles);
     CDG_HIP_TRY(hipGetLastError());
     return CDG_OK;
+}
+
+int launch_fold_rs(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
+    if (M->nup == 0) return CDG_OK;
+    return gemm_wide(M->n) ? launch_fold_rs_tm<64>(ctx, M, A) : launch_fold_rs_tm<48>(ctx, M, A);
 }
 
 int launch_fold_all(cdg_ctx *ctx, cdg_prec *M, const cdg_mat *A) {
