@@ -1202,7 +1202,7 @@ __device__ bool ginv_body_wide_rx(T *__restrict__ M, T *__restrict__ S, int n, T
 
 // Dinv_i = inv(D_i) for any n <= 64*NR: ginv_body_wide_rx with diagonal
 // pivots; a block whose check fails is flagged for k_fac_pivot.  LDS is dynamic.
-constexpr int kInvThreads = 512;   // global-memory inverse: 8 waves per block
+constexpr int kInvThreads = 512;    // partial-pivoting inverse (k_fac_pivot): 8 waves per block
 constexpr int kWideThreads = 1024;   // wide-panel inverse: 16 waves per block
 
 template <typename T, int NR, int UB>
@@ -1453,7 +1453,8 @@ __global__ void __launch_bounds__(kInvThreads) k_fac_pivot(T *__restrict__ D, in
     }
 }
 
-// Dynamic LDS of k_fac_ginv (FP32: wide panels) and k_fac_pivot (16-column: less)
+// Dynamic LDS of k_fac_ginv (the wide panel's halves) and k_fac_pivot (one
+// 16-column panel and its pivot rows); both kernels get the larger.
 template <typename T, int NR>
 size_t ginv_lds(int n) {
     const int NP = 16 * ((n + 15) / 16);
