@@ -1,11 +1,11 @@
 """The block-inverse kernels are chosen by block size (and CDG_INV, read once
 per process): the global-memory blocked inverse serves n > 192 by default and
 any n under CDG_INV=2.  This runs the factor and apply in a child process per
-setting at block sizes that are not multiples of 16 (n = 5, 40, 135: partial
-last panels, lanes outside the block) and that are (n = 64, 192), with
-diagonal blocks that need row
-pivots on every other row, against the oracle (precon_ILU0.f90:111-265,
-mod_inv.f90:99-150)."""
+setting at block sizes that are not multiples of 16 (n = 5, 40, 135, 243:
+partial last panels, lanes outside the block) and that are (n = 64, 192), with
+diagonal blocks that need row pivots on every other row, against the oracle
+(precon_ILU0.f90:111-265, mod_inv.f90:99-150).  n = 243 also takes the factor
+GEMMs' 64-edge tiles with padding (4 x 64 = 256)."""
 import os
 import subprocess
 import sys
@@ -30,7 +30,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 ctx = Context(0)
-for degree, nfields in ((0, 5), (1, 5), (2, 5), (1, 8), (3, 3)):   # n = 5, 40, 135, 64, 192
+for degree, nfields in ((0, 5), (1, 5), (2, 5), (1, 8), (3, 3), (2, 9)):   # n = 5, 40, 135, 64, 192, 243
     for pivots in (False, True):
         p, blocks = synthetic_system(3, 2, 2, degree, seed=41 + degree, nfields=nfields)
         n = p.n
