@@ -1023,13 +1023,14 @@ __device__ void ginv_pivot_body(T *__restrict__ M, T *__restrict__ S, int n, T *
 }
 
 // Wide-panel inverse with the pivot rows in registers: panels of H 16-column
-// halves (each half's Gauss-Jordan steps, then their transform on the other
-// halves in LDS), then every tile outside the panel takes the rank-16H update.
+// halves in LDS (each half's Gauss-Jordan steps, panel_gjw, then their
+// transform on the other halves on MFMA), then every tile outside the panel
+// takes the rank-16H update.
 // Each wave owns whole tile columns; before updating any tile of a column it
 // loads that column's pivot-row operands (16H rows, 4H values per lane) into
 // registers, so no other wave can have changed them (the column's tiles, its
 // pivot-row tiles among them, are only this wave's).  Fewer, wider panels
-// mean fewer passes over the slot, whose traffic sets the inverse's time;
+// mean fewer passes over the slot (each reads and writes the whole block);
 // with the pivot rows out of LDS, the halves alone take it.  The first panel
 // reads M and the last writes M (the slot S holds the passes between), so
 // neither a copy of M nor a copy back is needed; the result is inv(M) with
