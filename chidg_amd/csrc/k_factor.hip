@@ -1486,20 +1486,33 @@ __device__ __forceinline__ void store4(cdg_bf16 *dst, double a, double b, double
 
 // U_u = (S) A_{up_blk[u]}   (the apply's copy of A's upper blocks).  One
 // workgroup per block: 16-byte loads of two doubles, four values per thread and
-// step (slots start 512-byte aligned in A and 128-byte aligned in U).
+// step, four steps' loads before their stores (slots start 512-byte aligned in
+// A and 128-byte aligned in U).
 template <typename S>
 __global__ void __launch_bounds__(256) k_copy_upper(const double *__restrict__ Ab, int64_t lda, S *__restrict__ U,
                                                     int64_t ld, const int32_t *__restrict__ up_blk, int nup, int nn) {
+    const int B = blockDim.x;
     for (int u = blockIdx.x; u < nup; u += gridDim.x) {
-        const double *src = Ab + (size_t)up_blk[u] * lda;
+        const double2 *src = reinterpret_cast<const double2 *>(Ab + (size_t)up_blk[u] * lda);
         S *dst = U + (size_t)u * ld;
         const int n4 = nn / 4;
-        for (int q = threadIdx.x; q < n4; q += blockDim.x) {
-            const double2 a = reinterpret_cast<const double2 *>(src)[2 * q];
-            const double2 b = reinterpret_cast<const double2 *>(src)[2 * q + 1];
+        int q = threadIdx.x;
+        for (; q + 3 * B < n4; q += 4 * B) {
+            double2 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = src[2 * (q + i * B)];
+                b[i] = src[2 * (q + i * B) + 1];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) store4(dst + 4 * (q + i * B), a[i].x, a[i].y, b[i].x, b[i].y);
+        }
+        for (; q < n4; q += B) {
+            const double2 a = src[2 * q], b = src[2 * q + 1];
             store4(dst + 4 * q, a.x, a.y, b.x, b.y);
         }
-        for (int e = 4 * n4 + threadIdx.x; e < nn; e += blockDim.x) dst[e] = narrow<S>(src[e]);
+        const double *srcd = reinterpret_cast<const double *>(src);
+        for (int e = 4 * n4 + threadIdx.x; e < nn; e += B) dst[e] = narrow<S>(srcd[e]);
     }
 }
 
